@@ -1,0 +1,254 @@
+"""Benchmark of the MI355X collision path on the batched edge-validation workload.
+
+Workload (one step): 4,096 edges between uniform configurations of a
+Panda-class 7R arm with 60 spheres, among 16 mixed primitives, box motion
+validator at 0.05 rad (1-117 waypoints per edge), validated by one HIP-graph
+replay of the persistent edge kernel.  Inputs are resident in HBM.
+
+`value` counts collision-checked configurations per second: the waypoints the
+GPU evaluated (64-lane chunks up to each edge's first invalid chunk).  The
+same line carries edges/s, the other workloads' rates, the roofline figures
+of the edge kernel (live HIP events) and the CPU baseline (the FP64 oracle
+on this box's host cores, on a bounded sample).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def _soa(t, x):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x.T)).to("cuda")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--calib-copy-mb", type=int, default=0, help="run one device copy of this size (counter calibration)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every configuration, edge and trajectory is independent, so several GPUs "
+              "would each run this same kernel on a contiguous slice; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import plainmp_amd  # noqa: F401
+    from plainmp_amd import roofline, synth
+    from plainmp_amd.ompl import BoxMotionValidator
+    from plainmp_amd.synth import build_constraint as product_cst
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    # ------------------------------------------------ headline: batched edges
+    wl = synth.workload("edges_mixed16")
+    cst = product_cst(wl)
+    mv = BoxMotionValidator(cst, wl.box_width)
+    E = len(wl.S1)
+    with torch.cuda.stream(stream):
+        s1 = _soa(torch, wl.S1)
+        s2 = _soa(torch, wl.S2)
+        valid = torch.empty(E, dtype=torch.uint8, device=dev)
+        nchk = torch.empty(E, dtype=torch.int32, device=dev)
+        if args.calib_copy_mb:
+            a = torch.empty(args.calib_copy_mb * (1 << 20) // 4, dtype=torch.float32, device=dev)
+            b = torch.empty_like(a)
+            b.copy_(a)
+        graph = mv._dev_capture_graph(s1.data_ptr(), s2.data_ptr(), E, E, valid.data_ptr(), nchk.data_ptr(), sh)
+        for _ in range(args.warmup):
+            graph.launch()
+    stream.synchronize()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        ev0.record(stream)
+        for _ in range(args.steps):
+            graph.launch()
+        ev1.record(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    checked = int(nchk.sum().item())
+    ok = valid.cpu().numpy()
+    edges_per_s = E / (ms_step * 1e-3)
+    configs_per_s = checked / (ms_step * 1e-3)
+    n_wp_total = _total_waypoints(wl)
+    wp_flops = roofline.waypoint_flops(wl.robot, wl.prims)
+    achieved_tflops = checked * wp_flops / (ms_step * 1e-3) / 1e12
+
+    out = {
+        "metric": "collision-checked configs/sec + Jacobian evals/sec on 1 MI355X vs plain GPU & CPU",
+        "value": configs_per_s,
+        "unit": "configs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "wall_s_timed": wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 in-band re-decision, f64 edge step ratio and waypoint lerp)",
+        "data": "synthetic (seeded robot, scene and edges; see plainmp_amd/synth.py)",
+        "config": {
+            "workload": "edges_mixed16: Panda-class 7R, 60 spheres, 16 mixed primitives, 4096 edges, "
+                        "box validator 0.05 rad, one HIP-graph replay per step",
+            "edges": E,
+            "waypoints_total": n_wp_total,
+            "waypoints_checked": checked,
+            "edges_valid_frac": float(ok.astype(bool).mean()),
+        },
+        "edges_per_s": edges_per_s,
+        "covered_configs_per_s": n_wp_total / (ms_step * 1e-3),
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": roofline.PEAK_FP32_VALU_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / roofline.PEAK_FP32_VALU_TFLOPS,
+            "traffic": None,
+            "flops_per_waypoint": wp_flops,
+            "note": "FP32 VALU; packed FP32 is disabled in this build, so 78.6 TFLOP/s is the practical ceiling",
+        },
+    }
+
+    if not args.no_extra:
+        out["extra"] = _extra(torch, stream, dev)
+        out["plain_per_call_gpu"] = _plain_per_call(wl, mv)
+        out["speedup_vs_plain_per_call_edges"] = edges_per_s / out["plain_per_call_gpu"]["edges_per_s"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = _cpu_baseline(wl)
+        out["speedup_vs_cpu_configs"] = configs_per_s / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+def _total_waypoints(wl):
+    from plainmp_amd.synth import edge_waypoint_counts
+
+    return int(edge_waypoint_counts(wl.S1, wl.S2, wl.box_width).sum())
+
+
+def _time_stream(torch, stream, fn, steps=20, warmup=3):
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            fn()
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _extra(torch, stream, dev):
+    from plainmp_amd import roofline, synth
+    from plainmp_amd.synth import build_constraint as product_cst
+
+    res = {}
+    sh = stream.cuda_stream
+    # plain_two_bars: 1,024 configs, is_valid
+    wl = synth.workload("plain_two_bars")
+    cst = product_cst(wl)
+    B = len(wl.Q)
+    with torch.cuda.stream(stream):
+        q = _soa(torch, wl.Q)
+        valid = torch.empty(B, dtype=torch.uint8, device=dev)
+    ms = _time_stream(torch, stream, lambda: cst._dev_is_valid(q.data_ptr(), B, B, valid.data_ptr(), 0, sh))
+    res["plain_two_bars_is_valid"] = {"configs": B, "ms": ms, "configs_per_s": B / (ms * 1e-3),
+                                      "valid_frac": float(valid.float().mean().item())}
+    # grad_mixed32: 8,192 configs, every (sphere, primitive) distance + d/dq
+    wl = synth.workload("grad_mixed32")
+    cst = product_cst(wl)
+    B, S, P, D = len(wl.Q), cst.n_spheres(), cst.n_prims(), cst.q_dim()
+    with torch.cuda.stream(stream):
+        q = _soa(torch, wl.Q)
+        dist = torch.empty((S, P, B), dtype=torch.float32, device=dev)
+        grad = torch.empty((S, P, D, B), dtype=torch.float32, device=dev)
+    ms = _time_stream(torch, stream,
+                      lambda: cst._dev_pair_clearance(q.data_ptr(), B, B, dist.data_ptr(), grad.data_ptr(), sh))
+    bytes_out = (dist.numel() + grad.numel()) * 4 + q.numel() * 8
+    res["grad_mixed32_pair_jacobian"] = {
+        "configs": B, "ms": ms, "jacobians_per_s": B / (ms * 1e-3),
+        "hbm_gbs_algorithmic": bytes_out / (ms * 1e-3) / 1e9,
+        "frac_of_hbm_peak": bytes_out / (ms * 1e-3) / 1e9 / roofline.PEAK_HBM_GBS,
+    }
+    # reference-semantics evaluate (value + 1 Jacobian row) on the same batch
+    C = cst.cst_dim()
+    with torch.cuda.stream(stream):
+        vals = torch.empty((C, B), dtype=torch.float32, device=dev)
+        jac = torch.empty((C, D, B), dtype=torch.float32, device=dev)
+    ms = _time_stream(torch, stream, lambda: cst._dev_evaluate(q.data_ptr(), B, B, vals.data_ptr(), jac.data_ptr(), sh))
+    res["grad_mixed32_evaluate"] = {"configs": B, "ms": ms, "jacobians_per_s": B / (ms * 1e-3)}
+    # adjoint (VJP) used by the autograd backward
+    with torch.cuda.stream(stream):
+        gd = torch.ones((S, P, B), dtype=torch.float32, device=dev)
+        gq = torch.empty((D, B), dtype=torch.float32, device=dev)
+    ms = _time_stream(torch, stream,
+                      lambda: cst._dev_pair_clearance_vjp(q.data_ptr(), B, B, gd.data_ptr(), gq.data_ptr(), sh))
+    res["grad_mixed32_vjp"] = {"configs": B, "ms": ms, "vjps_per_s": B / (ms * 1e-3)}
+    return res
+
+
+def _plain_per_call(wl, mv, n=200):
+    """One edge per call: synchronous host->device copy, launch, device->host copy."""
+    t0 = time.perf_counter()
+    for e in range(n):
+        mv.checkMotion(wl.S1[e], wl.S2[e])
+    dt = time.perf_counter() - t0
+    return {"edges": n, "s": dt, "edges_per_s": n / dt}
+
+
+def _cpu_baseline(wl, budget_s=2.0):
+    """The FP64 oracle (a C++ restatement of the reference path) on host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from support import oracle_cst
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    oc = oracle_cst(wl)
+    E = len(wl.S1)
+    edges = checked = 0
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < budget_s:
+        sl = slice((k * 1024) % E, (k * 1024) % E + 1024)
+        ok, n = oc.check_motions(wl.S1[sl], wl.S2[sl], True, list(wl.box_width), 0.0, threads)
+        edges += len(ok)
+        checked += int(n.sum())
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": checked / dt, "unit": "configs/s", "cores": threads, "kind": "port",
+            "edges_per_s": edges / dt,
+            "sample": f"{edges} edges of edges_mixed16 ({checked} waypoints checked in reference order), "
+                      f"{dt:.1f} s wall on {threads} threads"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,237 @@
+/*
+ * plainmp_amd: C ABI of the MI355X (gfx950) FK + sphere-SDF collision path.
+ *
+ * The ABI is the drop-in boundary under plainmp's C++ classes: each batched
+ * entry point replaces a per-configuration reference method (cited next to
+ * it, paths relative to the plainmp source tree) with one call over a batch.
+ *
+ * Conventions
+ *   - Every function returns a pmp_status; PMP_OK is 0.  On failure a
+ *     thread-local message is available from pmp_last_error().
+ *     PMP_ERR_INVALID maps to std::invalid_argument / ValueError,
+ *     PMP_ERR_RUNTIME and PMP_ERR_HIP to std::runtime_error / RuntimeError
+ *     (the reference's error types, constraints/primitive_sphere_collision.cpp:171-174,
+ *     kinematics/kinematics.cpp:244,271).
+ *   - Descriptor structs (pmp_robot_desc, pmp_scene_desc) hold HOST pointers;
+ *     the tables are copied to device memory owned by the context.
+ *   - Batch inputs and outputs are DEVICE pointers owned by the caller.
+ *     Configurations are FP64 structure-of-arrays: q[col * ldq + b] for
+ *     column col of configuration b (ldq >= B).  Outputs use the same
+ *     "batch index fastest" layout with leading dimension B.
+ *   - Work is enqueued on the context's stream (pmp_set_stream); calls do
+ *     not synchronise the host unless documented.
+ *   - A context is not re-entrant: use one context per host thread.
+ */
+#ifndef PLAINMP_AMD_H_
+#define PLAINMP_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pmp_status;
+#define PMP_OK 0
+#define PMP_ERR_INVALID 1
+#define PMP_ERR_RUNTIME 2
+#define PMP_ERR_HIP 3
+
+/* Kinematic node kinds (kinematics/algorithm.cpp:109-158, 160-179). */
+#define PMP_NODE_ROOT 0        /* the base link; pose = base pose            */
+#define PMP_NODE_FIXED 1       /* static local transform                     */
+#define PMP_NODE_REVOLUTE 2    /* revolute or continuous joint driven by q    */
+#define PMP_NODE_PRISMATIC 3   /* prismatic joint driven by q                 */
+#define PMP_NODE_TRANSLATION 4 /* consider_rotation = false link (position only) */
+
+/* Base types (kinematics/kinematics.hpp:49). */
+#define PMP_BASE_FIXED 0
+#define PMP_BASE_PLANAR 1
+#define PMP_BASE_FLOATING 2
+
+/* Primitive types (collision/primitive_sdf.hpp:77). */
+#define PMP_PRIM_BOX 1
+#define PMP_PRIM_CYLINDER 2
+#define PMP_PRIM_SPHERE 3
+#define PMP_PRIM_GROUND 4
+#define PMP_PRIM_CLOUD 5
+
+/* Motion validator types (ompl/ompl_thin_wrap.hpp:50-59). */
+#define PMP_VALIDATOR_BOX 0
+#define PMP_VALIDATOR_EUCLIDEAN 1
+
+/*
+ * Kinematic tree + sphere tables of a SphereCollisionCst.
+ * Nodes are in evaluation order: every node's parent precedes it.  Node 0
+ * must be the root.  Replaces the state that KinematicModel
+ * (kinematics/kinematics.hpp:51-169) and SphereCollisionCst
+ * (constraints/primitive_sphere_collision.hpp:55-116) build in their ctors.
+ */
+typedef struct pmp_robot_desc {
+  int32_t n_nodes;
+  const int32_t* node_parent;        /* [n_nodes] parent node, -1 for the root          */
+  const int32_t* node_kind;          /* [n_nodes] PMP_NODE_*                           */
+  const int32_t* node_q_col;         /* [n_nodes] column of q for joints, else -1       */
+  const double* node_quat;           /* [n_nodes*4] local rotation x,y,z,w              */
+  const double* node_trans;          /* [n_nodes*3] local translation                  */
+  const double* node_axis;           /* [n_nodes*3] joint axis (not normalised)        */
+  const uint8_t* node_identity;      /* [n_nodes] local rotation skipped when composing */
+  int32_t n_q_cols;                  /* joint columns of q (= controlled joints)       */
+  int32_t base_type;                 /* PMP_BASE_*; PLANAR appends (x, y, yaw) and
+                                        FLOATING (x, y, z, roll, pitch, yaw) columns   */
+  double base_pose[7];               /* x, y, z, qx, qy, qz, qw when base_type is FIXED */
+  int32_t n_spheres;                 /* in SphereCollisionCst group order              */
+  const int32_t* sphere_node;        /* [n_spheres] node the sphere is attached to      */
+  const double* sphere_local;        /* [n_spheres*3] position in the node frame        */
+  const double* sphere_radius;       /* [n_spheres]                                    */
+  const int32_t* sphere_group;       /* [n_spheres] group index                         */
+  int32_t n_groups;
+  const uint8_t* group_only_self;    /* [n_groups] 1 = skipped by external checks      */
+  int32_t n_self_pairs;              /* group pairs checked for self collision         */
+  const int32_t* self_pairs;         /* [n_self_pairs*2] group indices                 */
+} pmp_robot_desc;
+
+/*
+ * One primitive of the flattened SDF list (all_sdfs_cache_,
+ * constraints/primitive_sphere_collision.cpp:426-448).  Pose, flags and AABB
+ * are given as the reference's Pose / update_aabb compute them
+ * (collision/primitive_sdf.hpp:30-75, 250-255, 400-421, 500-521, 558-561).
+ */
+typedef struct pmp_prim_desc {
+  int32_t type;             /* PMP_PRIM_*                                            */
+  uint8_t axis_aligned;
+  uint8_t z_axis_aligned;
+  double position[3];
+  double rotation[9];       /* row-major, world <- local                            */
+  double rotation_inv[9];   /* row-major general inverse of rotation                 */
+  double size[3];           /* box: width; cylinder: radius, height; sphere: radius;
+                               ground: height; cloud: point radius                   */
+  double lb[3], ub[3];      /* axis-aligned bounds                                   */
+  const double* points;     /* cloud only: [n_points*3] xyz                          */
+  int64_t n_points;
+} pmp_prim_desc;
+
+typedef struct pmp_scene_desc {
+  int32_t n_prims;
+  const pmp_prim_desc* prims;
+} pmp_scene_desc;
+
+/* Box / euclidean motion validator (ompl/motion_validator.hpp:42-63). */
+typedef struct pmp_validator_desc {
+  int32_t type;             /* PMP_VALIDATOR_*                                      */
+  double resolution;        /* EUCLIDEAN                                            */
+  const double* box_width;  /* BOX: [n_q_cols] host pointer                         */
+} pmp_validator_desc;
+
+/* Launch schedule of the persistent kernels, chosen by the autotuner. */
+typedef struct pmp_schedule {
+  int32_t block;            /* threads per workgroup (multiple of 64), 0 = default  */
+  int32_t waves_per_cu;     /* persistent waves per compute unit, 0 = default       */
+  int32_t lanes_per_item;   /* lanes that share one edge: 16, 32 or 64; 0 = default */
+} pmp_schedule;
+
+typedef struct pmp_context pmp_context;
+
+const char* pmp_last_error(void);
+int32_t pmp_abi_version(void);
+
+pmp_status pmp_create(int32_t device, pmp_context** out);
+pmp_status pmp_destroy(pmp_context* ctx);
+/* hip_stream is a hipStream_t (NULL = default stream). */
+pmp_status pmp_set_stream(pmp_context* ctx, void* hip_stream);
+pmp_status pmp_synchronize(pmp_context* ctx);
+
+/* Tables.  Replace KinematicModel(xml) + SphereCollisionCst ctor + set_sdf
+ * (kinematics/kinematics.cpp:183-223, constraints/primitive_sphere_collision.cpp:117-179,
+ *  constraints/primitive_sphere_collision.hpp:73-76). */
+pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* robot);
+pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* scene);
+pmp_status pmp_set_band(pmp_context* ctx, double band_m);      /* FP64 re-decision band, default 1e-5 m */
+pmp_status pmp_set_schedule(pmp_context* ctx, const pmp_schedule* schedule);
+
+/* World centres of all spheres, centers[(s*3 + k) * B + b] (FP32).
+ * Replaces update_kintree + get_all_spheres
+ * (constraints/primitive.hpp:38-62, constraints/primitive_sphere_collision.cpp:412-424).
+ * accurate = 0 uses the Taylor sin/cos of is_valid, 1 exact sin/cos. */
+pmp_status pmp_fk_spheres_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
+                                float* centers);
+
+/* Link poses of all nodes, poses[(n*7 + k) * B + b] as x,y,z,qx,qy,qz,qw
+ * (FP64).  Replaces set_joint_angles + get_link_pose
+ * (kinematics/algorithm.cpp:88-107, kinematics/kinematics.hpp:131-136). */
+pmp_status pmp_link_poses_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
+                                double* poses);
+
+/* Per-sphere minimum signed clearance over all primitives,
+ * clearance[s * B + b] = min_p (sdf_p(c_s) - r_s) (FP32), FK mode as above. */
+pmp_status pmp_sphere_clearance_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B,
+                                      int32_t accurate, float* clearance);
+
+/* Validity of each configuration: valid[b] in {0, 1}.
+ * Replaces IneqConstraintBase::is_valid -> SphereCollisionCst::is_valid_dirty
+ * (constraints/primitive.hpp:113-117, constraints/primitive_sphere_collision.cpp:181-254).
+ * FP32 with every decision within the band re-decided in FP64 on the GPU.
+ * n_recheck (device, optional) accumulates the number of re-decided configurations. */
+pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, uint8_t* valid,
+                              int32_t* n_recheck);
+
+/* Reference-semantics value and Jacobian rows, vals[k * B + b],
+ * jac[(k * q_dim + j) * B + b] for k < cst_dim (ext first, then self).
+ * Replaces ConstraintBase::evaluate -> SphereCollisionCst::evaluate_dirty
+ * (constraints/primitive.hpp:74-79, constraints/primitive_sphere_collision.cpp:256-399). */
+pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, float* vals,
+                              float* jac);
+
+/* Signed distance of every (sphere, primitive) pair with exact FK and its
+ * derivative with respect to q: dist[(s*P + p) * B + b],
+ * grad[((s*P + p) * q_dim + j) * B + b].  grad may be NULL.  No reference
+ * counterpart as one call: it batches the per-pair terms that
+ * evaluate_ext_collision reduces (constraints/primitive_sphere_collision.cpp:256-322). */
+pmp_status pmp_pair_clearance_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, float* dist,
+                                    float* grad);
+
+/* Adjoint of pmp_pair_clearance_batch: grad_q[j * B + b] =
+ * sum_{s,p} grad_dist[(s*P + p) * B + b] * d dist[s,p,b] / d q_j. */
+pmp_status pmp_pair_clearance_vjp(pmp_context* ctx, const double* q, int64_t ldq, int64_t B,
+                                  const float* grad_dist, float* grad_q);
+
+/* Edge validity: valid[e] for the motion s1[e] -> s2[e] (columns as q,
+ * leading dimension lds).  Replaces CustomValidatorBase::checkMotion with the
+ * box or euclidean step ratio (ompl/motion_validator.cpp:17-126).
+ * n_checked (device, optional): waypoints evaluated per edge. */
+pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
+                                 const pmp_validator_desc* validator, uint8_t* valid, int32_t* n_checked);
+
+/* Same as pmp_check_edges_batch, captured once into a HIP graph and replayed
+ * by pmp_graph_launch on the context stream.  The pointers must stay valid
+ * while the graph exists. */
+typedef struct pmp_graph pmp_graph;
+pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
+                                 const pmp_validator_desc* validator, uint8_t* valid, int32_t* n_checked,
+                                 pmp_graph** out);
+pmp_status pmp_graph_launch(pmp_graph* graph);
+pmp_status pmp_graph_destroy(pmp_graph* graph);
+
+/* Primitive SDF of points (FP64): out[i] = min over the given primitives of
+ * the signed distance at points[k * N + i] (mode 0), or is_outside(p, radius)
+ * of the union as 0/1 (mode 1).  Replaces SDFBase::evaluate / evaluate_batch /
+ * is_outside (collision/primitive_sdf.hpp:83-97, 147-170). */
+pmp_status pmp_prims_evaluate(pmp_context* ctx, const pmp_scene_desc* prims, const double* points, int64_t N,
+                              int32_t mode, double radius, double* out);
+
+/* Exact nearest neighbour by brute force (FP64): for each query
+ * queries[k * M + i], sqdist[i] and index[i] of the nearest of the N points
+ * points[k * N + n].  Replaces KDTree::query / sqdist (collision/kdtree.cpp:26-38). */
+pmp_status pmp_nearest_batch(pmp_context* ctx, const double* points, int64_t N, const double* queries, int64_t M,
+                             double* sqdist, int32_t* index);
+
+/* Number of spheres, primitives, q columns and constraint rows of the
+ * uploaded tables (host values). */
+pmp_status pmp_query_sizes(pmp_context* ctx, int32_t* n_spheres, int32_t* n_prims, int32_t* q_dim,
+                           int32_t* cst_dim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLAINMP_AMD_H_ */

@@ -1,0 +1,777 @@
+// C ABI of plainmp_amd (declared in include/plainmp_amd.h): context,
+// table packing and launches.  Host code compiled by hipcc.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "launch.h"
+
+using namespace pmp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+  pmp_status code;
+  Error(pmp_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PMP_HIP(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) throw Error(PMP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define PMP_REQUIRE(cond, msg)                          \
+  do {                                                  \
+    if (!(cond)) throw Error(PMP_ERR_INVALID, (msg));   \
+  } while (0)
+
+template <class F>
+pmp_status guarded(F&& f) {
+  try {
+    f();
+    return PMP_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return PMP_ERR_RUNTIME;
+  }
+}
+
+// Device buffer that grows on demand and is freed with the context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* get(size_t n) {
+    if (n > bytes) {
+      if (ptr) PMP_HIP(hipFree(ptr));
+      ptr = nullptr;
+      PMP_HIP(hipMalloc(&ptr, n));
+      bytes = n;
+    }
+    return ptr;
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v, hipStream_t st) {
+    T* p = (T*)get(std::max<size_t>(v.size() * sizeof(T), 16));
+    if (!v.empty()) PMP_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return p;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+// Farthest-first visiting order of the motion validators
+// (ompl/generate_sequence.py:11-63): element i+1 maximises the minimum
+// distance to the elements already chosen, ties to the larger value.
+std::vector<uint8_t> sequence_table() {
+  std::vector<uint8_t> t(128 * 128, 0);
+  for (int n = 1; n <= 128; ++n) {
+    std::vector<int> seq{0};
+    std::vector<bool> used(n, false);
+    used[0] = true;
+    for (int i = 1; i < n; ++i) {
+      int best = -1, best_d = -1;
+      for (int v = 0; v < n; ++v) {
+        if (used[v]) continue;
+        int d = n;
+        for (int s : seq) d = std::min(d, std::abs(v - s));
+        if (d > best_d || (d == best_d && v > best)) {
+          best_d = d;
+          best = v;
+        }
+      }
+      seq.push_back(best);
+      used[best] = true;
+    }
+    for (int i = 0; i < n; ++i) t[(n - 1) * 128 + i] = (uint8_t)seq[i];
+  }
+  return t;
+}
+
+}  // namespace
+
+struct pmp_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t capture_stream = nullptr;  // used for graph capture when no stream is set
+  float band = 1e-5f;
+  pmp_schedule sched{0, 0, 0};
+  int n_cu = 256;
+  // robot
+  bool has_robot = false;
+  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs;
+  KinArgs<float> k32{};
+  KinArgs<double> k64{};
+  int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
+  // scene
+  DevBuf prims32, prims64, cloud32, cloud64;
+  SceneArgs<float> s32{};
+  SceneArgs<double> s64{};
+  int n_clouds = 0;
+  // misc
+  DevBuf seq, counter, width, inv_width, tmp_prims, tmp_cloud;
+};
+
+struct pmp_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+
+void set_device(pmp_context* ctx) { PMP_HIP(hipSetDevice(ctx->device)); }
+hipStream_t st(pmp_context* ctx) { return ctx->stream; }
+
+template <class T>
+DevPrim<T> make_prim(const pmp_prim_desc& d, int cloud_begin) {
+  DevPrim<T> p{};
+  p.type = d.type;
+  p.flags = (d.axis_aligned ? 1 : 0) | (d.z_axis_aligned ? 2 : 0);
+  p.cloud_begin = cloud_begin;
+  p.cloud_count = d.type == PMP_PRIM_CLOUD ? (int32_t)d.n_points : 0;
+  for (int k = 0; k < 3; ++k) {
+    p.pos[k] = (T)d.position[k];
+    p.lb[k] = (T)d.lb[k];
+    p.ub[k] = (T)d.ub[k];
+  }
+  for (int k = 0; k < 9; ++k) {
+    p.R[k] = (T)d.rotation[k];
+    p.Rinv[k] = (T)d.rotation_inv[k];
+  }
+  switch (d.type) {
+    case PMP_PRIM_BOX:  // half_width_ = 0.5 * width (primitive_sdf.hpp:283)
+      p.p0 = (T)(0.5 * d.size[0]);
+      p.p1 = (T)(0.5 * d.size[1]);
+      p.p2 = (T)(0.5 * d.size[2]);
+      break;
+    case PMP_PRIM_CYLINDER:  // r, half height, r^2 (:431-437)
+      p.p0 = (T)d.size[0];
+      p.p1 = (T)(0.5 * d.size[1]);
+      p.p2 = (T)(d.size[0] * d.size[0]);
+      break;
+    case PMP_PRIM_SPHERE:  // r, r^2 (:533-537)
+      p.p0 = (T)d.size[0];
+      p.p1 = (T)(d.size[0] * d.size[0]);
+      break;
+    case PMP_PRIM_GROUND:
+    case PMP_PRIM_CLOUD:
+      p.p0 = (T)d.size[0];
+      break;
+  }
+  return p;
+}
+
+void check_prim(const pmp_prim_desc& d) {
+  PMP_REQUIRE(d.type >= PMP_PRIM_BOX && d.type <= PMP_PRIM_CLOUD, "unknown primitive type");
+  if (d.type == PMP_PRIM_CLOUD) PMP_REQUIRE(d.points != nullptr && d.n_points > 0, "cloud primitive without points");
+}
+
+// Pack a primitive list into FP32 / FP64 device records and cloud tables.
+void pack_prims(const pmp_scene_desc& sc, std::vector<DevPrim<float>>& p32, std::vector<DevPrim<double>>& p64,
+                std::vector<float>& c32, std::vector<double>& c64, int& n_clouds) {
+  n_clouds = 0;
+  int64_t cloud_at = 0;
+  for (int i = 0; i < sc.n_prims; ++i) {
+    const pmp_prim_desc& d = sc.prims[i];
+    check_prim(d);
+    p32.push_back(make_prim<float>(d, (int)cloud_at));
+    p64.push_back(make_prim<double>(d, (int)cloud_at));
+    if (d.type == PMP_PRIM_CLOUD) {
+      ++n_clouds;
+      PMP_REQUIRE(cloud_at + d.n_points < (int64_t)1 << 30, "cloud too large");
+      for (int64_t j = 0; j < d.n_points; ++j) {
+        const double* y = d.points + 3 * j;
+        double n2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+        for (int k = 0; k < 3; ++k) {
+          c32.push_back((float)y[k]);
+          c64.push_back(y[k]);
+        }
+        c32.push_back((float)n2);
+        c64.push_back(n2);
+      }
+      cloud_at += d.n_points;
+    }
+  }
+}
+
+void require_robot(pmp_context* ctx) {
+  PMP_REQUIRE(ctx != nullptr, "null context");
+  PMP_REQUIRE(ctx->has_robot, "no robot tables uploaded (call pmp_upload_robot first)");
+}
+void require_no_cloud(pmp_context* ctx, const char* what) {
+  if (ctx->n_clouds > 0)
+    throw Error(PMP_ERR_INVALID, std::string(what) + ": point-cloud primitives are served by the cloud entry points");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pmp_last_error(void) { return g_last_error.c_str(); }
+int32_t pmp_abi_version(void) { return 1; }
+
+pmp_status pmp_create(int32_t device, pmp_context** out) {
+  return guarded([&] {
+    PMP_REQUIRE(out != nullptr, "null output pointer");
+    int n = 0;
+    PMP_HIP(hipGetDeviceCount(&n));
+    PMP_REQUIRE(device >= 0 && device < n, "device index out of range");
+    auto* ctx = new pmp_context();
+    ctx->device = device;
+    try {
+      PMP_HIP(hipSetDevice(device));
+      hipDeviceProp_t prop;
+      PMP_HIP(hipGetDeviceProperties(&prop, device));
+      ctx->n_cu = prop.multiProcessorCount;
+      ctx->seq.upload(sequence_table(), nullptr);
+      ctx->counter.get(64);
+      PMP_HIP(hipDeviceSynchronize());
+    } catch (...) {
+      delete ctx;
+      throw;
+    }
+    *out = ctx;
+  });
+}
+
+pmp_status pmp_destroy(pmp_context* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    (void)hipDeviceSynchronize();
+    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->prims32,
+                      &ctx->prims64, &ctx->cloud32, &ctx->cloud64, &ctx->seq, &ctx->counter, &ctx->width,
+                      &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud})
+      b->release();
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    delete ctx;
+  });
+}
+
+pmp_status pmp_set_stream(pmp_context* ctx, void* stream) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr, "null context");
+    ctx->stream = (hipStream_t)stream;
+  });
+}
+
+pmp_status pmp_synchronize(pmp_context* ctx) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr, "null context");
+    set_device(ctx);
+    PMP_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+pmp_status pmp_set_band(pmp_context* ctx, double band) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr, "null context");
+    PMP_REQUIRE(band >= 0.0 && band < 1.0, "band must be in [0, 1) m");
+    ctx->band = (float)band;
+  });
+}
+
+pmp_status pmp_set_schedule(pmp_context* ctx, const pmp_schedule* s) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr && s != nullptr, "null argument");
+    PMP_REQUIRE(s->block == 0 || (s->block % 64 == 0 && s->block <= 1024), "block must be a multiple of 64, <= 1024");
+    PMP_REQUIRE(s->waves_per_cu >= 0 && s->waves_per_cu <= 32, "waves_per_cu must be in [0, 32]");
+    PMP_REQUIRE(s->lanes_per_item == 0 || s->lanes_per_item == 64, "lanes_per_item must be 64");
+    ctx->sched = *s;
+  });
+}
+
+pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr && r != nullptr, "null argument");
+    const int N = r->n_nodes;
+    PMP_REQUIRE(N >= 1 && N <= kMaxNodes, "n_nodes out of range");
+    PMP_REQUIRE(r->node_parent && r->node_kind && r->node_q_col && r->node_quat && r->node_trans && r->node_axis &&
+                    r->node_identity,
+                "missing node table");
+    PMP_REQUIRE(r->node_parent[0] == -1 && r->node_kind[0] == PMP_NODE_ROOT, "node 0 must be the root");
+    PMP_REQUIRE(r->base_type >= PMP_BASE_FIXED && r->base_type <= PMP_BASE_FLOATING, "unknown base type");
+    const int n_q = r->n_q_cols;
+    const int n_cols = n_q + (r->base_type == PMP_BASE_PLANAR ? 3 : 0) + (r->base_type == PMP_BASE_FLOATING ? 6 : 0);
+    PMP_REQUIRE(n_q >= 0 && n_cols <= kMaxCols, "too many q columns");
+    const int S = r->n_spheres, G = r->n_groups;
+    PMP_REQUIRE(S >= 0 && S <= kMaxSpheres, "n_spheres out of range");
+    PMP_REQUIRE(G >= 0 && (S == 0 || G >= 1), "n_groups out of range");
+    PMP_REQUIRE(r->n_self_pairs >= 0 && r->n_self_pairs <= kMaxSelfPairs, "n_self_pairs out of range");
+    std::vector<std::vector<int>> children(N);
+    for (int i = 1; i < N; ++i) {
+      const int p = r->node_parent[i];
+      PMP_REQUIRE(p >= 0 && p < i, "node parents must precede their children");
+      children[p].push_back(i);
+      const int k = r->node_kind[i];
+      PMP_REQUIRE(k >= PMP_NODE_FIXED && k <= PMP_NODE_TRANSLATION, "unknown node kind");
+      if (k == PMP_NODE_REVOLUTE || k == PMP_NODE_PRISMATIC)
+        PMP_REQUIRE(r->node_q_col[i] >= 0 && r->node_q_col[i] < n_q, "joint node without a valid q column");
+    }
+    for (int s = 0; s < S; ++s) {
+      PMP_REQUIRE(r->sphere_node[s] >= 0 && r->sphere_node[s] < N, "sphere attached to an unknown node");
+      PMP_REQUIRE(r->sphere_group[s] >= 0 && r->sphere_group[s] < G, "sphere group out of range");
+      PMP_REQUIRE(s == 0 || r->sphere_group[s] >= r->sphere_group[s - 1], "spheres must be in group order");
+    }
+    // nodes carrying a self-pair group keep their pose to the end
+    std::vector<int> group_node(G, -1);
+    for (int s = 0; s < S; ++s) {
+      const int g = r->sphere_group[s];
+      PMP_REQUIRE(group_node[g] < 0 || group_node[g] == r->sphere_node[s], "a group spans several nodes");
+      group_node[g] = r->sphere_node[s];
+    }
+    std::vector<bool> keep_to_end(N, false);
+    for (int k = 0; k < r->n_self_pairs; ++k)
+      for (int e = 0; e < 2; ++e) {
+        const int g = r->self_pairs[2 * k + e];
+        PMP_REQUIRE(g >= 0 && g < G && group_node[g] >= 0, "self pair names an empty or unknown group");
+        keep_to_end[group_node[g]] = true;
+      }
+    // register slots: a node keeps its pose if a later child is not the next
+    // node or if the self pass needs it
+    std::vector<int> slot(N, -1), last_child(N, -1);
+    for (int i = 0; i < N; ++i)
+      for (int c : children[i]) last_child[i] = std::max(last_child[i], c);
+    std::vector<bool> used(kMaxSlots, false);
+    std::vector<int> parent_slot(N, -1);
+    for (int i = 0; i < N; ++i) {
+      const int p = i == 0 ? -1 : r->node_parent[i];
+      if (p <= 0)
+        parent_slot[i] = -1;  // the root's pose is the base pose
+      else if (p == i - 1)
+        parent_slot[i] = -2;
+      else {
+        PMP_REQUIRE(slot[p] >= 0, "internal: parent pose not kept");
+        parent_slot[i] = slot[p];
+      }
+      // release parents whose last child is this node
+      if (p > 0 && last_child[p] == i && slot[p] >= 0 && !keep_to_end[p]) used[slot[p]] = false;
+      bool need = keep_to_end[i];
+      for (int c : children[i])
+        if (c != i + 1) need = true;
+      if (i == 0) need = false;
+      if (need) {
+        int f = -1;
+        for (int k = 0; k < kMaxSlots; ++k)
+          if (!used[k]) {
+            f = k;
+            break;
+          }
+        PMP_REQUIRE(f >= 0, "kinematic tree too branched: more than 8 live poses");
+        used[f] = true;
+        slot[i] = f;
+      }
+    }
+    // q columns that move each node (reference rptable_), base columns move all
+    std::vector<uint32_t> mask(N, 0);
+    uint32_t base_bits = 0;
+    for (int c = n_q; c < n_cols; ++c) base_bits |= 1u << c;
+    uint32_t rev = 0;
+    if (r->base_type == PMP_BASE_PLANAR) rev |= 1u << (n_q + 2);
+    if (r->base_type == PMP_BASE_FLOATING) rev |= (1u << (n_q + 3)) | (1u << (n_q + 4)) | (1u << (n_q + 5));
+    for (int i = 0; i < N; ++i) {
+      uint32_t m = base_bits;
+      for (int a = i; a > 0; a = r->node_parent[a]) {
+        const int k = r->node_kind[a];
+        if (k == PMP_NODE_REVOLUTE || k == PMP_NODE_PRISMATIC) m |= 1u << r->node_q_col[a];
+      }
+      mask[i] = m;
+      if (r->node_kind[i] == PMP_NODE_REVOLUTE) rev |= 1u << r->node_q_col[i];
+    }
+    // spheres sorted by node (stable: groups stay contiguous and in order)
+    std::vector<int> order(S);
+    for (int s = 0; s < S; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return r->sphere_node[a] < r->sphere_node[b]; });
+    std::vector<int> pos_of(S);
+    for (int k = 0; k < S; ++k) pos_of[order[k]] = k;
+    std::vector<DevSphere<float>> sp32(S);
+    std::vector<DevSphere<double>> sp64(S);
+    std::vector<DevSphereMeta> meta(S);
+    int n_ext = 0;
+    for (int k = 0; k < S; ++k) {
+      const int s = order[k];
+      const double* l = r->sphere_local + 3 * s;
+      sp32[k] = {(float)l[0], (float)l[1], (float)l[2], (float)r->sphere_radius[s]};
+      sp64[k] = {l[0], l[1], l[2], r->sphere_radius[s]};
+      const int g = r->sphere_group[s];
+      const bool ext = !(r->group_only_self && r->group_only_self[g]);
+      n_ext += ext;
+      meta[k] = {s, ext ? 1 : 0, r->sphere_node[s], g};
+    }
+    std::vector<DevNode<float>> n32(N);
+    std::vector<DevNode<double>> n64(N);
+    for (int i = 0; i < N; ++i) {
+      int b = S, e = S;
+      for (int k = 0; k < S; ++k)
+        if (meta[k].node == i) {
+          b = std::min(b, k);
+          e = k + 1;
+        }
+      if (b == S) b = e = 0;
+      auto fill = [&](auto& nd) {
+        using T = std::remove_reference_t<decltype(nd.quat[0])>;
+        nd.kind = r->node_kind[i];
+        nd.parent_slot = parent_slot[i];
+        nd.out_slot = slot[i];
+        nd.q_col = (nd.kind == PMP_NODE_REVOLUTE || nd.kind == PMP_NODE_PRISMATIC) ? r->node_q_col[i] : -1;
+        nd.sphere_begin = b;
+        nd.sphere_end = e;
+        nd.identity = r->node_identity[i] ? 1 : 0;
+        nd.col_mask = mask[i];
+        for (int k = 0; k < 4; ++k) nd.quat[k] = (T)r->node_quat[4 * i + k];
+        for (int k = 0; k < 3; ++k) {
+          nd.trans[k] = (T)r->node_trans[3 * i + k];
+          nd.axis[k] = (T)r->node_axis[3 * i + k];
+        }
+      };
+      fill(n32[i]);
+      fill(n64[i]);
+    }
+    std::vector<DevSelfPair> pairs;
+    for (int k = 0; k < r->n_self_pairs; ++k) {
+      int g1 = r->self_pairs[2 * k], g2 = r->self_pairs[2 * k + 1];
+      auto range = [&](int g, int& b, int& e) {
+        b = S;
+        e = 0;
+        for (int s = 0; s < S; ++s)
+          if (r->sphere_group[s] == g) {
+            b = std::min(b, pos_of[s]);
+            e = std::max(e, pos_of[s] + 1);
+          }
+      };
+      DevSelfPair pr{};
+      range(g1, pr.begin1, pr.end1);
+      range(g2, pr.begin2, pr.end2);
+      pr.node1 = group_node[g1];
+      pr.node2 = group_node[g2];
+      pr.slot1 = slot[pr.node1];
+      pr.slot2 = slot[pr.node2];
+      PMP_REQUIRE(pr.slot1 >= 0 && pr.slot2 >= 0, "internal: self-pair pose not kept");
+      pairs.push_back(pr);
+    }
+    set_device(ctx);
+    hipStream_t s0 = st(ctx);
+    KinArgs<float> a{};
+    KinArgs<double> d{};
+    a.nodes = ctx->nodes32.upload(n32, s0);
+    d.nodes = ctx->nodes64.upload(n64, s0);
+    a.spheres = ctx->sph32.upload(sp32, s0);
+    d.spheres = ctx->sph64.upload(sp64, s0);
+    a.meta = d.meta = ctx->meta.upload(meta, s0);
+    a.pairs = d.pairs = ctx->pairs.upload(pairs, s0);
+    a.n_nodes = d.n_nodes = N;
+    a.n_spheres = d.n_spheres = S;
+    a.n_pairs = d.n_pairs = (int)pairs.size();
+    a.n_cols = d.n_cols = n_cols;
+    a.n_q_cols = d.n_q_cols = n_q;
+    a.base_type = d.base_type = r->base_type;
+    a.n_ext_spheres = d.n_ext_spheres = n_ext;
+    a.rev_mask = d.rev_mask = rev;
+    for (int k = 0; k < 3; ++k) {
+      a.base_t[k] = (float)r->base_pose[k];
+      d.base_t[k] = r->base_pose[k];
+    }
+    for (int k = 0; k < 4; ++k) {
+      a.base_q[k] = (float)r->base_pose[3 + k];
+      d.base_q[k] = r->base_pose[3 + k];
+    }
+    PMP_HIP(hipStreamSynchronize(s0));
+    ctx->k32 = a;
+    ctx->k64 = d;
+    ctx->n_spheres = S;
+    ctx->n_groups = G;
+    ctx->n_cols = n_cols;
+    ctx->n_nodes = N;
+    ctx->has_robot = true;
+  });
+}
+
+pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr && sc != nullptr, "null argument");
+    PMP_REQUIRE(sc->n_prims >= 0 && sc->n_prims <= kMaxPrims, "n_prims out of range");
+    PMP_REQUIRE(sc->n_prims == 0 || sc->prims != nullptr, "missing primitive list");
+    std::vector<DevPrim<float>> p32;
+    std::vector<DevPrim<double>> p64;
+    std::vector<float> c32;
+    std::vector<double> c64;
+    int n_clouds = 0;
+    pack_prims(*sc, p32, p64, c32, c64, n_clouds);
+    set_device(ctx);
+    hipStream_t s0 = st(ctx);
+    SceneArgs<float> a{};
+    SceneArgs<double> d{};
+    a.prims = ctx->prims32.upload(p32, s0);
+    d.prims = ctx->prims64.upload(p64, s0);
+    a.cloud_xyz = ctx->cloud32.upload(c32, s0);
+    d.cloud_xyz = ctx->cloud64.upload(c64, s0);
+    a.n_prims = d.n_prims = sc->n_prims;
+    a.n_cloud = d.n_cloud = n_clouds;
+    PMP_HIP(hipStreamSynchronize(s0));
+    ctx->s32 = a;
+    ctx->s64 = d;
+    ctx->n_clouds = n_clouds;
+  });
+}
+
+pmp_status pmp_query_sizes(pmp_context* ctx, int32_t* n_spheres, int32_t* n_prims, int32_t* q_dim, int32_t* cst_dim) {
+  return guarded([&] {
+    require_robot(ctx);
+    if (n_spheres) *n_spheres = ctx->n_spheres;
+    if (n_prims) *n_prims = ctx->s32.n_prims;
+    if (q_dim) *q_dim = ctx->n_cols;
+    if (cst_dim) *cst_dim = (ctx->s32.n_prims > 0 ? 1 : 0) + (ctx->k32.n_pairs > 0 ? 1 : 0);
+  });
+}
+
+#define PMP_BATCH_CHECKS(q, ldq, B)                                       \
+  require_robot(ctx);                                                     \
+  PMP_REQUIRE((B) >= 0, "negative batch size");                           \
+  PMP_REQUIRE((B) == 0 || (q) != nullptr, "null configuration pointer");  \
+  PMP_REQUIRE((ldq) >= (B), "leading dimension smaller than the batch")
+
+pmp_status pmp_fk_spheres_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
+                                float* centers) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || centers, "null output");
+    if (B == 0 || ctx->n_spheres == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_centers(ctx->k32, q, ldq, B, accurate ? 0 : 1, centers, st(ctx)));
+  });
+}
+
+pmp_status pmp_link_poses_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
+                                double* poses) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || poses, "null output");
+    if (B == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_poses(ctx->k64, q, ldq, B, accurate ? 0 : 1, poses, st(ctx)));
+  });
+}
+
+pmp_status pmp_sphere_clearance_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
+                                      float* clearance) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || clearance, "null output");
+    require_no_cloud(ctx, "pmp_sphere_clearance_batch");
+    if (B == 0 || ctx->n_spheres == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_clearance(ctx->k32, ctx->s32, q, ldq, B, accurate ? 0 : 1, clearance, st(ctx)));
+  });
+}
+
+pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, uint8_t* valid,
+                              int32_t* n_recheck) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || valid, "null output");
+    require_no_cloud(ctx, "pmp_is_valid_batch");
+    if (B == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_valid(ctx->k32, ctx->k64, ctx->s32, ctx->s64, q, ldq, B, ctx->band, valid, n_recheck,
+                         ctx->sched.block, st(ctx)));
+  });
+}
+
+pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, float* vals, float* jac) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || (vals && jac), "null output");
+    require_no_cloud(ctx, "pmp_evaluate_batch");
+    if (B == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_evaluate(ctx->k32, ctx->s32, q, ldq, B, vals, jac, st(ctx)));
+  });
+}
+
+pmp_status pmp_pair_clearance_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, float* dist,
+                                    float* grad) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || dist, "null output");
+    require_no_cloud(ctx, "pmp_pair_clearance_batch");
+    if (B == 0 || ctx->n_spheres == 0 || ctx->s32.n_prims == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_pairs(ctx->k32, ctx->s32, q, ldq, B, dist, grad, st(ctx)));
+  });
+}
+
+pmp_status pmp_pair_clearance_vjp(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, const float* grad_dist,
+                                  float* grad_q) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || (grad_dist && grad_q), "null argument");
+    require_no_cloud(ctx, "pmp_pair_clearance_vjp");
+    if (B == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_vjp(ctx->k32, ctx->s32, q, ldq, B, grad_dist, grad_q, st(ctx)));
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
+                   const pmp_validator_desc* v, uint8_t* valid, int32_t* n_checked, EdgeArgs& ea) {
+  require_robot(ctx);
+  PMP_REQUIRE(E >= 0, "negative edge count");
+  PMP_REQUIRE(E == 0 || (s1 && s2 && valid), "null argument");
+  PMP_REQUIRE(lds >= E, "leading dimension smaller than the edge count");
+  PMP_REQUIRE(E < (int64_t)1 << 31, "too many edges");
+  PMP_REQUIRE(v != nullptr, "null validator");
+  require_no_cloud(ctx, "pmp_check_edges_batch");
+  const int D = ctx->n_cols;
+  std::vector<double> w(std::max(D, 1), 1.0), iw(std::max(D, 1), 1.0);
+  if (v->type == PMP_VALIDATOR_BOX) {
+    PMP_REQUIRE(v->box_width != nullptr, "box dimension and space dimension mismatch");
+    for (int i = 0; i < D; ++i) {
+      w[i] = v->box_width[i];
+      iw[i] = 1.0 / v->box_width[i];  // BoxMotionValidator ctor (motion_validator.cpp:81-87)
+    }
+  } else {
+    PMP_REQUIRE(v->type == PMP_VALIDATOR_EUCLIDEAN, "unknown validator type");
+  }
+  set_device(ctx);
+  ea.s1 = s1;
+  ea.s2 = s2;
+  ea.lds = lds;
+  ea.E = E;
+  ea.vtype = v->type;
+  ea.resolution = v->resolution;
+  ea.width = ctx->width.upload(w, st(ctx));
+  ea.inv_width = ctx->inv_width.upload(iw, st(ctx));
+  ea.seq = (const uint8_t*)ctx->seq.ptr;
+  ea.counter = (int32_t*)ctx->counter.ptr;
+  ea.valid = valid;
+  ea.n_checked = n_checked;
+  ea.n_recheck = ((int32_t*)ctx->counter.ptr) + 1;
+}
+
+int edge_grid(pmp_context* ctx, int block) {
+  const int wpc = ctx->sched.waves_per_cu > 0 ? ctx->sched.waves_per_cu : 8;
+  return std::max(1, ctx->n_cu * wpc * 64 / block);
+}
+
+}  // namespace
+
+extern "C" {
+
+pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
+                                 const pmp_validator_desc* v, uint8_t* valid, int32_t* n_checked) {
+  return guarded([&] {
+    EdgeArgs ea{};
+    prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ea);
+    if (E == 0) return;
+    const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
+    PMP_HIP(launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block), block,
+                         st(ctx)));
+  });
+}
+
+pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
+                                 const pmp_validator_desc* v, uint8_t* valid, int32_t* n_checked, pmp_graph** out) {
+  return guarded([&] {
+    PMP_REQUIRE(out != nullptr, "null output pointer");
+    EdgeArgs ea{};
+    prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ea);
+    PMP_REQUIRE(E > 0, "graph of an empty edge batch");
+    PMP_HIP(hipStreamSynchronize(st(ctx)));  // width tables are in place before capture
+    hipStream_t cs = ctx->stream;
+    if (!cs) {
+      if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+      cs = ctx->capture_stream;
+    }
+    const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
+    auto* g = new pmp_graph();
+    g->stream = cs;
+    try {
+      PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      hipError_t le = launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block),
+                                   block, cs);
+      hipError_t ce = hipStreamEndCapture(cs, &g->graph);
+      PMP_HIP(le);
+      PMP_HIP(ce);
+      PMP_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+pmp_status pmp_graph_launch(pmp_graph* g) {
+  return guarded([&] {
+    PMP_REQUIRE(g != nullptr, "null graph");
+    PMP_HIP(hipGraphLaunch(g->exec, g->stream));
+  });
+}
+
+pmp_status pmp_graph_destroy(pmp_graph* g) {
+  return guarded([&] {
+    if (!g) return;
+    (void)hipStreamSynchronize(g->stream);
+    if (g->exec) PMP_HIP(hipGraphExecDestroy(g->exec));
+    if (g->graph) PMP_HIP(hipGraphDestroy(g->graph));
+    delete g;
+  });
+}
+
+pmp_status pmp_prims_evaluate(pmp_context* ctx, const pmp_scene_desc* prims, const double* points, int64_t N,
+                              int32_t mode, double radius, double* out) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr && prims != nullptr, "null argument");
+    PMP_REQUIRE(N >= 0, "negative point count");
+    PMP_REQUIRE(N == 0 || (points && out), "null argument");
+    PMP_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (evaluate) or 1 (is_outside)");
+    PMP_REQUIRE(prims->n_prims >= 1 && prims->n_prims <= kMaxPrims, "primitive count out of range");
+    std::vector<DevPrim<float>> p32;
+    std::vector<DevPrim<double>> p64;
+    std::vector<float> c32;
+    std::vector<double> c64;
+    int nc = 0;
+    pack_prims(*prims, p32, p64, c32, c64, nc);
+    if (N == 0) return;
+    set_device(ctx);
+    auto* dp = ctx->tmp_prims.upload(p64, st(ctx));
+    auto* dc = ctx->tmp_cloud.upload(c64, st(ctx));
+    PMP_HIP(launch_prims_eval(dp, prims->n_prims, dc, points, N, mode, radius, out, st(ctx)));
+  });
+}
+
+pmp_status pmp_nearest_batch(pmp_context* ctx, const double* points, int64_t N, const double* queries, int64_t M,
+                             double* sqdist, int32_t* index) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr, "null context");
+    PMP_REQUIRE(N >= 1 && N < (int64_t)1 << 31, "point count out of range");
+    PMP_REQUIRE(M >= 0, "negative query count");
+    PMP_REQUIRE(M == 0 || (points && queries && sqdist && index), "null argument");
+    if (M == 0) return;
+    set_device(ctx);
+    PMP_HIP(launch_nearest(points, N, queries, M, sqdist, index, st(ctx)));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// Lane-per-configuration forward kinematics.
+//
+// One lane owns one configuration.  The node loop is wave-uniform: node
+// records, sphere tables and primitives are read with uniform indices
+// (scalar loads into SGPRs), while the pose chain and the per-configuration
+// results stay in VGPRs.  Poses that later nodes (or the self-collision pass)
+// need are kept in a small register file of slots whose indices the host
+// assigns; every slot index is uniform, so the accesses lower to register
+// moves rather than scratch memory.
+//
+// The chain follows KinematicModel::set_joint_angles_impl and
+// build_cache_until(_inner) (kinematics/algorithm.cpp:109-210):
+//   revolute   local q = origin_identity ? J : origin_q * J, J = (s axis, c)
+//              of the half angle; composed without the identity shortcut
+//   prismatic  local t = origin_t + axis q, local rotation identity (shortcut)
+//   fixed      static local transform with its identity flag
+//   translation-only links (consider_rotation = false) take the parent
+//   rotation matrix times the local offset and keep the parent rotation.
+#pragma once
+
+#include "geom.hpp"
+
+namespace pmp {
+
+__device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
+
+// Register file of kept poses.  The slot index is wave-uniform, so the
+// switch compiles to scalar branches around plain register moves.
+template <class T>
+struct SlotFile {
+  Pose<T> s0, s1, s2, s3, s4, s5, s6, s7;  // named members: no private array, no scratch
+  __device__ __forceinline__ Pose<T> get(int i) const {
+    if (i == 0) return s0;
+    if (i == 1) return s1;
+    if (i == 2) return s2;
+    if (i == 3) return s3;
+    if (i == 4) return s4;
+    if (i == 5) return s5;
+    if (i == 6) return s6;
+    return s7;
+  }
+  __device__ __forceinline__ void set(int i, const Pose<T>& p) {
+    if (i == 0) s0 = p;
+    else if (i == 1) s1 = p;
+    else if (i == 2) s2 = p;
+    else if (i == 3) s3 = p;
+    else if (i == 4) s4 = p;
+    else if (i == 5) s5 = p;
+    else if (i == 6) s6 = p;
+    else s7 = p;
+  }
+};
+static_assert(kMaxSlots == 8, "SlotFile holds eight poses");
+
+// Configuration sources.  FK reads each column once, in FP64, and converts.
+struct BatchQ {  // q[col * ldq + b]
+  const double* __restrict__ q;
+  int64_t ldq, b;
+  __device__ __forceinline__ double get(int col) const { return q[(int64_t)col * ldq + b]; }
+};
+// Waypoint of an edge: OMPL RealVectorStateSpace::interpolate, from + (to - from) * t,
+// or the end state itself (CustomValidatorBase::checkMotion checks s2 directly,
+// ompl/motion_validator.cpp:52).
+struct EdgeQ {
+  const double* __restrict__ s1;
+  const double* __restrict__ s2;
+  int64_t lds, e;
+  double t;
+  bool at_end;
+  __device__ __forceinline__ double get(int col) const {
+    double a = s1[(int64_t)col * lds + e], b = s2[(int64_t)col * lds + e];
+    return at_end ? b : a + (b - a) * t;
+  }
+};
+
+// Base pose: the constant base pose for a fixed base, else built from the
+// trailing columns of q as ConstraintBase::update_kintree does
+// (constraints/primitive.hpp:38-62).
+template <class T, class QS>
+__device__ __forceinline__ Pose<T> base_pose(const KinArgs<T>& ka, const QS& qs) {
+  Pose<T> base;
+  if (ka.base_type == PMP_BASE_FIXED) {
+    base.q = {ka.base_q[0], ka.base_q[1], ka.base_q[2], ka.base_q[3]};
+    base.t = {ka.base_t[0], ka.base_t[1], ka.base_t[2]};
+  } else if (ka.base_type == PMP_BASE_PLANAR) {
+    int n = ka.n_q_cols;
+    base.t = {(T)qs.get(n), (T)qs.get(n + 1), T(0)};
+    base.q = quat_from_rpy<T>(T(0), T(0), (T)qs.get(n + 2));
+  } else {
+    int n = ka.n_q_cols;
+    base.t = {(T)qs.get(n), (T)qs.get(n + 1), (T)qs.get(n + 2)};
+    base.q = quat_from_rpy<T>((T)qs.get(n + 3), (T)qs.get(n + 4),
+                              (T)qs.get(n + 5));
+  }
+  return base;
+}
+
+// Visitor interface:
+//   bool alive() const                          lane still has work
+//   void begin(const Pose<T>& base, const QS&)  base pose of this configuration
+//   void node(int n, const DevNode<T>&, const Pose<T>&)
+//   void sphere(int s, V3<T> c, T r)            s indexes the node-sorted table
+//   void self(const DevSelfPair&, int i, int j, V3<T> ci, V3<T> cj, T rsum)
+// `taylor` selects the collision check's sin/cos (a runtime, wave-uniform flag).
+template <class T, class QS, class Vis>
+__device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool taylor, Vis& vis) {
+  SlotFile<T> slots;
+  const Pose<T> base = base_pose(ka, qs);
+  vis.begin(base, qs);
+  Pose<T> prev = base;
+  for (int n = 0; n < ka.n_nodes; ++n) {
+    if (!wave_any(vis.alive())) return;
+    const DevNode<T>& nd = ka.nodes[n];
+    // parent_slot: -2 the previous node, -1 the base pose, >= 0 a kept pose
+    const Pose<T> par = nd.parent_slot == -2 ? prev : (nd.parent_slot == -1 ? base : slots.get(nd.parent_slot));
+    const V3<T> lt{nd.trans[0], nd.trans[1], nd.trans[2]};
+    const Q4<T> oq{nd.quat[0], nd.quat[1], nd.quat[2], nd.quat[3]};
+    Pose<T> P;
+    switch (nd.kind) {
+      case PMP_NODE_ROOT:
+        P = base;
+        break;
+      case PMP_NODE_FIXED:
+        P = compose(par, oq, lt, nd.identity != 0);
+        break;
+      case PMP_NODE_REVOLUTE: {
+        T x = (T)qs.get(nd.q_col) * T(0.5);
+        T s, c;
+        if (taylor)
+          taylor_sincos<T>(x, s, c);
+        else
+          exact_sincos(x, s, c);
+        Q4<T> jq{s * nd.axis[0], s * nd.axis[1], s * nd.axis[2], c};
+        Q4<T> lq = nd.identity ? jq : qmul(oq, jq);
+        P = compose(par, lq, lt, false);
+        break;
+      }
+      case PMP_NODE_PRISMATIC: {
+        T v = (T)qs.get(nd.q_col);
+        V3<T> t2{lt.x + nd.axis[0] * v, lt.y + nd.axis[1] * v, lt.z + nd.axis[2] * v};
+        P = compose(par, oq, t2, true);
+        break;
+      }
+      default: {  // PMP_NODE_TRANSLATION
+        M3<T> R = qmat(par.q);
+        P.q = par.q;
+        P.t = add(par.t, mv(R, lt));
+        break;
+      }
+    }
+    if (nd.out_slot >= 0) slots.set(nd.out_slot, P);
+    prev = P;
+    vis.node(n, nd, P);
+    if (nd.sphere_begin < nd.sphere_end) {
+      const M3<T> R = qmat(P.q);
+      for (int s = nd.sphere_begin; s < nd.sphere_end; ++s) {
+        const DevSphere<T> sp = ka.spheres[s];
+        V3<T> c = add(mv(R, V3<T>{sp.x, sp.y, sp.z}), P.t);
+        vis.sphere(s, c, sp.r);
+      }
+    }
+  }
+  // Self-collision sphere pairs, group pair by group pair
+  // (constraints/primitive_sphere_collision.cpp:222-254, 324-383).
+  if constexpr (!Vis::kSelf) return;
+  for (int k = 0; k < ka.n_pairs; ++k) {
+    if (!wave_any(vis.alive())) return;
+    const DevSelfPair pr = ka.pairs[k];
+    const Pose<T> P1 = slots.get(pr.slot1), P2 = slots.get(pr.slot2);
+    const M3<T> R1 = qmat(P1.q), R2 = qmat(P2.q);
+    for (int i = pr.begin1; i < pr.end1; ++i) {
+      const DevSphere<T> a = ka.spheres[i];
+      const V3<T> ci = add(mv(R1, V3<T>{a.x, a.y, a.z}), P1.t);
+      for (int j = pr.begin2; j < pr.end2; ++j) {
+        const DevSphere<T> bb = ka.spheres[j];
+        const V3<T> cj = add(mv(R2, V3<T>{bb.x, bb.y, bb.z}), P2.t);
+        vis.self(pr, i, j, ci, cj, a.r + bb.r);
+      }
+    }
+  }
+}
+
+}  // namespace pmp

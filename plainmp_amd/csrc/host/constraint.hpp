@@ -1,0 +1,495 @@
+// SphereCollisionCst and the motion validators above the C ABI.
+// Construction follows the reference ctor (constraints/primitive_sphere_collision.cpp:117-179):
+// groups with mean centre and radius max(|p - c| + r), the optional
+// informative sphere reorder, self pairs resolved by link name, and the SDF
+// tree flattened at set_sdf time with primitives shared by pointer.
+#pragma once
+
+#include "host.hpp"
+
+namespace pmp_host {
+
+struct SphereAttachmentSpec {  // primitive_sphere_collision.hpp:16-21
+  std::string parent_link_name;
+  std::vector<double> relative_positions;  // (k, 3) row-major
+  std::vector<double> radii;
+  bool only_self_collision = false;
+};
+
+// reorder_spheres_for_informative_collision_check (primitive_sphere_collision.cpp:53-96):
+// farthest-point order seeded from unordered_set<size_t>::begin().  For keys
+// 0..n-1 inserted in increasing order after reserve(n), libstdc++ iterates
+// them n-1 ... 0 and erase() keeps the order of the rest, so the seed is n-1
+// and ties go to the first candidate in descending index order.
+inline std::vector<size_t> informative_order(const std::vector<double>& pos) {
+  const size_t n = pos.size() / 3;
+  std::vector<size_t> unvisited;
+  for (size_t i = 0; i < n; ++i) unvisited.push_back(n - 1 - i);
+  std::vector<size_t> visited{unvisited.front()};
+  unvisited.erase(unvisited.begin());
+  auto dist = [&](size_t a, size_t b) {
+    double dx = pos[3 * a] - pos[3 * b], dy = pos[3 * a + 1] - pos[3 * b + 1], dz = pos[3 * a + 2] - pos[3 * b + 2];
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  while (!unvisited.empty()) {
+    double max_dist = 0.0;
+    size_t best = 0;
+    for (size_t k = 0; k < unvisited.size(); ++k) {
+      double mind = std::numeric_limits<double>::max();
+      for (size_t v : visited) mind = std::min(mind, dist(unvisited[k], v));
+      if (mind > max_dist) {
+        max_dist = mind;
+        best = k;
+      }
+    }
+    visited.push_back(unvisited[best]);
+    unvisited.erase(unvisited.begin() + best);
+  }
+  return visited;
+}
+
+struct SphereGroup {
+  std::string parent_link_name;
+  size_t parent_link_id = 0;
+  std::vector<double> rel;  // (k, 3)
+  std::vector<double> radii;
+  double group_radius = 0;
+  Vec3 group_rel{0, 0, 0};
+  bool only_self_collision = false;
+};
+
+class SphereCollisionCst {
+ public:
+  SphereCollisionCst(std::shared_ptr<KinematicModel> kin, const std::vector<std::string>& control_joint_names,
+                     int base_type, const std::vector<SphereAttachmentSpec>& specs,
+                     const std::vector<std::pair<std::string, std::string>>& selcol_pairs,
+                     std::optional<SDFBase::Ptr> fixed_sdf, bool reorder_spheres = true)
+      : kin_(std::move(kin)), control_names_(control_joint_names), base_type_(base_type) {
+    if (!kin_) throw std::invalid_argument("null kinematic model");
+    if (base_type < FIXED || base_type > FLOATING) throw std::invalid_argument("unknown base type");
+    control_ids_ = kin_->get_joint_ids(control_joint_names);
+    if (fixed_sdf && *fixed_sdf) fixed_sdf_ = *fixed_sdf;
+    for (auto& spec : specs) {
+      if (spec.relative_positions.size() != 3 * spec.radii.size() || spec.radii.empty())
+        throw std::invalid_argument("sphere spec needs (k, 3) positions and k radii, k >= 1");
+      SphereGroup g;
+      g.parent_link_name = spec.parent_link_name;
+      g.parent_link_id = kin_->get_link_ids({spec.parent_link_name})[0];
+      const size_t k = spec.radii.size();
+      Vec3 c{0, 0, 0};
+      for (size_t j = 0; j < k; ++j)
+        for (int a = 0; a < 3; ++a) c[a] += spec.relative_positions[3 * j + a];
+      for (int a = 0; a < 3; ++a) c[a] /= (double)k;
+      double max_dist = 0.0;
+      for (size_t j = 0; j < k; ++j) {
+        double dx = spec.relative_positions[3 * j] - c[0], dy = spec.relative_positions[3 * j + 1] - c[1],
+               dz = spec.relative_positions[3 * j + 2] - c[2];
+        max_dist = std::max(max_dist, std::sqrt(dx * dx + dy * dy + dz * dz) + spec.radii[j]);
+      }
+      g.rel = spec.relative_positions;
+      g.radii = spec.radii;
+      g.group_radius = max_dist;
+      g.group_rel = c;
+      g.only_self_collision = spec.only_self_collision;
+      groups_.push_back(g);
+      if (reorder_spheres)  // re-applied to every group on each push (:155-159)
+        for (auto& gg : groups_) {
+          auto ord = informative_order(gg.rel);
+          std::vector<double> rp, rr;
+          for (auto i : ord) {
+            rp.insert(rp.end(), {gg.rel[3 * i], gg.rel[3 * i + 1], gg.rel[3 * i + 2]});
+            rr.push_back(gg.radii[i]);
+          }
+          gg.rel = rp;
+          gg.radii = rr;
+        }
+    }
+    for (auto& pr : selcol_pairs) {
+      int a = -1, b = -1;
+      for (size_t i = 0; i < groups_.size(); ++i) {
+        if (a < 0 && groups_[i].parent_link_name == pr.first) a = (int)i;
+        if (b < 0 && groups_[i].parent_link_name == pr.second) b = (int)i;
+      }
+      if (a < 0 || b < 0) throw std::runtime_error("(cpp) Invalid pair of link names for self collision");
+      self_pairs_.push_back({a, b});
+    }
+    set_all_sdfs();
+  }
+
+  void set_sdf(const SDFBase::Ptr& sdf) {
+    sdf_ = sdf;
+    set_all_sdfs();
+  }
+  SDFBase::Ptr get_sdf() const { return sdf_; }
+
+  size_t q_dim() const { return control_ids_.size() + (base_type_ == FLOATING ? 6 : 0) + (base_type_ == PLANAR ? 3 : 0); }
+  bool ext_enabled() const { return !all_sdfs_.empty(); }
+  bool self_enabled() const { return !self_pairs_.empty(); }
+  size_t cst_dim() const { return (ext_enabled() ? 1 : 0) + (self_enabled() ? 1 : 0); }
+  size_t n_spheres() const {
+    size_t n = 0;
+    for (auto& g : groups_) n += g.radii.size();
+    return n;
+  }
+  size_t n_prims() const { return all_sdfs_.size(); }
+  std::shared_ptr<KinematicModel> get_kin() const { return kin_; }
+  std::vector<std::string> get_control_joint_names() const { return control_names_; }
+  int base_type() const { return base_type_; }
+  const std::vector<SphereGroup>& groups() const { return groups_; }
+
+  // ConstraintBase::update_kintree (constraints/primitive.hpp:38-62): the
+  // model state is updated so that later state queries see q.
+  void update_kintree(const std::vector<double>& q, bool high_accuracy) {
+    check_q(q);
+    const size_t n = control_ids_.size();
+    if (base_type_ != FIXED) {
+      kin_->set_joint_angles(control_ids_, std::vector<double>(q.begin(), q.begin() + n), true);
+      Quat bq;
+      std::array<double, 7> pose{};
+      if (base_type_ == FLOATING) {
+        bq = quat_from_rpy(q[n + 3], q[n + 4], q[n + 5]);
+        pose = {q[n], q[n + 1], q[n + 2], bq.x, bq.y, bq.z, bq.w};
+      } else {
+        bq = quat_from_rpy(0, 0, q[n + 2]);
+        pose = {q[n], q[n + 1], 0.0, bq.x, bq.y, bq.z, bq.w};
+      }
+      kin_->set_base_pose(pose);
+    } else {
+      kin_->set_joint_angles(control_ids_, q, high_accuracy);
+    }
+    state_q_ = q;
+    state_accurate_ = base_type_ != FIXED || high_accuracy;
+  }
+
+  // IneqConstraintBase::is_valid (primitive.hpp:113-117): one configuration on the GPU.
+  bool is_valid(const std::vector<double>& q) {
+    update_kintree(q, false);
+    auto v = is_valid_batch(q.data(), 1);
+    return v[0] != 0;
+  }
+
+  // ConstraintBase::evaluate (primitive.hpp:74-79): (values, Jacobian rows).
+  std::pair<std::vector<double>, std::vector<double>> evaluate(const std::vector<double>& q) {
+    update_kintree(q, true);
+    std::vector<double> vals, jac;
+    evaluate_batch(q.data(), 1, vals, jac);
+    return {vals, jac};
+  }
+
+  // ---------------------------------------------------------- batch (host arrays)
+  std::vector<uint8_t> is_valid_batch(const double* Q, int64_t B, int32_t* n_recheck = nullptr) {
+    sync_batch_tables();
+    std::vector<uint8_t> out(B);
+    if (B == 0) return out;
+    const int D = (int)q_dim();
+    double* dq = batch_.upload_soa(0, Q, B, D);
+    uint8_t* dv = (uint8_t*)batch_.buf(1, B);
+    int32_t* dn = (int32_t*)batch_.buf(2, 4);
+    int32_t zero = 0;
+    batch_.h2d(dn, &zero, 4);
+    check(pmp_is_valid_batch(batch_.ctx(), dq, B, B, dv, dn));
+    batch_.d2h(out.data(), dv, B);
+    if (n_recheck) batch_.d2h(n_recheck, dn, 4);
+    return out;
+  }
+  // vals (B, cst_dim), jac (B, cst_dim, q_dim), row-major
+  void evaluate_batch(const double* Q, int64_t B, std::vector<double>& vals, std::vector<double>& jac) {
+    sync_batch_tables();
+    const int D = (int)q_dim(), C = (int)cst_dim();
+    vals.assign((size_t)B * C, 0.0);
+    jac.assign((size_t)B * C * D, 0.0);
+    if (B == 0 || C == 0) return;
+    double* dq = batch_.upload_soa(0, Q, B, D);
+    float* dv = (float*)batch_.buf(1, sizeof(float) * B * C);
+    float* dj = (float*)batch_.buf(2, sizeof(float) * B * C * D);
+    check(pmp_evaluate_batch(batch_.ctx(), dq, B, B, dv, dj));
+    std::vector<float> hv((size_t)B * C), hj((size_t)B * C * D);
+    batch_.d2h(hv.data(), dv, hv.size() * sizeof(float));
+    batch_.d2h(hj.data(), dj, hj.size() * sizeof(float));
+    for (int64_t b = 0; b < B; ++b)
+      for (int k = 0; k < C; ++k) {
+        vals[b * C + k] = hv[(size_t)k * B + b];
+        for (int j = 0; j < D; ++j) jac[((size_t)b * C + k) * D + j] = hj[((size_t)k * D + j) * B + b];
+      }
+  }
+  // (B, S) per-sphere minimum clearance
+  std::vector<float> sphere_clearance_batch(const double* Q, int64_t B, bool accurate) {
+    sync_batch_tables();
+    const int D = (int)q_dim();
+    const size_t S = n_spheres();
+    std::vector<float> out((size_t)B * S);
+    if (B == 0 || S == 0) return out;
+    double* dq = batch_.upload_soa(0, Q, B, D);
+    float* dc = (float*)batch_.buf(1, sizeof(float) * B * S);
+    check(pmp_sphere_clearance_batch(batch_.ctx(), dq, B, B, accurate ? 1 : 0, dc));
+    std::vector<float> h((size_t)B * S);
+    batch_.d2h(h.data(), dc, h.size() * sizeof(float));
+    for (int64_t b = 0; b < B; ++b)
+      for (size_t s = 0; s < S; ++s) out[b * S + s] = h[s * B + b];
+    return out;
+  }
+  // (B, S, 3) sphere centres
+  std::vector<float> sphere_centers_batch(const double* Q, int64_t B, bool accurate) {
+    sync_batch_tables();
+    const int D = (int)q_dim();
+    const size_t S = n_spheres();
+    std::vector<float> out((size_t)B * S * 3);
+    if (B == 0 || S == 0) return out;
+    double* dq = batch_.upload_soa(0, Q, B, D);
+    float* dc = (float*)batch_.buf(1, sizeof(float) * B * S * 3);
+    check(pmp_fk_spheres_batch(batch_.ctx(), dq, B, B, accurate ? 1 : 0, dc));
+    std::vector<float> h((size_t)B * S * 3);
+    batch_.d2h(h.data(), dc, h.size() * sizeof(float));
+    for (int64_t b = 0; b < B; ++b)
+      for (size_t s = 0; s < S; ++s)
+        for (int k = 0; k < 3; ++k) out[(b * S + s) * 3 + k] = h[(s * 3 + k) * B + b];
+    return out;
+  }
+  // dist (B, S, P), grad (B, S, P, q_dim) or empty
+  void pair_clearance_batch(const double* Q, int64_t B, bool with_grad, std::vector<float>& dist,
+                            std::vector<float>& grad) {
+    sync_batch_tables();
+    const int D = (int)q_dim();
+    const size_t S = n_spheres(), P = n_prims();
+    dist.assign((size_t)B * S * P, 0.f);
+    grad.assign(with_grad ? (size_t)B * S * P * D : 0, 0.f);
+    if (B == 0 || S == 0 || P == 0) return;
+    double* dq = batch_.upload_soa(0, Q, B, D);
+    float* dd = (float*)batch_.buf(1, sizeof(float) * B * S * P);
+    float* dg = with_grad ? (float*)batch_.buf(2, sizeof(float) * B * S * P * D) : nullptr;
+    check(pmp_pair_clearance_batch(batch_.ctx(), dq, B, B, dd, dg));
+    std::vector<float> h((size_t)B * S * P);
+    batch_.d2h(h.data(), dd, h.size() * sizeof(float));
+    for (int64_t b = 0; b < B; ++b)
+      for (size_t r = 0; r < S * P; ++r) dist[b * S * P + r] = h[r * B + b];
+    if (with_grad) {
+      std::vector<float> g((size_t)B * S * P * D);
+      batch_.d2h(g.data(), dg, g.size() * sizeof(float));
+      for (int64_t b = 0; b < B; ++b)
+        for (size_t r = 0; r < S * P * D; ++r) grad[b * S * P * D + r] = g[r * B + b];
+    }
+  }
+
+  // ------------------------------------------------- batch (device pointers)
+  pmp_context* device_context(void* stream) {
+    sync_batch_tables();
+    check(pmp_set_stream(batch_.ctx(), stream));
+    return batch_.ctx();
+  }
+
+  // ----------------------------------------------------------- state queries
+  // get_all_spheres / get_group_spheres on the model's current state
+  // (constraints/primitive_sphere_collision.cpp:401-424): the control joints
+  // take the configuration of the last update_kintree; FK runs on the GPU.
+  std::vector<std::pair<Vec3, double>> get_all_spheres() { return state_spheres(false); }
+  std::vector<std::pair<Vec3, double>> get_group_spheres() { return state_spheres(true); }
+
+  // Tables for a batched call (also used by validators).
+  void sync_batch_tables() {
+    sync_robot(batch_, control_ids_, base_type_, false, batch_robot_version_);
+    sync_scene(batch_, batch_scene_sig_);
+  }
+
+ private:
+  void check_q(const std::vector<double>& q) const {
+    if (q.size() != q_dim())
+      throw std::invalid_argument("configuration has " + std::to_string(q.size()) + " entries, expected " +
+                                  std::to_string(q_dim()));
+  }
+  void set_all_sdfs() {
+    all_sdfs_.clear();
+    if (fixed_sdf_) fixed_sdf_->flatten(all_sdfs_);
+    if (sdf_) sdf_->flatten(all_sdfs_);
+    batch_scene_sig_.clear();
+    state_scene_sig_.clear();
+  }
+  // Robot tables: node tree from the model, spheres from the groups.  With
+  // `group_spheres` each group contributes one sphere (centre, group radius).
+  void sync_robot(Device& dev, const std::vector<size_t>& q_joints, int base_type, bool group_spheres,
+                  uint64_t& cached_version) {
+    const uint64_t v = kin_->version() * 4 + (group_spheres ? 2 : 0) + 1;
+    if (cached_version == v && !q_joints_changed(dev, q_joints)) return;
+    PackedRobot pr;
+    kin_->pack(q_joints, pr);
+    std::vector<int32_t> s_node, s_group;
+    std::vector<double> s_local, s_radius;
+    std::vector<uint8_t> only_self;
+    for (size_t g = 0; g < groups_.size(); ++g) {
+      const auto& gr = groups_[g];
+      only_self.push_back(gr.only_self_collision ? 1 : 0);
+      if (group_spheres) {
+        s_node.push_back((int32_t)gr.parent_link_id);
+        s_group.push_back((int32_t)g);
+        s_local.insert(s_local.end(), gr.group_rel.begin(), gr.group_rel.end());
+        s_radius.push_back(gr.group_radius);
+      } else {
+        for (size_t i = 0; i < gr.radii.size(); ++i) {
+          s_node.push_back((int32_t)gr.parent_link_id);
+          s_group.push_back((int32_t)g);
+          s_local.insert(s_local.end(), {gr.rel[3 * i], gr.rel[3 * i + 1], gr.rel[3 * i + 2]});
+          s_radius.push_back(gr.radii[i]);
+        }
+      }
+    }
+    std::vector<int32_t> pairs;
+    for (auto& p : self_pairs_) pairs.insert(pairs.end(), {p.first, p.second});
+    pmp_robot_desc d{};
+    d.n_nodes = (int32_t)pr.parent.size();
+    d.node_parent = pr.parent.data();
+    d.node_kind = pr.kind.data();
+    d.node_q_col = pr.q_col.data();
+    d.node_quat = pr.quat.data();
+    d.node_trans = pr.trans.data();
+    d.node_axis = pr.axis.data();
+    d.node_identity = pr.identity.data();
+    d.n_q_cols = (int32_t)q_joints.size();
+    d.base_type = base_type;
+    auto bp = kin_->get_base_pose();
+    for (int k = 0; k < 7; ++k) d.base_pose[k] = bp[k];
+    d.n_spheres = (int32_t)s_radius.size();
+    d.sphere_node = s_node.data();
+    d.sphere_local = s_local.data();
+    d.sphere_radius = s_radius.data();
+    d.sphere_group = s_group.data();
+    d.n_groups = (int32_t)groups_.size();
+    d.group_only_self = only_self.data();
+    d.n_self_pairs = group_spheres ? 0 : (int32_t)self_pairs_.size();
+    d.self_pairs = pairs.data();
+    check(pmp_upload_robot(dev.ctx(), &d));
+    cached_version = v;
+    uploaded_q_[&dev] = q_joints;
+  }
+  bool q_joints_changed(Device& dev, const std::vector<size_t>& q) {
+    auto it = uploaded_q_.find(&dev);
+    return it == uploaded_q_.end() || it->second != q;
+  }
+  void sync_scene(Device& dev, std::vector<uint64_t>& sig) {
+    std::vector<uint64_t> now;
+    for (auto& p : all_sdfs_) {
+      now.push_back((uint64_t)(uintptr_t)p.get());
+      now.push_back(p->version);
+    }
+    now.push_back(all_sdfs_.size());
+    if (now == sig) return;
+    std::vector<pmp_prim_desc> d(all_sdfs_.size());
+    for (size_t i = 0; i < all_sdfs_.size(); ++i) all_sdfs_[i]->describe(d[i]);
+    pmp_scene_desc sc{(int32_t)d.size(), d.data()};
+    check(pmp_upload_scene(dev.ctx(), &sc));
+    sig = now;
+  }
+  std::vector<std::pair<Vec3, double>> state_spheres(bool groups) {
+    // q columns: the control joints (and base) as last set through this constraint
+    std::vector<double> q = state_q_;
+    if (q.size() != q_dim()) {  // never updated: current model angles, fixed base pose
+      q = kin_->get_joint_angles(control_ids_);
+      q.resize(q_dim(), 0.0);
+    }
+    Device& dev = groups ? state_groups_ : state_;
+    uint64_t& ver = groups ? state_groups_version_ : state_version_;
+    // a non-fixed base is rebuilt from q, so the model's base pose is not used there
+    std::vector<size_t> cols = control_ids_;
+    bool fresh = kin_->version() * 4 + (groups ? 2 : 0) + 1 != ver;
+    sync_robot(dev, cols, base_type_, groups, ver);
+    (void)fresh;
+    const int D = (int)q_dim();
+    const size_t S = groups ? groups_.size() : n_spheres();
+    std::vector<std::pair<Vec3, double>> out;
+    if (S == 0) return out;
+    double* dq = dev.upload_soa(0, q.data(), 1, D);
+    float* dc = (float*)dev.buf(1, sizeof(float) * S * 3);
+    check(pmp_fk_spheres_batch(dev.ctx(), dq, 1, 1, state_accurate_ ? 1 : 0, dc));
+    std::vector<float> h(S * 3);
+    dev.d2h(h.data(), dc, h.size() * sizeof(float));
+    size_t s = 0;
+    for (auto& g : groups_) {
+      if (groups) {
+        out.push_back({{h[3 * s], h[3 * s + 1], h[3 * s + 2]}, g.group_radius});
+        ++s;
+      } else {
+        for (double r : g.radii) {
+          out.push_back({{h[3 * s], h[3 * s + 1], h[3 * s + 2]}, r});
+          ++s;
+        }
+      }
+    }
+    return out;
+  }
+
+  std::shared_ptr<KinematicModel> kin_;
+  std::vector<std::string> control_names_;
+  std::vector<size_t> control_ids_;
+  int base_type_;
+  std::vector<SphereGroup> groups_;
+  std::vector<std::pair<int, int>> self_pairs_;
+  SDFBase::Ptr fixed_sdf_, sdf_;
+  std::vector<std::shared_ptr<PrimitiveSDFBase>> all_sdfs_;
+  Device batch_, state_, state_groups_;
+  uint64_t batch_robot_version_ = 0, state_version_ = 0, state_groups_version_ = 0;
+  std::vector<uint64_t> batch_scene_sig_, state_scene_sig_;
+  std::unordered_map<const Device*, std::vector<size_t>> uploaded_q_;
+  std::vector<double> state_q_;
+  bool state_accurate_ = true;
+};
+
+// ValidatorConfig (ompl/ompl_thin_wrap.hpp:50-59)
+struct ValidatorConfig {
+  enum class Type { BOX = 0, EUCLIDEAN = 1 };
+  Type type = Type::BOX;
+  double resolution = 0.0;
+  std::vector<double> box_width;
+};
+
+// Box / euclidean motion validators over a SphereCollisionCst
+// (ompl/motion_validator.cpp:17-126): checkMotion for one edge, and
+// check_motions for a batch in one persistent launch.
+class MotionValidator {
+ public:
+  MotionValidator(std::shared_ptr<SphereCollisionCst> cst, const ValidatorConfig& vc) : cst_(std::move(cst)), vc_(vc) {
+    if (!cst_) throw std::runtime_error("ineq_cst is nullptr. You should provide a constraint");
+    if (vc.type == ValidatorConfig::Type::BOX) {
+      if (vc.box_width.size() != cst_->q_dim()) throw std::runtime_error("box dimension and space dimension mismatch");
+    } else if (vc.type != ValidatorConfig::Type::EUCLIDEAN) {
+      throw std::runtime_error("unknown validator type");
+    }
+  }
+  bool checkMotion(const std::vector<double>& s1, const std::vector<double>& s2) {
+    if (s1.size() != cst_->q_dim() || s2.size() != cst_->q_dim())
+      throw std::invalid_argument("state dimension mismatch");
+    return check_motions(s1.data(), s2.data(), 1, nullptr)[0] != 0;
+  }
+  // S1, S2 row-major (E, q_dim)
+  std::vector<uint8_t> check_motions(const double* S1, const double* S2, int64_t E, std::vector<int32_t>* n_checked) {
+    cst_->sync_batch_tables();
+    pmp_context* ctx = cst_->device_context(nullptr);
+    const int D = (int)cst_->q_dim();
+    std::vector<uint8_t> out(E);
+    if (E == 0) return out;
+    double* d1 = dev_.upload_soa(0, S1, E, D);
+    double* d2 = dev_.upload_soa(1, S2, E, D);
+    uint8_t* dv = (uint8_t*)dev_.buf(2, E);
+    int32_t* dn = (int32_t*)dev_.buf(3, sizeof(int32_t) * E);
+    pmp_validator_desc v = desc();
+    check(pmp_check_edges_batch(ctx, d1, d2, E, E, &v, dv, dn));
+    dev_.d2h(out.data(), dv, E);
+    if (n_checked) {
+      n_checked->resize(E);
+      dev_.d2h(n_checked->data(), dn, sizeof(int32_t) * E);
+    }
+    return out;
+  }
+  pmp_validator_desc desc() const {
+    pmp_validator_desc v{};
+    v.type = vc_.type == ValidatorConfig::Type::BOX ? PMP_VALIDATOR_BOX : PMP_VALIDATOR_EUCLIDEAN;
+    v.resolution = vc_.resolution;
+    v.box_width = vc_.box_width.empty() ? nullptr : vc_.box_width.data();
+    return v;
+  }
+  std::shared_ptr<SphereCollisionCst> constraint() const { return cst_; }
+
+ private:
+  std::shared_ptr<SphereCollisionCst> cst_;
+  ValidatorConfig vc_;
+  Device dev_;  // edge staging buffers (the tables live in the constraint's context)
+};
+
+}  // namespace pmp_host

@@ -1,0 +1,730 @@
+// gfx950 kernels of the FK + sphere-SDF collision path.
+//
+// All batch kernels use one lane per configuration (or per edge waypoint):
+// a wave carries 64 configurations through the same node / sphere /
+// primitive loop, the tables are wave-uniform, and results are written in
+// "batch index fastest" layout so that every store of a wave is one
+// coalesced 256-byte line.
+#include <float.h>
+
+#include "fk.hpp"
+#include "launch.h"
+
+namespace pmp {
+
+constexpr float kCutoff = 0.1f;  // SphereCollisionCst::cutoff_dist_ (primitive_sphere_collision.hpp:115)
+
+template <class T>
+__device__ __forceinline__ const DevPrim<T>& prim_at(const SceneArgs<T>& sc, int p) {
+  return reinterpret_cast<const DevPrim<T>*>(sc.prims)[p];
+}
+
+__device__ __forceinline__ int64_t gid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ------------------------------------------------------------------ centres
+template <class T>
+struct CentersVis {
+  static constexpr bool kSelf = false;
+  const KinArgs<T>& ka;
+  float* out;
+  int64_t B, b;
+  bool active;
+  __device__ bool alive() const { return true; }
+  template <class QS>
+  __device__ void begin(const Pose<T>&, const QS&) {}
+  __device__ void node(int, const DevNode<T>&, const Pose<T>&) {}
+  __device__ void sphere(int s, V3<T> c, T) {
+    if (!active) return;
+    const int o = ka.meta[s].out;
+    out[(int64_t)(o * 3 + 0) * B + b] = (float)c.x;
+    out[(int64_t)(o * 3 + 1) * B + b] = (float)c.y;
+    out[(int64_t)(o * 3 + 2) * B + b] = (float)c.z;
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<T>, V3<T>, T) {}
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) centers_kernel(KinArgs<T> ka, const double* q, int64_t ldq, int64_t B,
+                                                      int taylor, float* out) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  CentersVis<T> v{ka, out, B, b, active};
+  run_fk(ka, qs, taylor != 0, v);
+}
+
+// ------------------------------------------------------------- link poses
+struct PosesVis {
+  static constexpr bool kSelf = false;
+  double* out;
+  int64_t B, b;
+  bool active;
+  __device__ bool alive() const { return true; }
+  template <class QS>
+  __device__ void begin(const Pose<double>&, const QS&) {}
+  __device__ void node(int n, const DevNode<double>&, const Pose<double>& P) {
+    if (!active) return;
+    double v[7] = {P.t.x, P.t.y, P.t.z, P.q.x, P.q.y, P.q.z, P.q.w};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) out[(int64_t)(n * 7 + k) * B + b] = v[k];
+  }
+  __device__ void sphere(int, V3<double>, double) {}
+  __device__ void self(const DevSelfPair&, int, int, V3<double>, V3<double>, double) {}
+};
+
+__global__ void __launch_bounds__(256) poses_kernel(KinArgs<double> ka, const double* q, int64_t ldq, int64_t B,
+                                                    int taylor, double* out) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  PosesVis v{out, B, b, active};
+  run_fk(ka, qs, taylor != 0, v);
+}
+
+// -------------------------------------------------------------- clearance
+struct ClearanceVis {
+  static constexpr bool kSelf = false;
+  const KinArgs<float>& ka;
+  const SceneArgs<float>& sc;
+  float* out;
+  int64_t B, b;
+  bool active;
+  __device__ bool alive() const { return true; }
+  template <class QS>
+  __device__ void begin(const Pose<float>&, const QS&) {}
+  __device__ void node(int, const DevNode<float>&, const Pose<float>&) {}
+  __device__ void sphere(int s, V3<float> c, float r) {
+    float best = INFINITY;
+    for (int p = 0; p < sc.n_prims; ++p) {
+      const DevPrim<float>& P = prim_at(sc, p);
+      if (P.type == PMP_PRIM_CLOUD) continue;
+      best = fminf(best, sdf_eval(P, c) - r);
+    }
+    if (active) out[(int64_t)ka.meta[s].out * B + b] = best;
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
+};
+
+__global__ void __launch_bounds__(256) clearance_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
+                                                        int64_t ldq, int64_t B, int taylor, float* out) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  ClearanceVis v{ka, sc, out, B, b, active};
+  run_fk(ka, qs, taylor != 0, v);
+}
+
+// --------------------------------------------------------------- validity
+// FP32 pass: a collision decided outside the band ends the lane's work
+// ("definite"); a decision inside the band marks the configuration for the
+// FP64 re-decision ("uncertain").
+struct ValidVis32 {
+  static constexpr bool kSelf = true;
+  const KinArgs<float>& ka;
+  const SceneArgs<float>& sc;
+  float band;
+  bool idle = false;
+  bool definite = false;
+  bool uncertain = false;
+  __device__ bool alive() const { return !definite && !idle; }
+  template <class QS>
+  __device__ void begin(const Pose<float>&, const QS&) {}
+  __device__ void node(int, const DevNode<float>&, const Pose<float>&) {}
+  __device__ void sphere(int s, V3<float> c, float r) {
+    if (!alive() || !ka.meta[s].ext) return;
+    for (int p = 0; p < sc.n_prims; ++p) {
+      const DevPrim<float>& P = prim_at(sc, p);
+      if (P.type == PMP_PRIM_CLOUD) continue;
+      bool inb = false;
+      const bool col = collide_fast(P, c, r, band, inb);
+      if (col && !inb) {
+        definite = true;
+        return;
+      }
+      uncertain |= inb;
+    }
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<float> ci, V3<float> cj, float rs) {
+    if (!alive()) return;
+    const V3<float> d = sub(ci, cj);
+    const float m = dot(d, d) - rs * rs;
+    if (fabsf(m) <= band * (2.0f * rs + band))
+      uncertain = true;
+    else if (m < 0.0f)
+      definite = true;
+  }
+};
+
+// FP64 re-decision with the reference's own tests: per sphere the AABB
+// pre-test and is_outside (check_ext_collision), per self pair the strict
+// squared-distance test (check_self_collision).  The group-sphere broad
+// phase of the reference only culls, so it is not needed for the result.
+struct ValidVis64 {
+  static constexpr bool kSelf = true;
+  const KinArgs<double>& ka;
+  const SceneArgs<double>& sc;
+  bool collided = false;
+  __device__ bool alive() const { return !collided; }
+  template <class QS>
+  __device__ void begin(const Pose<double>&, const QS&) {}
+  __device__ void node(int, const DevNode<double>&, const Pose<double>&) {}
+  __device__ void sphere(int s, V3<double> c, double r) {
+    if (collided || !ka.meta[s].ext) return;
+    for (int p = 0; p < sc.n_prims; ++p) {
+      const DevPrim<double>& P = prim_at(sc, p);
+      if (P.type == PMP_PRIM_CLOUD) continue;
+      if (!outside_aabb(P, c, r) && !is_outside_ref(P, c, r)) {
+        collided = true;
+        return;
+      }
+    }
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<double> ci, V3<double> cj, double rs) {
+    if (collided) return;
+    const V3<double> d = sub(ci, cj);
+    if (dot(d, d) < rs * rs) collided = true;
+  }
+};
+
+// valid[b]: 0 collision, 1 free, 2 pending FP64 re-decision.
+__global__ void __launch_bounds__(256) valid_kernel(KinArgs<float> k32, SceneArgs<float> s32, const double* q,
+                                                    int64_t ldq, int64_t B, float band, uint8_t* valid) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  const bool taylor = k32.base_type == PMP_BASE_FIXED;  // constraints/primitive.hpp:59-61
+  ValidVis32 v{k32, s32, band};
+  v.idle = !active;
+  run_fk(k32, qs, taylor, v);
+  if (active) valid[b] = v.definite ? 0 : (v.uncertain ? 2 : 1);
+}
+
+// FP64 re-decision of the configurations left pending by valid_kernel.
+__global__ void __launch_bounds__(64) valid_recheck_kernel(KinArgs<double> k64, SceneArgs<double> s64,
+                                                           const double* q, int64_t ldq, int64_t B, uint8_t* valid,
+                                                           int32_t* n_recheck) {
+  const int64_t b = gid();
+  if (b >= B || valid[b] != 2) return;
+  BatchQ qs{q, ldq, b};
+  ValidVis64 v{k64, s64};
+  run_fk(k64, qs, k64.base_type == PMP_BASE_FIXED, v);
+  valid[b] = v.collided ? 0 : 1;
+  if (n_recheck) atomicAdd(n_recheck, 1);
+}
+
+// ------------------------------------------------------------------ edges
+// Persistent kernel: each wave takes the next edge from a global counter,
+// walks its waypoints 64 at a time in the validator's visiting order and
+// stops at the first chunk holding an invalid waypoint.  The edge's answer
+// does not depend on the visiting order (every waypoint must be valid); the
+// farthest-first order only makes the first chunk the most informative.
+__device__ __forceinline__ double edge_step_ratio(const EdgeArgs& ea, int D, int64_t e) {
+  // Step ratio (ompl/motion_validator.cpp:89-126), FP64.
+  if (ea.vtype == PMP_VALIDATOR_BOX) {
+    double max_diff = -INFINITY, diff_longest = 0.0;
+    int longest = 0;
+    for (int i = 0; i < D; ++i) {
+      const double diff = ea.s2[(int64_t)i * ea.lds + e] - ea.s1[(int64_t)i * ea.lds + e];
+      const double asd = fabs(diff) * ea.inv_width[i];
+      if (asd > max_diff) {
+        max_diff = asd;
+        longest = i;
+        diff_longest = diff;
+      }
+    }
+    return fabs(diff_longest) < 1e-6 ? INFINITY : ea.width[longest] / fabs(diff_longest);
+  }
+  double d2 = 0.0;
+  for (int i = 0; i < D; ++i) {
+    const double diff = ea.s1[(int64_t)i * ea.lds + e] - ea.s2[(int64_t)i * ea.lds + e];
+    d2 += diff * diff;
+  }
+  return ea.resolution / sqrt(d2);
+}
+
+// Waypoint k (0-based) of an edge with n_test states: k = 0 is s2 itself,
+// k >= 1 is interior position k + 1 of the farthest-first sequence when
+// n_test <= 128 and position k of the linear sweep otherwise
+// (ompl/motion_validator.cpp:44-78).
+__device__ __forceinline__ EdgeQ edge_waypoint(const EdgeArgs& ea, int64_t e, int64_t n_test, double ratio,
+                                               int64_t k) {
+  double t = 0.0;
+  if (k > 0) t = n_test <= 128 ? (double)ea.seq[(n_test - 1) * 128 + (k + 1)] * ratio : (double)k * ratio;
+  return EdgeQ{ea.s1, ea.s2, ea.lds, e, t, k == 0};
+}
+
+// valid[e]: 0 invalid, 1 valid, 2 pending FP64 re-decision.
+__global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, EdgeArgs ea,
+                                                    float band) {
+  const int lane = threadIdx.x & 63;
+  const bool taylor = k32.base_type == PMP_BASE_FIXED;
+  for (;;) {
+    int e = 0;
+    if (lane == 0) e = atomicAdd(ea.counter, 1);
+    e = __shfl(e, 0, 64);
+    if (e >= ea.E) break;
+    const double ratio = edge_step_ratio(ea, k32.n_cols, e);
+    int state = 1;
+    int64_t checked = 0;
+    if (ratio != INFINITY) {
+      if (!(ratio > 0.0)) {  // NaN states: no waypoint can be placed
+        state = 0;
+      } else {
+        const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
+        const int64_t n_wp = n_test - 1;  // s2 plus the interior waypoints
+        bool unc = false;
+        for (int64_t base = 0; base < n_wp; base += 64) {
+          const int64_t k = base + lane;
+          const bool has = k < n_wp;
+          ValidVis32 v{k32, s32, band};
+          v.idle = !has;
+          run_fk(k32, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), taylor, v);
+          checked = base + 64 < n_wp ? base + 64 : n_wp;
+          unc |= has && v.uncertain;
+          if (__any(has && v.definite)) {
+            state = 0;
+            break;
+          }
+        }
+        if (state == 1 && __any(unc)) state = 2;
+      }
+    }
+    if (lane == 0) {
+      ea.valid[e] = (uint8_t)state;
+      if (ea.n_checked) ea.n_checked[e] = (int32_t)checked;
+    }
+  }
+}
+
+// FP64 walk of the edges left pending by edges_kernel, one wave per edge.
+__global__ void __launch_bounds__(64) edges_recheck_kernel(KinArgs<double> k64, SceneArgs<double> s64,
+                                                           EdgeArgs ea) {
+  const int64_t e = blockIdx.x;
+  if (e >= ea.E || ea.valid[e] != 2) return;
+  const int lane = threadIdx.x & 63;
+  const double ratio = edge_step_ratio(ea, k64.n_cols, e);
+  const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
+  const int64_t n_wp = n_test - 1;
+  bool ok = true;
+  for (int64_t base = 0; base < n_wp; base += 64) {
+    const int64_t k = base + lane;
+    const bool has = k < n_wp;
+    ValidVis64 v{k64, s64};
+    v.collided = !has;
+    run_fk(k64, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), k64.base_type == PMP_BASE_FIXED, v);
+    if (__any(has && v.collided)) {
+      ok = false;
+      break;
+    }
+  }
+  if (lane == 0) {
+    ea.valid[e] = ok ? 1 : 0;
+    if (ea.n_recheck) atomicAdd(ea.n_recheck, 1);
+  }
+}
+
+__global__ void zero_kernel(int32_t* p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------- Jacobian-carrying visitors
+// Joint frames (world axis, origin) per q column, filled while walking the
+// chain; base columns are set from the base pose.  Column j moves a point c
+// on a node whose col_mask has bit j by
+//   revolute: a_j x (c - o_j)      prismatic: a_j
+// (KinematicModel::get_attached_point_jacobian, kinematics/algorithm.cpp:346-433;
+// the base columns are the analytic form of its finite differences).
+template <int MAXC>
+struct Frames {
+  float ax[MAXC], ay[MAXC], az[MAXC], ox[MAXC], oy[MAXC], oz[MAXC];
+  __device__ __forceinline__ void set(int col, V3<float> a, V3<float> o) {
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j == col) {
+        ax[j] = a.x;
+        ay[j] = a.y;
+        az[j] = a.z;
+        ox[j] = o.x;
+        oy[j] = o.y;
+        oz[j] = o.z;
+      }
+  }
+  template <class QS>
+  __device__ __forceinline__ void set_base(const KinArgs<float>& ka, const Pose<float>& base, const QS& qs) {
+    const int n = ka.n_q_cols;
+    if (ka.base_type == PMP_BASE_PLANAR) {
+      set(n, {1.f, 0.f, 0.f}, base.t);
+      set(n + 1, {0.f, 1.f, 0.f}, base.t);
+      set(n + 2, {0.f, 0.f, 1.f}, base.t);
+    } else if (ka.base_type == PMP_BASE_FLOATING) {
+      const float pitch = (float)qs.get(n + 4), yaw = (float)qs.get(n + 5);
+      float sp, cp, sy, cy;
+      sincosf(pitch, &sp, &cp);
+      sincosf(yaw, &sy, &cy);
+      set(n, {1.f, 0.f, 0.f}, base.t);
+      set(n + 1, {0.f, 1.f, 0.f}, base.t);
+      set(n + 2, {0.f, 0.f, 1.f}, base.t);
+      set(n + 3, {cy * cp, sy * cp, -sp}, base.t);  // Rz(yaw) Ry(pitch) x
+      set(n + 4, {-sy, cy, 0.f}, base.t);           // Rz(yaw) y
+      set(n + 5, {0.f, 0.f, 1.f}, base.t);
+    }
+  }
+  __device__ __forceinline__ V3<float> col(int j, uint32_t mask, uint32_t rev, V3<float> c) const {
+    V3<float> r{0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < MAXC; ++k)
+      if (k == j && ((mask >> k) & 1u)) {
+        const V3<float> a{ax[k], ay[k], az[k]};
+        r = ((rev >> k) & 1u) ? cross(a, V3<float>{c.x - ox[k], c.y - oy[k], c.z - oz[k]}) : a;
+      }
+    return r;
+  }
+};
+
+template <int MAXC>
+struct EvalVis {
+  static constexpr bool kSelf = true;
+  const KinArgs<float>& ka;
+  const SceneArgs<float>& sc;
+  Frames<MAXC> fr;
+  float best = kCutoff;
+  int bkey = 0x7fffffff, bs = -1, bp = -1;
+  V3<float> bc{0.f, 0.f, 0.f};
+  float sbest = kCutoff;
+  int si = -1, sj = -1;
+  V3<float> sci{0.f, 0.f, 0.f}, scj{0.f, 0.f, 0.f};
+  __device__ bool alive() const { return true; }
+  template <class QS>
+  __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
+  __device__ void node(int, const DevNode<float>& nd, const Pose<float>& P) {
+    if (nd.q_col < 0) return;
+    fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
+  }
+  // evaluate_ext_collision (constraints/primitive_sphere_collision.cpp:256-322): the minimum
+  // over (group, primitive, sphere) with the reference's visiting order as tie-break key.
+  __device__ void sphere(int s, V3<float> c, float r) {
+    const DevSphereMeta m = ka.meta[s];
+    if (!m.ext) return;
+    for (int p = 0; p < sc.n_prims; ++p) {
+      const DevPrim<float>& P = prim_at(sc, p);
+      if (P.type == PMP_PRIM_CLOUD) continue;
+      if (outside_aabb(P, c, r + kCutoff)) continue;
+      const float v = sdf_eval(P, c) - r;
+      const int key = (m.group * sc.n_prims + p) * ka.n_spheres + m.out;
+      if (v < best || (bs >= 0 && v == best && key < bkey)) {
+        best = v;
+        bkey = key;
+        bs = s;
+        bp = p;
+        bc = c;
+      }
+    }
+  }
+  // evaluate_self_collision (:324-383): pairs are visited in the reference order.
+  __device__ void self(const DevSelfPair&, int i, int j, V3<float> ci, V3<float> cj, float rs) {
+    const V3<float> d = sub(ci, cj);
+    const float dist = sqrtf(dot(d, d)) - rs;
+    if (dist < sbest) {
+      sbest = dist;
+      si = i;
+      sj = j;
+      sci = ci;
+      scj = cj;
+    }
+  }
+};
+
+template <int MAXC>
+__global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
+                                                       int64_t ldq, int64_t B, float* vals, float* jac) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  EvalVis<MAXC> v{ka, sc};
+  run_fk(ka, qs, false, v);
+  if (!active) return;
+  const int D = ka.n_cols;
+  int row = 0;
+  if (sc.n_prims > 0) {
+    vals[b] = v.best;
+    V3<float> g{0.f, 0.f, 0.f};
+    uint32_t mask = 0;
+    if (v.bs >= 0) {
+      g = sdf_grad(prim_at(sc, v.bp), v.bc);
+      mask = ka.nodes[ka.meta[v.bs].node].col_mask;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j < D) jac[(int64_t)j * B + b] = dot(g, v.fr.col(j, mask, ka.rev_mask, v.bc));
+    row = 1;
+  }
+  if (ka.n_pairs > 0) {
+    vals[(int64_t)row * B + b] = v.sbest;
+    V3<float> n{0.f, 0.f, 0.f};
+    uint32_t m1 = 0, m2 = 0;
+    if (v.si >= 0) {
+      const V3<float> d = sub(v.sci, v.scj);
+      n = scale(1.0f / sqrtf(dot(d, d)), d);
+      m1 = ka.nodes[ka.meta[v.si].node].col_mask;
+      m2 = ka.nodes[ka.meta[v.sj].node].col_mask;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j < D)
+        jac[((int64_t)row * D + j) * B + b] =
+            dot(n, sub(v.fr.col(j, m1, ka.rev_mask, v.sci), v.fr.col(j, m2, ka.rev_mask, v.scj)));
+  }
+}
+
+// ------------------------------------------------ per-pair values and adjoint
+template <int MAXC>
+struct PairVis {
+  static constexpr bool kSelf = false;
+  const KinArgs<float>& ka;
+  const SceneArgs<float>& sc;
+  Frames<MAXC> fr;
+  float* dist;
+  float* grad;
+  int64_t B, b;
+  bool active;
+  __device__ bool alive() const { return true; }
+  template <class QS>
+  __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
+  __device__ void node(int, const DevNode<float>& nd, const Pose<float>& P) {
+    if (nd.q_col < 0) return;
+    fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
+  }
+  __device__ void sphere(int s, V3<float> c, float r) {
+    const DevSphereMeta m = ka.meta[s];
+    const uint32_t mask = ka.nodes[m.node].col_mask;
+    const int D = ka.n_cols, NP = sc.n_prims;
+    V3<float> J[MAXC];
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) J[j] = fr.col(j, mask, ka.rev_mask, c);
+    for (int p = 0; p < NP; ++p) {
+      const DevPrim<float>& P = prim_at(sc, p);
+      if (P.type == PMP_PRIM_CLOUD) continue;
+      const float d = sdf_eval(P, c) - r;
+      const int64_t row = (int64_t)m.out * NP + p;
+      if (active) dist[row * B + b] = d;
+      if (grad) {
+        const V3<float> g = sdf_grad(P, c);
+#pragma unroll
+        for (int j = 0; j < MAXC; ++j)
+          if (j < D && active) grad[(row * D + j) * B + b] = dot(g, J[j]);
+      }
+    }
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
+};
+
+template <int MAXC>
+__global__ void __launch_bounds__(256) pair_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
+                                                   int64_t ldq, int64_t B, float* dist, float* grad) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  PairVis<MAXC> v{ka, sc, {}, dist, grad, B, b, active};
+  run_fk(ka, qs, false, v);
+}
+
+// Reverse mode: per sphere the upstream weights of its primitive distances
+// are folded into one world-space force g = sum_p w_p grad sdf_p(c); each
+// q column then receives g . (a_j x (c - o_j)) (or g . a_j).
+template <int MAXC>
+struct VjpVis {
+  static constexpr bool kSelf = false;
+  const KinArgs<float>& ka;
+  const SceneArgs<float>& sc;
+  Frames<MAXC> fr;
+  const float* gd;
+  int64_t B, b;
+  float acc[MAXC];
+  __device__ bool alive() const { return true; }
+  template <class QS>
+  __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
+  __device__ void node(int, const DevNode<float>& nd, const Pose<float>& P) {
+    if (nd.q_col < 0) return;
+    fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
+  }
+  __device__ void sphere(int s, V3<float> c, float) {
+    const DevSphereMeta m = ka.meta[s];
+    const uint32_t mask = ka.nodes[m.node].col_mask;
+    const int NP = sc.n_prims;
+    V3<float> g{0.f, 0.f, 0.f};
+    for (int p = 0; p < NP; ++p) {
+      const DevPrim<float>& P = prim_at(sc, p);
+      if (P.type == PMP_PRIM_CLOUD) continue;
+      const float w = gd[((int64_t)m.out * NP + p) * B + b];
+      const V3<float> gp = sdf_grad(P, c);
+      g = add(g, scale(w, gp));
+    }
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) acc[j] += dot(g, fr.col(j, mask, ka.rev_mask, c));
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
+};
+
+template <int MAXC>
+__global__ void __launch_bounds__(256) vjp_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
+                                                  int64_t ldq, int64_t B, const float* gd, float* grad_q) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  const int64_t bb = active ? b : B - 1;
+  BatchQ qs{q, ldq, bb};
+  VjpVis<MAXC> v{ka, sc, {}, gd, B, bb};
+#pragma unroll
+  for (int j = 0; j < MAXC; ++j) v.acc[j] = 0.f;
+  run_fk(ka, qs, false, v);
+  if (!active) return;
+#pragma unroll
+  for (int j = 0; j < MAXC; ++j)
+    if (j < ka.n_cols) grad_q[(int64_t)j * B + b] = v.acc[j];
+}
+
+// ---------------------------------------------------- point queries (FP64)
+// SDF of the union of `n` primitives at points (UnionSDF::evaluate: min;
+// UnionSDF::is_outside: all outside), collision/primitive_sdf.hpp:155-170.
+__global__ void __launch_bounds__(256) prims_eval_kernel(const DevPrim<double>* prims, int n,
+                                                         const double* cloud_xyz, const double* pts, int64_t N,
+                                                         int mode, double radius, double* out) {
+  const int64_t i = gid();
+  if (i >= N) return;
+  const V3<double> c{pts[i], pts[N + i], pts[2 * N + i]};
+  double val = DBL_MAX;
+  bool outside = true;
+  for (int k = 0; k < n; ++k) {
+    const DevPrim<double>& P = prims[k];
+    double d;
+    if (P.type == PMP_PRIM_CLOUD) {
+      double best = DBL_MAX;
+      for (int j = 0; j < P.cloud_count; ++j) {
+        const double* y = cloud_xyz + 4 * (int64_t)(P.cloud_begin + j);
+        const double dx = y[0] - c.x, dy = y[1] - c.y, dz = y[2] - c.z;
+        const double s = dx * dx + dy * dy + dz * dz;
+        if (s < best) best = s;
+      }
+      d = sqrt(best) - P.p0;
+      if (!(d > radius)) outside = false;
+    } else {
+      d = sdf_eval(P, c);
+      if (!is_outside_ref(P, c, radius)) outside = false;
+    }
+    val = d < val ? d : val;
+  }
+  out[i] = mode == 0 ? val : (outside ? 1.0 : 0.0);
+}
+
+// Exact nearest neighbour, brute force with LDS tiles (KDTree::query / sqdist).
+__global__ void __launch_bounds__(256) nearest_kernel(const double* pts, int64_t N, const double* qry, int64_t M,
+                                                      double* sqd, int32_t* idx) {
+  __shared__ double tile[3][256];
+  const int64_t i = gid();
+  const bool active = i < M;
+  V3<double> c{0, 0, 0};
+  if (active) c = {qry[i], qry[M + i], qry[2 * M + i]};
+  double best = DBL_MAX;
+  int32_t bi = -1;
+  for (int64_t base = 0; base < N; base += 256) {
+    const int64_t j = base + threadIdx.x;
+    __syncthreads();
+    if (j < N) {
+      tile[0][threadIdx.x] = pts[j];
+      tile[1][threadIdx.x] = pts[N + j];
+      tile[2][threadIdx.x] = pts[2 * N + j];
+    }
+    __syncthreads();
+    const int cnt = (int)(N - base < 256 ? N - base : 256);
+    for (int k = 0; k < cnt; ++k) {
+      const double dx = tile[0][k] - c.x, dy = tile[1][k] - c.y, dz = tile[2][k] - c.z;
+      const double s = dx * dx + dy * dy + dz * dz;
+      if (s < best) {
+        best = s;
+        bi = (int32_t)(base + k);
+      }
+    }
+  }
+  if (active) {
+    sqd[i] = best;
+    idx[i] = bi;
+  }
+}
+
+// ------------------------------------------------------------ launchers
+static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int taylor, float* out,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(centers_kernel<float>, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, q, ldq, B, taylor, out);
+  return hipGetLastError();
+}
+hipError_t launch_poses(const KinArgs<double>& ka, const double* q, int64_t ldq, int64_t B, int taylor, double* out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(poses_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, q, ldq, B, taylor, out);
+  return hipGetLastError();
+}
+hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                            int64_t B, int taylor, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(clearance_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, taylor, out);
+  return hipGetLastError();
+}
+hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
+                        const SceneArgs<double>& s64, const double* q, int64_t ldq, int64_t B, float band,
+                        uint8_t* valid, int32_t* n_recheck, int block, hipStream_t st) {
+  if (block <= 0) block = 256;
+  hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), 0, st, k32, s32, q, ldq, B, band, valid);
+  hipLaunchKernelGGL(valid_recheck_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, k64, s64, q, ldq, B, valid,
+                     n_recheck);
+  return hipGetLastError();
+}
+hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
+                        const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ea.counter, 2);
+  hipLaunchKernelGGL(edges_kernel, dim3(grid), dim3(block), 0, st, k32, s32, ea, band);
+  hipLaunchKernelGGL(edges_recheck_kernel, dim3((unsigned)ea.E), dim3(64), 0, st, k64, s64, ea);
+  return hipGetLastError();
+}
+
+#define PMP_DISPATCH_MAXC(ncols, KERNEL, ...)                                                 \
+  do {                                                                                        \
+    if ((ncols) <= 8)                                                                         \
+      hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__);                                             \
+    else if ((ncols) <= 16)                                                                   \
+      hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__);                                            \
+    else                                                                                      \
+      hipLaunchKernelGGL(KERNEL<32>, __VA_ARGS__);                                            \
+  } while (0)
+
+hipError_t launch_evaluate(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                           int64_t B, float* vals, float* jac, hipStream_t st) {
+  PMP_DISPATCH_MAXC(ka.n_cols, evaluate_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, vals,
+                    jac);
+  return hipGetLastError();
+}
+hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                        int64_t B, float* dist, float* grad, hipStream_t st) {
+  PMP_DISPATCH_MAXC(ka.n_cols, pair_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, dist,
+                    grad);
+  return hipGetLastError();
+}
+hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
+                      const float* gd, float* grad_q, hipStream_t st) {
+  PMP_DISPATCH_MAXC(ka.n_cols, vjp_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, gd,
+                    grad_q);
+  return hipGetLastError();
+}
+hipError_t launch_prims_eval(const DevPrim<double>* prims, int n, const double* cloud_xyz, const double* pts,
+                             int64_t N, int mode, double radius, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(prims_eval_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, prims, n, cloud_xyz, pts, N,
+                     mode, radius, out);
+  return hipGetLastError();
+}
+hipError_t launch_nearest(const double* pts, int64_t N, const double* qry, int64_t M, double* sqd, int32_t* idx,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(nearest_kernel, dim3(blocks_for(M, 256)), dim3(256), 0, st, pts, N, qry, M, sqd, idx);
+  return hipGetLastError();
+}
+
+}  // namespace pmp

@@ -1,0 +1,49 @@
+// Host-side launchers of the kernels in kernels.hip (and cloud.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "geom.hpp"
+#include "tables.h"
+
+namespace pmp {
+
+struct EdgeArgs {
+  const double* s1;
+  const double* s2;
+  int64_t lds, E;
+  int32_t vtype;
+  double resolution;
+  const double* width;      // device [n_cols]
+  const double* inv_width;  // device [n_cols], 1 / width as the reference precomputes
+  const uint8_t* seq;       // device [128 * 128] farthest-first sequence table
+  int32_t* counter;         // device work counter, zeroed before each launch
+  uint8_t* valid;
+  int32_t* n_checked;
+  int32_t* n_recheck;
+};
+
+hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int taylor, float* out,
+                          hipStream_t st);
+hipError_t launch_poses(const KinArgs<double>& ka, const double* q, int64_t ldq, int64_t B, int taylor, double* out,
+                        hipStream_t st);
+hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                            int64_t B, int taylor, float* out, hipStream_t st);
+hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
+                        const SceneArgs<double>& s64, const double* q, int64_t ldq, int64_t B, float band,
+                        uint8_t* valid, int32_t* n_recheck, int block, hipStream_t st);
+hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
+                        const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
+                        hipStream_t st);
+hipError_t launch_evaluate(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                           int64_t B, float* vals, float* jac, hipStream_t st);
+hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                        int64_t B, float* dist, float* grad, hipStream_t st);
+hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
+                      const float* gd, float* grad_q, hipStream_t st);
+hipError_t launch_prims_eval(const DevPrim<double>* prims, int n, const double* cloud_xyz, const double* pts,
+                             int64_t N, int mode, double radius, double* out, hipStream_t st);
+hipError_t launch_nearest(const double* pts, int64_t N, const double* qry, int64_t M, double* sqd, int32_t* idx,
+                          hipStream_t st);
+
+}  // namespace pmp

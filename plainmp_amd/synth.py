@@ -1,0 +1,377 @@
+"""Seeded synthetic robots, sphere tables and scenes for the benchmark workloads.
+
+There is no network here, so the robot models and scenes are generated:
+
+* ``panda_like``: 7-revolute serial arm with Panda-style +-pi/2 origin
+  rotations, z joint axes, joint limits +-2.9 rad and link offsets drawn in
+  [0.08, 0.33] m; 60 collision spheres on 8 links.
+* ``mobile_torso``: prismatic torso (0-0.4 m, identity origin rotation) under
+  the same 7R arm; 90 spheres.
+* ``dual_arm``: two 7R arms on a fixed torso; 160 spheres; self-collision
+  link pairs whose sphere-pair counts sum to 480.
+
+Every generator takes a seed and uses ``numpy.random.default_rng``.
+Primitives are plain dicts ``{"type": "box"|"sphere"|"cylinder"|"ground",
+"width"|"radius"|"height", "position", "rotation"}`` so that both the
+product (``plainmp_amd.psdf``) and the test oracle can be built from them.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+HALF_PI = math.pi / 2
+
+
+# --------------------------------------------------------------- URDF text
+def _joint(name, jtype, parent, child, xyz, rpy, axis=(0, 0, 1), lower=-2.9, upper=2.9):
+    s = f'  <joint name="{name}" type="{jtype}">\n'
+    s += f'    <parent link="{parent}"/>\n    <child link="{child}"/>\n'
+    s += '    <origin xyz="{:.17g} {:.17g} {:.17g}" rpy="{:.17g} {:.17g} {:.17g}"/>\n'.format(*xyz, *rpy)
+    if jtype != "fixed":
+        s += '    <axis xyz="{:.17g} {:.17g} {:.17g}"/>\n'.format(*axis)
+    if jtype in ("revolute", "prismatic"):
+        s += f'    <limit lower="{lower:.17g}" upper="{upper:.17g}" effort="87" velocity="2.175"/>\n'
+    s += "  </joint>\n"
+    return s
+
+
+def _link(name, mass=1.0):
+    return (
+        f'  <link name="{name}">\n    <inertial>\n      <origin xyz="0 0 0.05" rpy="0 0 0"/>\n'
+        f'      <mass value="{mass}"/>\n'
+        '      <inertia ixx="0.01" ixy="0" ixz="0" iyy="0.01" iyz="0" izz="0.01"/>\n'
+        "    </inertial>\n  </link>\n"
+    )
+
+
+def _arm_chain(rng, prefix: str, root: str, mount_xyz, mount_rpy, mount_type="fixed", mount_limits=None):
+    """7R Panda-style chain.  Returns (links, joints_xml, seg) where seg[link]
+    is the offset (in that link's frame) of the next joint, used to place spheres."""
+    L1, L3, L5 = rng.uniform(0.08, 0.33, size=3)
+    a4, a7, L8 = rng.uniform(0.08, 0.12, size=3)
+    spec = [  # (xyz, rpy) of joint i, i = 1..7, then flange and hand
+        ((0.0, 0.0, L1), (0.0, 0.0, 0.0)),
+        ((0.0, 0.0, 0.0), (-HALF_PI, 0.0, 0.0)),
+        ((0.0, -L3, 0.0), (HALF_PI, 0.0, 0.0)),
+        ((a4, 0.0, 0.0), (HALF_PI, 0.0, 0.0)),
+        ((-a4, L5, 0.0), (-HALF_PI, 0.0, 0.0)),
+        ((0.0, 0.0, 0.0), (HALF_PI, 0.0, 0.0)),
+        ((a7, 0.0, 0.0), (HALF_PI, 0.0, 0.0)),
+    ]
+    links = [f"{prefix}link{i}" for i in range(1, 8)] + [f"{prefix}flange", f"{prefix}hand"]
+    xml = ""
+    parent = root
+    for i, (xyz, rpy) in enumerate(spec):
+        xml += _joint(f"{prefix}joint{i + 1}", "revolute", parent, links[i], xyz, rpy)
+        parent = links[i]
+    xml += _joint(f"{prefix}flange_joint", "fixed", links[6], links[7], (0, 0, L8), (0, 0, 0))
+    xml += _joint(f"{prefix}hand_joint", "fixed", links[7], links[8], (0, 0, 0), (0, 0, -math.pi / 4))
+    seg = {}
+    for i in range(6):
+        seg[links[i]] = np.array(spec[i + 1][0], dtype=float)
+    seg[links[6]] = np.array([0.0, 0.0, L8])
+    seg[links[8]] = np.array([0.0, 0.0, 0.10])
+    joint_names = [f"{prefix}joint{i}" for i in range(1, 8)]
+    return links, xml, seg, joint_names
+
+
+def _spheres_on(rng, link: str, seg: np.ndarray, k: int, rmin=0.03, rmax=0.10):
+    t = rng.uniform(0.0, 1.0, size=(k, 1))
+    pos = t * seg[None, :] + rng.normal(0.0, 0.02, size=(k, 3))
+    radii = rng.uniform(rmin, rmax, size=k)
+    return (link, pos, radii, False)
+
+
+@dataclass
+class Robot:
+    urdf: str
+    joint_names: List[str]
+    sphere_specs: List[Tuple[str, np.ndarray, np.ndarray, bool]]
+    self_pairs: List[Tuple[str, str]] = field(default_factory=list)
+    lower: Optional[np.ndarray] = None
+    upper: Optional[np.ndarray] = None
+
+    @property
+    def dof(self) -> int:
+        return len(self.joint_names)
+
+    @property
+    def n_spheres(self) -> int:
+        return int(sum(len(s[2]) for s in self.sphere_specs))
+
+
+def panda_like(seed: int = 0, n_spheres: int = 60) -> Robot:
+    rng = np.random.default_rng(1000 + seed)
+    links, xml, seg, joints = _arm_chain(rng, "", "base_link", None, None)
+    urdf = '<?xml version="1.0"?>\n<robot name="panda_like">\n' + _link("base_link")
+    for ln in links:
+        urdf += _link(ln)
+    urdf += xml + "</robot>\n"
+    carriers = links[:7] + [links[8]]
+    counts = np.full(8, n_spheres // 8)
+    counts[: n_spheres - counts.sum()] += 1
+    specs = [_spheres_on(rng, ln, seg[ln], int(c)) for ln, c in zip(carriers, counts)]
+    return Robot(urdf, joints, specs, [], np.full(7, -2.9), np.full(7, 2.9))
+
+
+def mobile_torso(seed: int = 0) -> Robot:
+    rng = np.random.default_rng(2000 + seed)
+    links, xml, seg, joints = _arm_chain(rng, "", "torso_link", None, None)
+    urdf = '<?xml version="1.0"?>\n<robot name="mobile_torso">\n' + _link("base_link") + _link("torso_link")
+    for ln in links:
+        urdf += _link(ln)
+    urdf += _joint("torso_lift_joint", "prismatic", "base_link", "torso_link", (0.0, 0.0, 0.4), (0, 0, 0),
+                   axis=(0, 0, 1), lower=0.0, upper=0.4)
+    urdf += xml + "</robot>\n"
+    carriers = links[:7] + [links[8]]
+    specs = [_spheres_on(rng, "torso_link", np.array([0.0, 0.0, -0.3]), 10)]
+    for ln in carriers:
+        specs.append(_spheres_on(rng, ln, seg[ln], 10))
+    lower = np.concatenate([[0.0], np.full(7, -2.9)])
+    upper = np.concatenate([[0.4], np.full(7, 2.9)])
+    return Robot(urdf, ["torso_lift_joint"] + joints, specs, [], lower, upper)
+
+
+def dual_arm(seed: int = 0) -> Robot:
+    rng = np.random.default_rng(3000 + seed)
+    urdf = '<?xml version="1.0"?>\n<robot name="dual_arm">\n' + _link("base_link") + _link("torso_link")
+    urdf += _joint("torso_fixed", "fixed", "base_link", "torso_link", (0.0, 0.0, 0.5), (0, 0, 0))
+    joints: List[str] = []
+    specs = []
+    counts = [10, 10, 10, 10, 10, 10, 12, 8]  # link1..link7, hand
+    for side, y, roll in (("l_", 0.25, -HALF_PI), ("r_", -0.25, HALF_PI)):
+        mount = f"{side}mount"
+        urdf += _link(mount)
+        urdf += _joint(f"{side}mount_joint", "fixed", "torso_link", mount, (0.0, y, 0.3), (roll, 0.0, 0.0))
+        links, xml, seg, jn = _arm_chain(rng, side, mount, None, None)
+        for ln in links:
+            urdf += _link(ln)
+        urdf += xml
+        joints += jn
+        carriers = links[:7] + [links[8]]
+        for ln, c in zip(carriers, counts):
+            specs.append(_spheres_on(rng, ln, seg[ln], c))
+    urdf += "</robot>\n"
+    pairs = [
+        ("l_hand", "r_hand"),  # 8 x 8
+        ("l_link7", "r_link7"),  # 12 x 12
+        ("l_link6", "r_hand"),  # 10 x 8
+        ("l_hand", "r_link7"),  # 8 x 12
+        ("l_link7", "r_hand"),  # 12 x 8
+    ]
+    return Robot(urdf, joints, specs, pairs, np.full(14, -2.9), np.full(14, 2.9))
+
+
+def self_pair_count(robot: Robot) -> int:
+    sizes = {s[0]: len(s[2]) for s in robot.sphere_specs}
+    return int(sum(sizes[a] * sizes[b] for a, b in robot.self_pairs))
+
+
+# ------------------------------------------------------------------ scenes
+def _rot_z(yaw):
+    c, s = math.cos(yaw), math.sin(yaw)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _rot_rpy(r, p, y):
+    cr, sr, cp, sp, cy, sy = math.cos(r), math.sin(r), math.cos(p), math.sin(p), math.cos(y), math.sin(y)
+    Rx = np.array([[1, 0, 0], [0, cr, -sr], [0, sr, cr]])
+    Ry = np.array([[cp, 0, sp], [0, 1, 0], [-sp, 0, cp]])
+    Rz = np.array([[cy, -sy, 0], [sy, cy, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def two_bars(seed: int = 0) -> List[Dict]:
+    """Two axis-aligned 0.05 x 0.05 x 1.0 m bars at seeded positions in a 1 m cube."""
+    rng = np.random.default_rng(4000 + seed)
+    prims = []
+    while len(prims) < 2:
+        c = np.array([rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.6), rng.uniform(0.3, 0.7)])
+        if np.hypot(c[0], c[1]) < 0.3:
+            continue
+        prims.append({"type": "box", "width": np.array([0.05, 0.05, 1.0]), "position": c, "rotation": np.eye(3)})
+    return prims
+
+
+def mixed_primitives(n: int, seed: int = 1) -> List[Dict]:
+    """n primitives, about a third each of boxes (sides U[0.05,0.4]; half with
+    a drawn yaw), spheres (r U[0.05,0.2]) and cylinders (r U[0.03,0.15],
+    h U[0.2,1.0]; half with a full random rotation), centres in the 1 m cube
+    x in [0.2, 1.2], y in [-0.5, 0.5], z in [0, 1], none within 0.45 m of the
+    base axis (about 60% of random 0.05 rad box-validated edges are then valid)."""
+    rng = np.random.default_rng(5000 + seed)
+    prims = []
+    kinds = ["box", "sphere", "cylinder"]
+    while len(prims) < n:
+        kind = kinds[len(prims) % 3]
+        c = np.array([rng.uniform(0.2, 1.2), rng.uniform(-0.5, 0.5), rng.uniform(0.0, 1.0)])
+        if np.hypot(c[0], c[1]) < 0.45:
+            continue
+        if kind == "box":
+            w = rng.uniform(0.05, 0.4, size=3)
+            R = _rot_z(rng.uniform(0, math.pi)) if rng.uniform() < 0.5 else np.eye(3)
+            prims.append({"type": "box", "width": w, "position": c, "rotation": R})
+        elif kind == "sphere":
+            prims.append({"type": "sphere", "radius": rng.uniform(0.05, 0.2), "position": c, "rotation": np.eye(3)})
+        else:
+            r = rng.uniform(0.03, 0.15)
+            h = rng.uniform(0.2, 1.0)
+            if rng.uniform() < 0.5:
+                R = _rot_rpy(*rng.uniform(-math.pi, math.pi, size=3))
+            else:
+                R = np.eye(3)
+            prims.append({"type": "cylinder", "radius": r, "height": h, "position": c, "rotation": R})
+    return prims
+
+
+def shelf_cloud(n_points: int = 65536, seed: int = 2) -> np.ndarray:
+    """Points uniform on a table and two shelf planes (~1 x 1 m each) with
+    N(0, 2 mm) noise; returns (n, 3) float64."""
+    rng = np.random.default_rng(6000 + seed)
+    planes = [  # (x range, y range, z)
+        ((0.35, 1.35), (-0.5, 0.5), 0.55),
+        ((0.45, 1.45), (-0.5, 0.5), 0.95),
+        ((0.45, 1.45), (-0.5, 0.5), 1.35),
+    ]
+    per = np.full(3, n_points // 3)
+    per[: n_points - per.sum()] += 1
+    out = []
+    for (xr, yr, z), k in zip(planes, per):
+        p = np.stack([rng.uniform(*xr, size=k), rng.uniform(*yr, size=k), np.full(k, z)], axis=1)
+        out.append(p)
+    pts = np.concatenate(out, axis=0) + rng.normal(0.0, 0.002, size=(n_points, 3))
+    return pts
+
+
+# --------------------------------------------------------- configurations
+def uniform_configs(robot: Robot, n: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return rng.uniform(robot.lower, robot.upper, size=(n, robot.dof))
+
+
+def uniform_edges(robot: Robot, n: int, seed: int) -> Tuple[np.ndarray, np.ndarray]:
+    rng = np.random.default_rng(seed)
+    s1 = rng.uniform(robot.lower, robot.upper, size=(n, robot.dof))
+    s2 = rng.uniform(robot.lower, robot.upper, size=(n, robot.dof))
+    return s1, s2
+
+
+def uneven_trajectories(robot: Robot, n: int, seed: int, tmin: int = 20, tmax: int = 100):
+    """n trajectories with lengths U{tmin..tmax}: straight lines between two
+    uniform configurations plus a small seeded wiggle.  Returns (offsets, X)
+    where X is (sum T, dof) float64 and trajectory i spans rows
+    offsets[i]:offsets[i+1]."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(tmin, tmax + 1, size=n)
+    offsets = np.zeros(n + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(lens)
+    X = np.empty((int(offsets[-1]), robot.dof))
+    for i, T in enumerate(lens):
+        a = rng.uniform(robot.lower, robot.upper)
+        b = rng.uniform(robot.lower, robot.upper)
+        t = np.linspace(0.0, 1.0, T)[:, None]
+        X[offsets[i]: offsets[i + 1]] = a + t * (b - a) + rng.normal(0.0, 0.02, size=(T, robot.dof))
+    return offsets, X
+
+
+# ------------------------------------------------------- named workloads
+@dataclass
+class Workload:
+    name: str
+    robot: Robot
+    prims: List[Dict]
+    cloud: Optional[np.ndarray] = None
+    cloud_radius: float = 0.002
+    Q: Optional[np.ndarray] = None
+    S1: Optional[np.ndarray] = None
+    S2: Optional[np.ndarray] = None
+    box_width: Optional[np.ndarray] = None
+    traj_offsets: Optional[np.ndarray] = None
+    traj_X: Optional[np.ndarray] = None
+
+
+def workload(name: str, scale: float = 1.0) -> Workload:
+    """The five benchmark workloads; ``scale`` shrinks the batch sizes (not the
+    robot or scene) for quick parity tests."""
+
+    def sz(n):
+        return max(1, int(round(n * scale)))
+
+    if name == "plain_two_bars":  # 7R, 60 spheres, 2 bars, 1,024 configs (seed 0)
+        r = panda_like(0)
+        return Workload(name, r, two_bars(0), Q=uniform_configs(r, sz(1024), seed=0))
+    if name == "edges_mixed16":  # 7R, 16 mixed primitives, 4,096 edges, box validator 0.05 rad
+        r = panda_like(0)
+        s1, s2 = uniform_edges(r, sz(4096), seed=1)
+        return Workload(name, r, mixed_primitives(16, seed=1), S1=s1, S2=s2, box_width=np.full(7, 0.05))
+    if name == "cloud_shelves":  # 8-DOF, 90 spheres, 65,536-point cloud, 16,384 configs
+        r = mobile_torso(0)
+        return Workload(name, r, [], cloud=shelf_cloud(65536, seed=2), Q=uniform_configs(r, sz(16384), seed=3))
+    if name == "grad_mixed32":  # 7R, 32 primitives, 8,192 configs, exact FK
+        r = panda_like(0)
+        return Workload(name, r, mixed_primitives(32, seed=4), Q=uniform_configs(r, sz(8192), seed=4))
+    if name == "traj_dual_arm":  # 14R, 160 spheres, 480 self pairs, 24 primitives, 2,048 trajectories
+        r = dual_arm(0)
+        off, X = uneven_trajectories(r, sz(2048), seed=5)
+        return Workload(name, r, mixed_primitives(24, seed=5), traj_offsets=off, traj_X=X)
+    raise KeyError(name)
+
+
+WORKLOADS = ("plain_two_bars", "edges_mixed16", "cloud_shelves", "grad_mixed32", "traj_dual_arm")
+
+
+# ------------------------------------------------------ product objects
+def build_sdf(p):
+    """A plainmp_amd.psdf object from a primitive dict."""
+    from . import psdf
+
+    t = p["type"]
+    if t == "ground":
+        return psdf.GroundSDF(float(p["height"]))
+    pose = psdf.Pose(np.asarray(p["position"], float), np.asarray(p["rotation"], float))
+    if t == "box":
+        return psdf.BoxSDF(np.asarray(p["width"], float), pose)
+    if t == "sphere":
+        return psdf.SphereSDF(float(p["radius"]), pose)
+    if t == "cylinder":
+        return psdf.CylinderSDF(float(p["radius"]), float(p["height"]), pose)
+    raise KeyError(t)
+
+
+def build_constraint(wl: Workload, reorder: bool = True, base_type=None, fixed_prims=()):
+    """SphereCollisionCst of a workload, with its scene set."""
+    from . import psdf
+    from .constraint import SphereAttachmentSpec, SphereCollisionCst
+    from .kinematics import BaseType, KinematicModel
+
+    if base_type is None:
+        base_type = BaseType.FIXED
+    kin = KinematicModel(wl.robot.urdf)
+    specs = [SphereAttachmentSpec(ln, np.asarray(pos, float).T, np.asarray(r, float), bool(only))
+             for ln, pos, r, only in wl.robot.sphere_specs]
+    fixed = psdf.UnionSDF([build_sdf(p) for p in fixed_prims]) if fixed_prims else None
+    cst = SphereCollisionCst(kin, wl.robot.joint_names, base_type, specs, list(wl.robot.self_pairs), fixed, reorder)
+    sdfs = [build_sdf(p) for p in wl.prims]
+    if wl.cloud is not None:
+        sdfs.append(psdf.CloudSDF(np.asarray(wl.cloud, float), float(wl.cloud_radius)))
+    if sdfs:
+        cst.set_sdf(psdf.UnionSDF(sdfs))
+    return cst
+
+
+def edge_waypoint_counts(S1: np.ndarray, S2: np.ndarray, width: np.ndarray) -> np.ndarray:
+    """States a box validator checks per edge: n_test - 1 with
+    n_test = floor(1 / ratio) + 2, 0 for an infinite ratio (motion_validator.cpp:17-111)."""
+    diff = S2 - S1
+    scaled = np.abs(diff) * (1.0 / np.asarray(width))
+    longest = np.argmax(scaled, axis=1)
+    d = np.abs(diff[np.arange(len(diff)), longest])
+    with np.errstate(divide="ignore"):
+        ratio = np.where(d < 1e-6, np.inf, np.asarray(width)[longest] / np.where(d < 1e-6, 1.0, d))
+    n_test = np.where(np.isinf(ratio), 1, np.floor(1.0 / ratio).astype(np.int64) + 2)
+    return (n_test - 1).astype(np.int64)

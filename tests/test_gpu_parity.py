@@ -1,0 +1,212 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs.
+
+Bars (stated per test):
+* validity booleans: bit-exact wherever the oracle's clearance is farther than
+  the 1e-5 m band from zero; inside the band the GPU re-decides in FP64, and
+  the tests require exactness there too;
+* FP32 sphere centres / clearances: max |FP32 - FP64| <= 5e-6 m;
+* Jacobians: relative error <= 1e-4 against FP64 central differences.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import _oracle
+from plainmp_amd import synth
+from support import oracle_cst, oracle_min_margin, product_cst
+
+pytestmark = pytest.mark.gpu
+
+BAND = 1e-5
+
+
+def _report(name, **kv):
+    print(f"[{name}] " + " ".join(f"{k}={v}" for k, v in kv.items()), flush=True)
+
+
+def test_valid_plain_two_bars_full():
+    wl = synth.workload("plain_two_bars")  # 1,024 configs, seed 0
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref = oc.is_valid_batch(wl.Q, 8).astype(bool)
+    got, n_re = pc.is_valid_batch_stats(wl.Q)
+    margin = oracle_min_margin(oc, wl.Q)
+    outside = margin > BAND
+    _report("valid_plain", B=len(ref), valid_frac=ref.mean(), rechecked=n_re, in_band=int((~outside).sum()))
+    assert np.array_equal(got[outside], ref[outside])
+    assert np.array_equal(got, ref)
+
+
+def test_clearance_and_centers_plain_two_bars():
+    wl = synth.workload("plain_two_bars")
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    Q = wl.Q[:256]
+    for accurate in (False, True):
+        ref = oc.sphere_clearance_batch(Q, accurate, 8)
+        got = pc.sphere_clearance_batch(Q, accurate)
+        err = np.abs(got - ref).max()
+        cref = oc.sphere_centers_batch(Q, accurate)
+        cgot = pc.sphere_centers_batch(Q, accurate)
+        cerr = np.abs(cgot - cref).max()
+        _report("clearance", accurate=accurate, max_err=err, center_err=cerr)
+        assert err <= 5e-6
+        assert cerr <= 5e-6
+
+
+def test_edges_mixed16_subset():
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    wl = synth.workload("edges_mixed16", scale=0.125)  # 512 edges
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(wl.box_width), 0.0, 8)
+    mv = BoxMotionValidator(pc, wl.box_width)
+    got, n_got = mv.check_motions(wl.S1, wl.S2)
+    _report("edges", E=len(ref), valid_frac=ref.mean(), mean_checked_ref=n_ref.mean(), mean_checked_gpu=n_got.mean())
+    assert np.array_equal(got, ref.astype(bool))
+    # single-edge entry point agrees with the batch
+    for e in range(8):
+        assert mv.checkMotion(wl.S1[e], wl.S2[e]) == bool(ref[e])
+
+
+def test_edges_euclidean_subset():
+    from plainmp_amd.ompl import EuclideanMotionValidator
+
+    wl = synth.workload("edges_mixed16", scale=0.0625)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref, _ = oc.check_motions(wl.S1, wl.S2, False, [], 0.1, 8)
+    got, _ = EuclideanMotionValidator(pc, 0.1).check_motions(wl.S1, wl.S2)
+    assert np.array_equal(got, ref.astype(bool))
+
+
+def _second_gap(d):
+    """Gap between the two smallest entries of each row (ties make argmins ambiguous)."""
+    s = np.sort(d.reshape(len(d), -1), axis=1)
+    return s[:, 1] - s[:, 0]
+
+
+def test_evaluate_rows_grad_mixed32():
+    wl = synth.workload("grad_mixed32", scale=1 / 32)  # 256 configs
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    vref, jref = oc.evaluate_batch(wl.Q, 8)
+    vgot, jgot = pc.evaluate_batch(wl.Q)
+    assert vgot.shape == vref.shape and jgot.shape == jref.shape
+    verr = np.abs(vgot - vref).max()
+    pd = oc.pair_clearance_batch(wl.Q, True, 8)
+    clear = _second_gap(pd) > 1e-5  # unambiguous argmin pair
+    jerr = np.abs(jgot - jref)[clear].max()
+    scale = np.maximum(np.abs(jref[clear]), 1.0)
+    _report("evaluate", B=len(vref), val_err=verr, jac_err=jerr, ambiguous=int((~clear).sum()))
+    assert verr <= 5e-6
+    # reference rows use a forward-difference SDF gradient (eps 1e-6): compare at 1e-4
+    assert (np.abs(jgot - jref)[clear] <= 1e-4 * scale).all()
+
+
+def test_pair_clearance_grad_vs_central_fd():
+    wl = synth.workload("grad_mixed32", scale=1 / 128)  # 64 configs
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    dref = oc.pair_clearance_batch(wl.Q, True, 8)
+    dist, grad = pc.pair_clearance_batch(wl.Q, True)
+    assert np.abs(dist - dref).max() <= 5e-6
+    h = 1e-6
+    B, S, P, D = grad.shape
+    fd = np.empty((B, S, P, D))
+    fd2 = np.empty((B, S, P, D))
+    for j in range(D):
+        E = np.zeros(D)
+        E[j] = 1.0
+        fd[..., j] = (oc.pair_clearance_batch(wl.Q + h * E, True, 8) - oc.pair_clearance_batch(wl.Q - h * E, True, 8)) / (2 * h)
+        fd2[..., j] = (oc.pair_clearance_batch(wl.Q + 4 * h * E, True, 8) - oc.pair_clearance_batch(wl.Q - 4 * h * E, True, 8)) / (8 * h)
+    smooth = np.abs(fd - fd2) <= 1e-6  # away from SDF kinks
+    rel = np.abs(grad - fd) / np.maximum(np.abs(fd), 1e-2)
+    _report("pair_grad", entries=fd.size, smooth=int(smooth.sum()), max_rel=rel[smooth].max())
+    assert rel[smooth].max() <= 1e-4
+
+
+def test_self_collision_dual_arm():
+    wl = synth.workload("traj_dual_arm", scale=0.005)  # ~10 trajectories of waypoints
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    Q = wl.traj_X[:512]
+    ref = oc.is_valid_batch(Q, 8).astype(bool)
+    got = pc.is_valid_batch(Q)
+    _report("dual_arm", B=len(Q), valid_frac=ref.mean())
+    assert np.array_equal(got, ref)
+    vref, jref = oc.evaluate_batch(Q[:64], 8)
+    vgot, jgot = pc.evaluate_batch(Q[:64])
+    assert np.abs(vgot - vref).max() <= 5e-6
+
+
+@pytest.mark.parametrize("base", ["PLANAR", "FLOATING"])
+def test_base_types(base):
+    from plainmp_amd.kinematics import BaseType
+
+    wl = synth.workload("grad_mixed32", scale=1 / 64)
+    extra = 3 if base == "PLANAR" else 6
+    rng = np.random.default_rng(11)
+    Q = np.concatenate([wl.Q, rng.normal(0, 0.1, size=(len(wl.Q), extra))], axis=1)
+    oc = oracle_cst(wl, base_type=getattr(_oracle.BaseType, base))
+    pc = product_cst(wl, base_type=getattr(BaseType, base))
+    assert np.array_equal(pc.is_valid_batch(Q), oc.is_valid_batch(Q, 8).astype(bool))
+    vref, jref = oc.evaluate_batch(Q, 8)
+    vgot, jgot = pc.evaluate_batch(Q)
+    assert np.abs(vgot - vref).max() <= 5e-6
+    pd = oc.pair_clearance_batch(Q, True, 8)
+    clear = _second_gap(pd) > 1e-5
+    # base columns of the reference are forward differences with eps 1e-7
+    assert (np.abs(jgot - jref)[clear] <= 2e-4 * np.maximum(np.abs(jref[clear]), 1.0)).all()
+
+
+def test_single_config_api_matches_batch():
+    wl = synth.workload("plain_two_bars", scale=1 / 64)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    for q in wl.Q[:4]:
+        assert pc.is_valid(q) == oc.is_valid(q)
+        v, j = pc.evaluate(q)
+        vr, jr = oc.evaluate(q)
+        assert np.allclose(v, vr, atol=5e-6)
+        pc.update_kintree(q, False)
+        oc.update_kintree(q, False)
+        got = np.array([c for c, _ in pc.get_all_spheres()])
+        ref, _ = oc.get_all_spheres()
+        assert np.abs(got - ref).max() <= 5e-6
+        gs = np.array([c for c, _ in pc.get_group_spheres()])
+        gref, _ = oc.get_group_spheres()
+        assert np.abs(gs - gref).max() <= 5e-6
+
+
+def test_psdf_against_oracle():
+    from plainmp_amd import psdf
+    from support import oracle_prim, product_sdf
+
+    prims = synth.mixed_primitives(12, seed=7) + [{"type": "ground", "height": 0.3}]
+    rng = np.random.default_rng(3)
+    pts = rng.normal(0.5, 0.6, size=(3, 500))
+    for p in prims:
+        o, g = oracle_prim(p), product_sdf(p)
+        assert np.allclose(g.evaluate_batch(pts), o.evaluate_batch(pts), atol=1e-12, rtol=0)
+        for r in (0.0, 0.05, 0.3):
+            got = g.is_outside_batch(pts, r)
+            ref = np.array([o.is_outside(pts[:, i], r) for i in range(pts.shape[1])])
+            assert np.array_equal(got, ref)
+        assert np.allclose(g.lb, o.lb) and np.allclose(g.ub, o.ub)
+    u = psdf.UnionSDF([product_sdf(p) for p in prims[:4]])
+    ref = np.min([oracle_prim(p).evaluate_batch(pts) for p in prims[:4]], axis=0)
+    assert np.allclose(u.evaluate_batch(pts), ref, atol=1e-12, rtol=0)
+    assert abs(psdf.GroundSDF(1.0).evaluate(np.zeros(3)) - 1.0) == 0.0  # tests/test_psdf.py:201-204
+
+
+def test_kdtree_and_cloud():
+    from plainmp_amd import psdf
+    from plainmp_amd.kdtree import KDTree
+
+    rng = np.random.default_rng(5)
+    pts = rng.normal(size=(1000, 3))
+    tree = KDTree(pts)
+    for _ in range(50):
+        q = rng.normal(size=3)
+        i = np.argmin(((pts - q) ** 2).sum(1))
+        assert np.allclose(tree.query(q), pts[i])
+        assert np.isclose(tree.sqdist(q), ((pts[i] - q) ** 2).sum())
+    obst = rng.uniform(size=(100, 3))
+    cs = psdf.CloudSDF(obst, 0.1)
+    ev = rng.normal(size=(3, 300)) * 1.5
+    ref = _oracle.Prim.cloud(obst, 0.1).evaluate_batch(ev)
+    assert np.allclose(cs.evaluate_batch(ev), ref, atol=1e-12)
