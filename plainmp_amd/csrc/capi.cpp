@@ -59,10 +59,15 @@ struct DevBuf {
     }
     return ptr;
   }
+  // Synchronous: the host vectors are usually temporaries, and an
+  // asynchronous copy from pageable memory may read them after they are freed.
   template <class T>
   T* upload(const std::vector<T>& v, hipStream_t st) {
     T* p = (T*)get(std::max<size_t>(v.size() * sizeof(T), 16));
-    if (!v.empty()) PMP_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    if (!v.empty()) {
+      PMP_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+      PMP_HIP(hipStreamSynchronize(st));
+    }
     return p;
   }
   void release() {

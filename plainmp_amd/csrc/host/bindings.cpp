@@ -152,7 +152,11 @@ PYBIND11_MODULE(_native, m) {
            [](SDFBase& s, const DArr& P) {
              int64_t N;
              const double* d = points3n(P, N);
-             auto v = s.evaluate_batch(d, N);
+             std::vector<double> v;
+             {
+               py::gil_scoped_release rel;
+               v = s.evaluate_batch(d, N);
+             }
              DArr out(N);
              std::copy(v.begin(), v.end(), out.mutable_data());
              return out;
@@ -162,7 +166,11 @@ PYBIND11_MODULE(_native, m) {
            [](SDFBase& s, const DArr& P, double r) {
              int64_t N;
              const double* d = points3n(P, N);
-             auto v = s.is_outside_batch(d, N, r);
+             std::vector<uint8_t> v;
+             {
+               py::gil_scoped_release rel;
+               v = s.is_outside_batch(d, N, r);
+             }
              py::array_t<bool> out(N);
              for (int64_t i = 0; i < N; ++i) out.mutable_data()[i] = v[i] != 0;
              return out;
@@ -209,7 +217,10 @@ PYBIND11_MODULE(_native, m) {
           for (int k = 0; k < 3; ++k) soa[k * M + i] = Q.data()[3 * i + k];
         std::vector<double> sq;
         std::vector<int32_t> idx;
-        t.nearest(soa.data(), M, sq, idx);
+        {
+          py::gil_scoped_release rel;
+          t.nearest(soa.data(), M, sq, idx);
+        }
         DArr P({M, (int64_t)3}), S(M);
         for (int64_t i = 0; i < M; ++i) {
           for (int k = 0; k < 3; ++k) P.mutable_data()[3 * i + k] = t.points()[3 * idx[i] + k];
@@ -260,10 +271,20 @@ PYBIND11_MODULE(_native, m) {
       .def("update_kintree",
            [](SphereCollisionCst& c, const DArr& q, bool acc) { c.update_kintree(vecd(q), acc); }, py::arg("q"),
            py::arg("high_accuracy") = true)
-      .def("is_valid", [](SphereCollisionCst& c, const DArr& q) { return c.is_valid(vecd(q)); })
+      .def("is_valid",
+           [](SphereCollisionCst& c, const DArr& q) {
+             auto v = vecd(q);
+             py::gil_scoped_release rel;
+             return c.is_valid(v);
+           })
       .def("evaluate",
            [](SphereCollisionCst& c, const DArr& q) {
-             auto r = c.evaluate(vecd(q));
+             auto qv = vecd(q);
+             std::pair<std::vector<double>, std::vector<double>> r;
+             {
+               py::gil_scoped_release rel;
+               r = c.evaluate(qv);
+             }
              const int64_t C = (int64_t)c.cst_dim(), D = (int64_t)c.q_dim();
              DArr vals(C), jac({C, D});
              std::copy(r.first.begin(), r.first.end(), vals.mutable_data());
@@ -288,7 +309,11 @@ PYBIND11_MODULE(_native, m) {
              int64_t B;
              const double* q = rows(Q, c.q_dim(), B);
              int32_t n = 0;
-             auto v = c.is_valid_batch(q, B, &n);
+             std::vector<uint8_t> v;
+             {
+               py::gil_scoped_release rel;
+               v = c.is_valid_batch(q, B, &n);
+             }
              py::array_t<bool> out(B);
              for (int64_t i = 0; i < B; ++i) out.mutable_data()[i] = v[i] != 0;
              return out;
@@ -298,7 +323,11 @@ PYBIND11_MODULE(_native, m) {
              int64_t B;
              const double* q = rows(Q, c.q_dim(), B);
              int32_t n = 0;
-             auto v = c.is_valid_batch(q, B, &n);
+             std::vector<uint8_t> v;
+             {
+               py::gil_scoped_release rel;
+               v = c.is_valid_batch(q, B, &n);
+             }
              py::array_t<bool> out(B);
              for (int64_t i = 0; i < B; ++i) out.mutable_data()[i] = v[i] != 0;
              return std::make_pair(out, n);
@@ -308,7 +337,10 @@ PYBIND11_MODULE(_native, m) {
              int64_t B;
              const double* q = rows(Q, c.q_dim(), B);
              std::vector<double> vals, jac;
-             c.evaluate_batch(q, B, vals, jac);
+             {
+               py::gil_scoped_release rel;
+               c.evaluate_batch(q, B, vals, jac);
+             }
              const int64_t C = (int64_t)c.cst_dim(), D = (int64_t)c.q_dim();
              DArr v({B, C}), j({B, C, D});
              std::copy(vals.begin(), vals.end(), v.mutable_data());
@@ -319,7 +351,11 @@ PYBIND11_MODULE(_native, m) {
            [](SphereCollisionCst& c, const DArr& Q, bool accurate) {
              int64_t B;
              const double* q = rows(Q, c.q_dim(), B);
-             auto v = c.sphere_clearance_batch(q, B, accurate);
+             std::vector<float> v;
+             {
+               py::gil_scoped_release rel;
+               v = c.sphere_clearance_batch(q, B, accurate);
+             }
              py::array_t<float> out({B, (int64_t)c.n_spheres()});
              std::copy(v.begin(), v.end(), out.mutable_data());
              return out;
@@ -329,7 +365,11 @@ PYBIND11_MODULE(_native, m) {
            [](SphereCollisionCst& c, const DArr& Q, bool accurate) {
              int64_t B;
              const double* q = rows(Q, c.q_dim(), B);
-             auto v = c.sphere_centers_batch(q, B, accurate);
+             std::vector<float> v;
+             {
+               py::gil_scoped_release rel;
+               v = c.sphere_centers_batch(q, B, accurate);
+             }
              py::array_t<float> out({B, (int64_t)c.n_spheres(), (int64_t)3});
              std::copy(v.begin(), v.end(), out.mutable_data());
              return out;
@@ -340,7 +380,10 @@ PYBIND11_MODULE(_native, m) {
              int64_t B;
              const double* q = rows(Q, c.q_dim(), B);
              std::vector<float> d, g;
-             c.pair_clearance_batch(q, B, with_grad, d, g);
+             {
+               py::gil_scoped_release rel;
+               c.pair_clearance_batch(q, B, with_grad, d, g);
+             }
              const int64_t S = c.n_spheres(), P = c.n_prims(), D = c.q_dim();
              py::array_t<float> dist({B, S, P});
              std::copy(d.begin(), d.end(), dist.mutable_data());
@@ -415,7 +458,9 @@ PYBIND11_MODULE(_native, m) {
   py::class_<MotionValidator, std::shared_ptr<MotionValidator>>(mo, "MotionValidator")
       .def(py::init<std::shared_ptr<SphereCollisionCst>, const ValidatorConfig&>())
       .def("checkMotion", [](MotionValidator& v, const DArr& s1, const DArr& s2) {
-        return v.checkMotion(vecd(s1), vecd(s2));
+        auto a = vecd(s1), b = vecd(s2);
+        py::gil_scoped_release rel;
+        return v.checkMotion(a, b);
       })
       .def("check_motions",
            [](MotionValidator& v, const DArr& S1, const DArr& S2) {
@@ -424,7 +469,11 @@ PYBIND11_MODULE(_native, m) {
              const double* b = rows(S2, v.constraint()->q_dim(), E2);
              if (E != E2) throw std::invalid_argument("S1 and S2 differ in length");
              std::vector<int32_t> n;
-             auto ok = v.check_motions(a, b, E, &n);
+             std::vector<uint8_t> ok;
+             {
+               py::gil_scoped_release rel;
+               ok = v.check_motions(a, b, E, &n);
+             }
              py::array_t<bool> out(E);
              py::array_t<int32_t> nc(E);
              for (int64_t i = 0; i < E; ++i) {

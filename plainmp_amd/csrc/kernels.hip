@@ -13,6 +13,9 @@
 namespace pmp {
 
 constexpr float kCutoff = 0.1f;  // SphereCollisionCst::cutoff_dist_ (primitive_sphere_collision.hpp:115)
+// Edges are walked 64 states at a time; an edge needing more states than this
+// (a step ratio below 2^-24) is reported invalid with n_checked = -1 instead.
+constexpr double kMaxEdgeStates = 16777216.0;
 
 template <class T>
 __device__ __forceinline__ const DevPrim<T>& prim_at(const SceneArgs<T>& sc, int p) {
@@ -259,16 +262,18 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
   const int lane = threadIdx.x & 63;
   const bool taylor = k32.base_type == PMP_BASE_FIXED;
   for (;;) {
-    int e = 0;
-    if (lane == 0) e = atomicAdd(ea.counter, 1);
-    e = __shfl(e, 0, 64);
+    // Every lane takes part in the atomic (lane 0 adds 1, the others 0) and
+    // the edge index is read from the first lane into a scalar register, so
+    // the index and every branch on it are wave-uniform by construction.
+    const int e = __builtin_amdgcn_readfirstlane(atomicAdd(ea.counter, lane == 0 ? 1 : 0));
     if (e >= ea.E) break;
     const double ratio = edge_step_ratio(ea, k32.n_cols, e);
     int state = 1;
     int64_t checked = 0;
     if (ratio != INFINITY) {
-      if (!(ratio > 0.0)) {  // NaN states: no waypoint can be placed
+      if (!(ratio > 0.0) || 1.0 / ratio > kMaxEdgeStates) {  // NaN states, or more states than an edge may hold
         state = 0;
+        checked = -1;
       } else {
         const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
         const int64_t n_wp = n_test - 1;  // s2 plus the interior waypoints
