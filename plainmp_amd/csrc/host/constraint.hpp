@@ -284,8 +284,11 @@ class SphereCollisionCst {
   std::vector<std::pair<Vec3, double>> get_all_spheres() { return state_spheres(false); }
   std::vector<std::pair<Vec3, double>> get_group_spheres() { return state_spheres(true); }
 
-  // Tables for a batched call (also used by validators).
+  // Tables for a batched call (also used by validators).  Host-array calls
+  // run on the null stream, ordered with their synchronous copies; calls on
+  // device pointers set the caller's stream afterwards (device_context).
   void sync_batch_tables() {
+    check(pmp_set_stream(batch_.ctx(), nullptr));
     sync_robot(batch_, control_ids_, base_type_, false, batch_robot_version_);
     sync_scene(batch_, batch_scene_sig_);
   }
@@ -307,8 +310,11 @@ class SphereCollisionCst {
   // `group_spheres` each group contributes one sphere (centre, group radius).
   void sync_robot(Device& dev, const std::vector<size_t>& q_joints, int base_type, bool group_spheres,
                   uint64_t& cached_version) {
-    const uint64_t v = kin_->version() * 4 + (group_spheres ? 2 : 0) + 1;
-    if (cached_version == v && !q_joints_changed(dev, q_joints)) return;
+    (void)cached_version;
+    std::vector<double> sig = kin_->pack_signature(q_joints, base_type == FIXED);
+    sig.push_back(group_spheres ? 1.0 : 0.0);
+    auto it = robot_sig_.find(&dev);
+    if (it != robot_sig_.end() && it->second == sig && !q_joints_changed(dev, q_joints)) return;
     PackedRobot pr;
     kin_->pack(q_joints, pr);
     std::vector<int32_t> s_node, s_group;
@@ -356,7 +362,7 @@ class SphereCollisionCst {
     d.n_self_pairs = group_spheres ? 0 : (int32_t)self_pairs_.size();
     d.self_pairs = pairs.data();
     check(pmp_upload_robot(dev.ctx(), &d));
-    cached_version = v;
+    robot_sig_[&dev] = sig;
     uploaded_q_[&dev] = q_joints;
   }
   bool q_joints_changed(Device& dev, const std::vector<size_t>& q) {
@@ -386,11 +392,7 @@ class SphereCollisionCst {
     }
     Device& dev = groups ? state_groups_ : state_;
     uint64_t& ver = groups ? state_groups_version_ : state_version_;
-    // a non-fixed base is rebuilt from q, so the model's base pose is not used there
-    std::vector<size_t> cols = control_ids_;
-    bool fresh = kin_->version() * 4 + (groups ? 2 : 0) + 1 != ver;
-    sync_robot(dev, cols, base_type_, groups, ver);
-    (void)fresh;
+    sync_robot(dev, control_ids_, base_type_, groups, ver);
     const int D = (int)q_dim();
     const size_t S = groups ? groups_.size() : n_spheres();
     std::vector<std::pair<Vec3, double>> out;
@@ -427,6 +429,7 @@ class SphereCollisionCst {
   uint64_t batch_robot_version_ = 0, state_version_ = 0, state_groups_version_ = 0;
   std::vector<uint64_t> batch_scene_sig_, state_scene_sig_;
   std::unordered_map<const Device*, std::vector<size_t>> uploaded_q_;
+  std::unordered_map<const Device*, std::vector<double>> robot_sig_;
   std::vector<double> state_q_;
   bool state_accurate_ = true;
 };

@@ -356,6 +356,20 @@ class KinematicModel {
       out.identity.push_back(ident ? 1 : 0);
     }
   }
+  // What pack() reads besides the static tree: the non-column joints' angles
+  // and modes, the base pose (fixed bases) and the link count.
+  std::vector<double> pack_signature(const std::vector<size_t>& q_joints, bool with_base) const {
+    std::vector<bool> is_col(joint_type_.size(), false);
+    for (auto j : q_joints) is_col.at(j) = true;
+    std::vector<double> sig{(double)link_names_.size()};
+    for (size_t j = 0; j < joint_type_.size(); ++j)
+      if (!is_col[j]) {
+        sig.push_back(joint_angles_[j]);
+        sig.push_back(joint_mode_[j]);
+      }
+    if (with_base) sig.insert(sig.end(), base_.begin(), base_.end());
+    return sig;
+  }
   int last_mode() const { return last_mode_; }
   const std::vector<int8_t>& joint_modes() const { return joint_mode_; }
 
