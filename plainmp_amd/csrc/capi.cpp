@@ -120,13 +120,15 @@ struct pmp_context {
   KinArgs<float> k32{};
   KinArgs<double> k64{};
   int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
+  DevBuf rad_out;  // sphere radius by group-order index (FP32)
   // scene
-  DevBuf prims32, prims64, cloud32, cloud64;
+  DevBuf prims32, prims64, tiles32, tiles64, tileB, pts64;
   SceneArgs<float> s32{};
   SceneArgs<double> s64{};
   int n_clouds = 0;
+  std::vector<CloudRange> clouds;
   // misc
-  DevBuf seq, counter, width, inv_width, tmp_prims, tmp_cloud;
+  DevBuf seq, counter, width, inv_width, tmp_prims, tmp_cloud, ws_cen, ws_cloud, ws_flags;
 };
 
 struct pmp_graph {
@@ -212,13 +214,96 @@ void pack_prims(const pmp_scene_desc& sc, std::vector<DevPrim<float>>& p32, std:
   }
 }
 
+// Interleave the low 10 bits of v into every third bit (Morton code).
+uint32_t spread10(uint32_t v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// Tiles of kCloudTile points in Morton order with their bounds, centre, the
+// matrix-core B operands in tile-local coordinates and FP64 world points.
+void tile_cloud(const pmp_prim_desc& d, std::vector<CloudTile>& t32, std::vector<CloudTile64>& t64,
+                std::vector<float>& tb, std::vector<double>& p64) {
+  const int64_t n = d.n_points;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], d.points[3 * i + k]);
+      hi[k] = std::max(hi[k], d.points[3 * i + k]);
+    }
+  std::vector<std::pair<uint32_t, int64_t>> key(n);
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t c[3];
+    for (int k = 0; k < 3; ++k) {
+      double span = hi[k] - lo[k];
+      double u = span > 0 ? (d.points[3 * i + k] - lo[k]) / span : 0.0;
+      c[k] = (uint32_t)std::min(1023.0, std::max(0.0, u * 1023.0));
+    }
+    key[i] = {spread10(c[0]) | (spread10(c[1]) << 1) | (spread10(c[2]) << 2), i};
+  }
+  std::sort(key.begin(), key.end());
+  const int64_t nt = (n + kCloudTile - 1) / kCloudTile;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t b = t * kCloudTile, e = std::min(n, b + kCloudTile);
+    CloudTile64 T64;
+    for (int k = 0; k < 3; ++k) {
+      T64.lb[k] = 1e300;
+      T64.ub[k] = -1e300;
+    }
+    for (int64_t j = b; j < e; ++j)
+      for (int k = 0; k < 3; ++k) {
+        const double v = d.points[3 * key[j].second + k];
+        T64.lb[k] = std::min(T64.lb[k], v);
+        T64.ub[k] = std::max(T64.ub[k], v);
+      }
+    CloudTile T{};
+    double o[3];
+    for (int k = 0; k < 3; ++k) {
+      o[k] = 0.5 * (T64.lb[k] + T64.ub[k]);
+      T.o[k] = (float)o[k];
+      T.lb[k] = std::nextafter((float)T64.lb[k], -INFINITY);  // conservative FP32 bounds
+      T.ub[k] = std::nextafter((float)T64.ub[k], INFINITY);
+    }
+    T.n = (int32_t)(e - b);
+    t32.push_back(T);
+    t64.push_back(T64);
+    // B operand [m][k][j] for point 16 m + j of the tile: (-2 p', |p'|^2) with p' = p - o
+    std::vector<float> blk(4 * kCloudTile, 0.f);
+    for (int jj = 0; jj < kCloudTile; ++jj) {
+      const int m = jj / 16, j = jj % 16;
+      float* col = &blk[m * 64];
+      if (b + jj < e) {
+        const double* pw = d.points + 3 * key[b + jj].second;
+        double pl[3], n2 = 0;
+        for (int k = 0; k < 3; ++k) {
+          pl[k] = (double)(float)(pw[k] - (double)T.o[k]);
+          n2 += pl[k] * pl[k];
+        }
+        for (int k = 0; k < 3; ++k) col[k * 16 + j] = (float)(-2.0 * pl[k]);
+        col[3 * 16 + j] = (float)n2;
+        p64.insert(p64.end(), {pw[0], pw[1], pw[2], 0.0});
+      } else {  // padding: never the nearest
+        for (int k = 0; k < 3; ++k) col[k * 16 + j] = 0.f;
+        col[3 * 16 + j] = 1e30f;
+        p64.insert(p64.end(), {1e30, 1e30, 1e30, 0.0});
+      }
+    }
+    tb.insert(tb.end(), blk.begin(), blk.end());
+  }
+}
+
 void require_robot(pmp_context* ctx) {
   PMP_REQUIRE(ctx != nullptr, "null context");
   PMP_REQUIRE(ctx->has_robot, "no robot tables uploaded (call pmp_upload_robot first)");
 }
 void require_no_cloud(pmp_context* ctx, const char* what) {
   if (ctx->n_clouds > 0)
-    throw Error(PMP_ERR_INVALID, std::string(what) + ": point-cloud primitives are served by the cloud entry points");
+    throw Error(PMP_ERR_INVALID, std::string(what) + ": point-cloud primitives are supported by is_valid and "
+                                                    "sphere_clearance only");
 }
 
 }  // namespace
@@ -258,9 +343,10 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
-    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->prims32,
-                      &ctx->prims64, &ctx->cloud32, &ctx->cloud64, &ctx->seq, &ctx->counter, &ctx->width,
-                      &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud})
+    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->rad_out,
+                      &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
+                      &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
+                      &ctx->ws_cloud, &ctx->ws_flags})
       b->release();
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     delete ctx;
@@ -415,8 +501,8 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
       sp64[k] = {l[0], l[1], l[2], r->sphere_radius[s]};
       const int g = r->sphere_group[s];
       const bool ext = !(r->group_only_self && r->group_only_self[g]);
+      meta[k] = {s, ext ? n_ext : -1, r->sphere_node[s], g};
       n_ext += ext;
-      meta[k] = {s, ext ? 1 : 0, r->sphere_node[s], g};
     }
     std::vector<DevNode<float>> n32(N);
     std::vector<DevNode<double>> n64(N);
@@ -479,6 +565,9 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     d.spheres = ctx->sph64.upload(sp64, s0);
     a.meta = d.meta = ctx->meta.upload(meta, s0);
     a.pairs = d.pairs = ctx->pairs.upload(pairs, s0);
+    std::vector<float> rad(std::max(S, 1), 0.f);
+    for (int s2 = 0; s2 < S; ++s2) rad[s2] = (float)r->sphere_radius[s2];
+    ctx->rad_out.upload(rad, s0);
     a.n_nodes = d.n_nodes = N;
     a.n_spheres = d.n_spheres = S;
     a.n_pairs = d.n_pairs = (int)pairs.size();
@@ -517,20 +606,39 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     std::vector<double> c64;
     int n_clouds = 0;
     pack_prims(*sc, p32, p64, c32, c64, n_clouds);
+    PMP_REQUIRE(n_clouds <= kMaxClouds, "at most 4 point clouds per scene");
+    std::vector<CloudTile> t32;
+    std::vector<CloudTile64> t64;
+    std::vector<float> tb;
+    std::vector<double> pts;
+    std::vector<CloudRange> clouds;
+    for (int i = 0; i < sc->n_prims; ++i) {
+      if (sc->prims[i].type != PMP_PRIM_CLOUD) continue;
+      const int begin = (int)t32.size(), oc = (int)clouds.size();
+      tile_cloud(sc->prims[i], t32, t64, tb, pts);
+      const int nt = (int)t32.size() - begin;
+      p32[i].cloud_begin = p64[i].cloud_begin = begin;
+      p32[i].cloud_count = p64[i].cloud_count = nt;
+      p32[i].flags |= oc << 4;
+      p64[i].flags |= oc << 4;
+      clouds.push_back({begin, nt, (float)sc->prims[i].size[0]});
+    }
     set_device(ctx);
     hipStream_t s0 = st(ctx);
     SceneArgs<float> a{};
     SceneArgs<double> d{};
     a.prims = ctx->prims32.upload(p32, s0);
     d.prims = ctx->prims64.upload(p64, s0);
-    a.cloud_xyz = ctx->cloud32.upload(c32, s0);
-    d.cloud_xyz = ctx->cloud64.upload(c64, s0);
+    a.tiles = d.tiles = ctx->tiles32.upload(t32, s0);
+    a.tiles64 = d.tiles64 = ctx->tiles64.upload(t64, s0);
+    a.tileB = d.tileB = ctx->tileB.upload(tb, s0);
+    a.pts64 = d.pts64 = ctx->pts64.upload(pts, s0);
     a.n_prims = d.n_prims = sc->n_prims;
     a.n_cloud = d.n_cloud = n_clouds;
-    PMP_HIP(hipStreamSynchronize(s0));
     ctx->s32 = a;
     ctx->s64 = d;
     ctx->n_clouds = n_clouds;
+    ctx->clouds = clouds;
   });
 }
 
@@ -577,10 +685,17 @@ pmp_status pmp_sphere_clearance_batch(pmp_context* ctx, const double* q, int64_t
   return guarded([&] {
     PMP_BATCH_CHECKS(q, ldq, B);
     PMP_REQUIRE(B == 0 || clearance, "null output");
-    require_no_cloud(ctx, "pmp_sphere_clearance_batch");
     if (B == 0 || ctx->n_spheres == 0) return;
     set_device(ctx);
-    PMP_HIP(launch_clearance(ctx->k32, ctx->s32, q, ldq, B, accurate ? 0 : 1, clearance, st(ctx)));
+    const int64_t S = ctx->n_spheres, Q = S * B;
+    float* cen = ctx->n_clouds ? (float*)ctx->ws_cen.get(sizeof(float) * 3 * Q) : nullptr;
+    PMP_HIP(launch_clearance(ctx->k32, ctx->s32, q, ldq, B, accurate ? 0 : 1, clearance, cen, st(ctx)));
+    for (const CloudRange& c : ctx->clouds) {  // nearest point of every cloud, no cap
+      float* d2 = (float*)ctx->ws_cloud.get(sizeof(float) * Q);
+      PMP_HIP(launch_cloud_min(ctx->s32.tiles, ctx->s32.tileB, c.tile_begin, c.n_tiles, cen, nullptr, B, Q, d2,
+                               st(ctx)));
+      PMP_HIP(launch_cloud_clearance(clearance, d2, (const float*)ctx->rad_out.ptr, c.radius, B, S, st(ctx)));
+    }
   });
 }
 
@@ -589,11 +704,23 @@ pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, in
   return guarded([&] {
     PMP_BATCH_CHECKS(q, ldq, B);
     PMP_REQUIRE(B == 0 || valid, "null output");
-    require_no_cloud(ctx, "pmp_is_valid_batch");
     if (B == 0) return;
     set_device(ctx);
+    CloudWork cw{};
+    int32_t* flags = nullptr;
+    if (ctx->n_clouds) {
+      const int64_t Q = (int64_t)ctx->k32.n_ext_spheres * B;
+      cw.B = B;
+      cw.cen = (float*)ctx->ws_cen.get(sizeof(float) * std::max<int64_t>(3 * Q, 1));
+      float* w = (float*)ctx->ws_cloud.get(sizeof(float) * std::max<int64_t>(3 * Q * ctx->n_clouds, 1));
+      for (int oc = 0; oc < ctx->n_clouds; ++oc) {  // per cloud: cap2, rr, d2
+        cw.cap2[oc] = w + 3 * Q * oc;
+        cw.rr[oc] = cw.cap2[oc] + Q;
+      }
+      flags = (int32_t*)ctx->ws_flags.get(sizeof(int32_t) * B);
+    }
     PMP_HIP(launch_valid(ctx->k32, ctx->k64, ctx->s32, ctx->s64, q, ldq, B, ctx->band, valid, n_recheck,
-                         ctx->sched.block, st(ctx)));
+                         ctx->sched.block, cw, ctx->clouds.data(), ctx->n_clouds, flags, st(ctx)));
   });
 }
 

@@ -1,0 +1,170 @@
+// Sphere-vs-point-cloud distances on the matrix cores.
+//
+// A cloud is stored as tiles of 64 points (Morton order, so each tile is
+// spatially compact) with the tile's AABB and centre o.  For a sphere centre
+// c the squared distance to point p is
+//     |c - p|^2 = |c - o|^2 + ( |p - o|^2 - 2 (c - o) . (p - o) )
+// and the bracket over 16 centres x 16 points is one K = 4 MFMA:
+//     A (16 x 4) = [ c - o , 1 ],   B (4 x 16) = [ -2 (p - o) ; |p - o|^2 ].
+// Working in tile-local coordinates keeps every term near the tile size, so
+// the FP32 cancellation error stays around 1e-9 m^2 for the tiles that can
+// matter.  A wave handles 16 queries; a tile is skipped when no query's cap
+// reaches the tile's AABB.
+#include <float.h>
+
+#include "fk.hpp"
+#include "launch.h"
+
+namespace pmp {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// min |c - p|^2 over the tiles [tile_begin, tile_begin + n_tiles) for queries
+// q < Q whose cap2[q] >= 0; centres in cen[(q / B * 3 + k) * B + q % B].
+// Writes out_d2[q] = min squared distance (FLT_MAX when no point lies within
+// the cap).  `skip` (optional, per configuration q % B): queries of
+// configurations already decided (0) are not evaluated.
+__global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, const float* tileB, int tile_begin,
+                                                        int n_tiles, const float* cen, const float* cap2, int64_t B,
+                                                        int64_t Q, const uint8_t* skip, float* out_d2) {
+  const int lane = threadIdx.x & 63;
+  const int64_t q0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 16;
+  if (q0 >= Q) return;  // wave-uniform
+  const int i = lane & 15, k = lane >> 4;
+  // the A-row query of this lane
+  const int64_t qi = q0 + i;
+  float cx = 0.f, cy = 0.f, cz = 0.f, capi = -1.f;
+  if (qi < Q) {
+    const int64_t e = qi / B, b = qi % B;
+    cx = cen[(e * 3 + 0) * B + b];
+    cy = cen[(e * 3 + 1) * B + b];
+    cz = cen[(e * 3 + 2) * B + b];
+    capi = cap2 ? cap2[qi] : FLT_MAX;
+    if (skip && skip[b] == 0) capi = -1.f;
+  }
+  // centres of the four output rows 4k + r held by this lane
+  float rx[4], ry[4], rz[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t q = q0 + 4 * k + r;
+    rx[r] = ry[r] = rz[r] = 0.f;
+    if (q < Q) {
+      const int64_t e = q / B, b = q % B;
+      rx[r] = cen[(e * 3 + 0) * B + b];
+      ry[r] = cen[(e * 3 + 1) * B + b];
+      rz[r] = cen[(e * 3 + 2) * B + b];
+    }
+  }
+  float best[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < n_tiles; ++t) {
+    const CloudTile T = tiles[tile_begin + t];
+    const float dx = fmaxf(fmaxf(T.lb[0] - cx, cx - T.ub[0]), 0.f);
+    const float dy = fmaxf(fmaxf(T.lb[1] - cy, cy - T.ub[1]), 0.f);
+    const float dz = fmaxf(fmaxf(T.lb[2] - cz, cz - T.ub[2]), 0.f);
+    const bool need = dx * dx + dy * dy + dz * dz <= capi;
+    if (!__any(need)) continue;
+    const float a = k == 0 ? cx - T.o[0] : (k == 1 ? cy - T.o[1] : (k == 2 ? cz - T.o[2] : 1.f));
+    const float* Bt = tileB + (int64_t)(tile_begin + t) * 256;
+    floatx4 d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[lane], zero, 0, 0, 0);
+    floatx4 d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[64 + lane], zero, 0, 0, 0);
+    floatx4 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[128 + lane], zero, 0, 0, 0);
+    floatx4 d3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[192 + lane], zero, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float ox = rx[r] - T.o[0], oy = ry[r] - T.o[1], oz = rz[r] - T.o[2];
+      const float off = ox * ox + oy * oy + oz * oz;
+      const float m = fminf(fminf(d0[r], d1[r]), fminf(d2[r], d3[r]));
+      best[r] = fminf(best[r], m + off);
+    }
+  }
+  // minimum over the 16 lanes that share the rows (columns j = lane & 15)
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    for (int s = 1; s < 16; s <<= 1) best[r] = fminf(best[r], __shfl_xor(best[r], s, 64));
+  if (i == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t q = q0 + 4 * k + r;
+      if (q < Q) out_d2[q] = best[r];
+    }
+  }
+}
+
+// Collision decision per cloud query: collide iff sqrt(d2) - rc <= r, i.e.
+// d2 <= (r + rc)^2 (CloudSDF::is_outside, primitive_sdf.hpp:597-603).
+// rr[q] = r + rc of the query, or < 0 when not queried.  flags[b]: bit 0 a
+// collision outside the band, bit 1 a decision inside the band.
+__global__ void __launch_bounds__(256) cloud_decide_kernel(const float* d2, const float* rr, int64_t B, int64_t Q,
+                                                           float band, int32_t* flags) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  const float R = rr[q];
+  if (R < 0.f) return;
+  const float m = d2[q] - R * R;
+  const int64_t b = q % B;
+  if (fabsf(m) <= band * (2.f * R + band))
+    atomicOr(&flags[b], 2);
+  else if (m < 0.f)
+    atomicOr(&flags[b], 1);
+}
+
+// valid[b] in {0, 1, 2} merged with the cloud flags.
+__global__ void __launch_bounds__(256) cloud_merge_kernel(uint8_t* valid, const int32_t* flags, int64_t B) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const uint8_t v = valid[b];
+  if (v == 0) return;
+  const int f = flags[b];
+  if (f & 1)
+    valid[b] = 0;
+  else if ((f & 2) || v == 2)
+    valid[b] = 2;
+}
+
+__global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// clearance[s * B + b] = min(clearance, sqrt(d2) - rc - r_s)
+__global__ void __launch_bounds__(256) cloud_clearance_kernel(float* clearance, const float* d2,
+                                                              const float* radius_out, float rc, int64_t B,
+                                                              int64_t S) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= S * B) return;
+  const float v = sqrtf(d2[q]) - rc - radius_out[q / B];
+  clearance[q] = fminf(clearance[q], v);
+}
+
+hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float* radius_out, float rc, int64_t B,
+                                  int64_t S, hipStream_t st) {
+  hipLaunchKernelGGL(cloud_clearance_kernel, dim3((unsigned)((S * B + 255) / 256)), dim3(256), 0, st, clearance, d2,
+                     radius_out, rc, B, S);
+  return hipGetLastError();
+}
+
+// The FP32 validity pass has filled cen / cap2 / rr; d2 is written after rr
+// (the workspace holds Q floats there).
+hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
+                              const float* cen, const float* cap2, const float* rr, int64_t B, int64_t Q, float band,
+                              uint8_t* valid, int32_t* flags, hipStream_t st) {
+  auto blocks = [](int64_t n, int t) { return (unsigned)((n + t - 1) / t); };
+  float* d2 = const_cast<float*>(rr) + Q;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, flags, 0, B);
+  hipLaunchKernelGGL(cloud_min_kernel, dim3(blocks((Q + 15) / 16 * 64, 256)), dim3(256), 0, st, tiles, tileB,
+                     tile_begin, n_tiles, cen, cap2, B, Q, valid, d2);
+  hipLaunchKernelGGL(cloud_decide_kernel, dim3(blocks(Q, 256)), dim3(256), 0, st, d2, rr, B, Q, band, flags);
+  hipLaunchKernelGGL(cloud_merge_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, valid, flags, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
+                            const float* cen, const float* cap2, int64_t B, int64_t Q, float* out_d2,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(cloud_min_kernel, dim3((unsigned)(((Q + 15) / 16 * 64 + 255) / 256)), dim3(256), 0, st, tiles,
+                     tileB, tile_begin, n_tiles, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2);
+  return hipGetLastError();
+}
+
+}  // namespace pmp

@@ -92,6 +92,7 @@ struct ClearanceVis {
   float* out;
   int64_t B, b;
   bool active;
+  float* cen;  // [S][3][B] centres for the cloud pass, or null
   __device__ bool alive() const { return true; }
   template <class QS>
   __device__ void begin(const Pose<float>&, const QS&) {}
@@ -103,17 +104,24 @@ struct ClearanceVis {
       if (P.type == PMP_PRIM_CLOUD) continue;
       best = fminf(best, sdf_eval(P, c) - r);
     }
-    if (active) out[(int64_t)ka.meta[s].out * B + b] = best;
+    if (!active) return;
+    const int64_t o = ka.meta[s].out;
+    out[o * B + b] = best;
+    if (cen) {
+      cen[(o * 3 + 0) * B + b] = c.x;
+      cen[(o * 3 + 1) * B + b] = c.y;
+      cen[(o * 3 + 2) * B + b] = c.z;
+    }
   }
   __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
 };
 
 __global__ void __launch_bounds__(256) clearance_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
-                                                        int64_t ldq, int64_t B, int taylor, float* out) {
+                                                        int64_t ldq, int64_t B, int taylor, float* out, float* cen) {
   const int64_t b = gid();
   const bool active = b < B;
   BatchQ qs{q, ldq, active ? b : B - 1};
-  ClearanceVis v{ka, sc, out, B, b, active};
+  ClearanceVis v{ka, sc, out, B, b, active, cen};
   run_fk(ka, qs, taylor != 0, v);
 }
 
@@ -126,6 +134,8 @@ struct ValidVis32 {
   const KinArgs<float>& ka;
   const SceneArgs<float>& sc;
   float band;
+  const CloudWork* cw = nullptr;  // cloud queries are written here when the scene holds clouds
+  int64_t b = 0;
   bool idle = false;
   bool definite = false;
   bool uncertain = false;
@@ -134,10 +144,25 @@ struct ValidVis32 {
   __device__ void begin(const Pose<float>&, const QS&) {}
   __device__ void node(int, const DevNode<float>&, const Pose<float>&) {}
   __device__ void sphere(int s, V3<float> c, float r) {
-    if (!alive() || !ka.meta[s].ext) return;
+    const int e = ka.meta[s].ext;
+    if (!alive() || e < 0) return;
+    if (cw) {
+      const int64_t B = cw->B;
+      cw->cen[((int64_t)e * 3 + 0) * B + b] = c.x;
+      cw->cen[((int64_t)e * 3 + 1) * B + b] = c.y;
+      cw->cen[((int64_t)e * 3 + 2) * B + b] = c.z;
+    }
     for (int p = 0; p < sc.n_prims; ++p) {
       const DevPrim<float>& P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) continue;
+      if (P.type == PMP_PRIM_CLOUD) {
+        // queried by the matrix-core pass unless the AABB pre-test culls it
+        const int oc = (P.flags >> 4) & 15;
+        const bool qd = !outside_aabb(P, c, r);
+        const float R = r + P.p0;
+        cw->cap2[oc][(int64_t)e * cw->B + b] = qd ? (R + band) * (R + band) : -1.0f;
+        cw->rr[oc][(int64_t)e * cw->B + b] = qd ? R : -1.0f;
+        continue;
+      }
       bool inb = false;
       const bool col = collide_fast(P, c, r, band, inb);
       if (col && !inb) {
@@ -172,10 +197,34 @@ struct ValidVis64 {
   __device__ void begin(const Pose<double>&, const QS&) {}
   __device__ void node(int, const DevNode<double>&, const Pose<double>&) {}
   __device__ void sphere(int s, V3<double> c, double r) {
-    if (collided || !ka.meta[s].ext) return;
+    if (collided || ka.meta[s].ext < 0) return;
     for (int p = 0; p < sc.n_prims; ++p) {
       const DevPrim<double>& P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) continue;
+      if (P.type == PMP_PRIM_CLOUD) {
+        // CloudSDF::is_outside: sqrt(min |p - c|^2) - rc > r (primitive_sdf.hpp:597-603),
+        // the minimum taken over the tiles that can reach within r + rc.
+        if (outside_aabb(P, c, r)) continue;
+        const double R = r + P.p0, lim = R * R * (1.0 + 1e-12) + 1e-300;
+        double best = DBL_MAX;
+        for (int t = 0; t < P.cloud_count; ++t) {
+          const CloudTile64 T = sc.tiles64[P.cloud_begin + t];
+          const double dx = fmax(fmax(T.lb[0] - c.x, c.x - T.ub[0]), 0.0);
+          const double dy = fmax(fmax(T.lb[1] - c.y, c.y - T.ub[1]), 0.0);
+          const double dz = fmax(fmax(T.lb[2] - c.z, c.z - T.ub[2]), 0.0);
+          if (dx * dx + dy * dy + dz * dz > lim) continue;
+          const double* pt = sc.pts64 + (int64_t)(P.cloud_begin + t) * kCloudTile * 4;
+          for (int j = 0; j < kCloudTile; ++j) {
+            const double ex = pt[4 * j] - c.x, ey = pt[4 * j + 1] - c.y, ez = pt[4 * j + 2] - c.z;
+            const double d2 = ex * ex + ey * ey + ez * ez;
+            if (d2 < best) best = d2;
+          }
+        }
+        if (!(sqrt(best) - P.p0 > r)) {
+          collided = true;
+          return;
+        }
+        continue;
+      }
       if (!outside_aabb(P, c, r) && !is_outside_ref(P, c, r)) {
         collided = true;
         return;
@@ -191,12 +240,14 @@ struct ValidVis64 {
 
 // valid[b]: 0 collision, 1 free, 2 pending FP64 re-decision.
 __global__ void __launch_bounds__(256) valid_kernel(KinArgs<float> k32, SceneArgs<float> s32, const double* q,
-                                                    int64_t ldq, int64_t B, float band, uint8_t* valid) {
+                                                    int64_t ldq, int64_t B, float band, uint8_t* valid, CloudWork cw) {
   const int64_t b = gid();
   const bool active = b < B;
   BatchQ qs{q, ldq, active ? b : B - 1};
   const bool taylor = k32.base_type == PMP_BASE_FIXED;  // constraints/primitive.hpp:59-61
   ValidVis32 v{k32, s32, band};
+  v.cw = s32.n_cloud > 0 && active ? &cw : nullptr;
+  v.b = b;
   v.idle = !active;
   run_fk(k32, qs, taylor, v);
   if (active) valid[b] = v.definite ? 0 : (v.uncertain ? 2 : 1);
@@ -410,7 +461,7 @@ struct EvalVis {
   // over (group, primitive, sphere) with the reference's visiting order as tie-break key.
   __device__ void sphere(int s, V3<float> c, float r) {
     const DevSphereMeta m = ka.meta[s];
-    if (!m.ext) return;
+    if (m.ext < 0) return;
     for (int p = 0; p < sc.n_prims; ++p) {
       const DevPrim<float>& P = prim_at(sc, p);
       if (P.type == PMP_PRIM_CLOUD) continue;
@@ -670,15 +721,25 @@ hipError_t launch_poses(const KinArgs<double>& ka, const double* q, int64_t ldq,
   return hipGetLastError();
 }
 hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
-                            int64_t B, int taylor, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(clearance_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, taylor, out);
+                            int64_t B, int taylor, float* out, float* cen, hipStream_t st) {
+  hipLaunchKernelGGL(clearance_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, taylor, out,
+                     cen);
   return hipGetLastError();
 }
 hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const double* q, int64_t ldq, int64_t B, float band,
-                        uint8_t* valid, int32_t* n_recheck, int block, hipStream_t st) {
+                        uint8_t* valid, int32_t* n_recheck, int block, const CloudWork& cw,
+                        const CloudRange* clouds, int n_clouds, int32_t* flags, hipStream_t st) {
   if (block <= 0) block = 256;
-  hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), 0, st, k32, s32, q, ldq, B, band, valid);
+  hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), 0, st, k32, s32, q, ldq, B, band, valid,
+                     cw);
+  // sphere-vs-cloud queries on the matrix cores, one pass per cloud
+  const int64_t Q = (int64_t)k32.n_ext_spheres * B;
+  for (int oc = 0; oc < n_clouds && Q > 0; ++oc) {
+    hipError_t e = launch_cloud_valid(s32.tiles, s32.tileB, clouds[oc].tile_begin, clouds[oc].n_tiles, cw.cen,
+                                      cw.cap2[oc], cw.rr[oc], B, Q, band, valid, flags, st);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(valid_recheck_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, k64, s64, q, ldq, B, valid,
                      n_recheck);
   return hipGetLastError();

@@ -27,11 +27,26 @@ hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq
                           hipStream_t st);
 hipError_t launch_poses(const KinArgs<double>& ka, const double* q, int64_t ldq, int64_t B, int taylor, double* out,
                         hipStream_t st);
+struct CloudRange {
+  int32_t tile_begin, n_tiles;
+  float radius;
+};
+
 hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
-                            int64_t B, int taylor, float* out, hipStream_t st);
+                            int64_t B, int taylor, float* out, float* cen, hipStream_t st);
 hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const double* q, int64_t ldq, int64_t B, float band,
-                        uint8_t* valid, int32_t* n_recheck, int block, hipStream_t st);
+                        uint8_t* valid, int32_t* n_recheck, int block, const CloudWork& cw,
+                        const CloudRange* clouds, int n_clouds, int32_t* flags, hipStream_t st);
+// cloud.hip
+hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
+                              const float* cen, const float* cap2, const float* rr, int64_t B, int64_t Q, float band,
+                              uint8_t* valid, int32_t* flags, hipStream_t st);
+hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
+                            const float* cen, const float* cap2, int64_t B, int64_t Q, float* out_d2,
+                            hipStream_t st);
+hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float* radius_out, float rc, int64_t B,
+                                  int64_t S, hipStream_t st);
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
                         hipStream_t st);

@@ -16,6 +16,8 @@ constexpr int kMaxNodes = 256;
 constexpr int kMaxSpheres = 4096;
 constexpr int kMaxPrims = 512;
 constexpr int kMaxSelfPairs = 1024;
+constexpr int kMaxClouds = 4;
+constexpr int kCloudTile = 64;  // points per cloud tile
 
 template <class T>
 struct alignas(16) DevNode {
@@ -40,9 +42,27 @@ struct alignas(16) DevSphere {
 
 struct alignas(16) DevSphereMeta {
   int32_t out;    // index in SphereCollisionCst group order
-  int32_t ext;    // 1 when checked against primitives (group not only-self)
+  int32_t ext;    // index among the spheres checked against primitives, -1 for only-self groups
   int32_t node;
   int32_t group;
+};
+
+// Point-cloud tile: AABB and centre (FP32, for culling and the local frame of
+// the matrix-core distances) and FP64 bounds for the exact re-decision.
+struct alignas(16) CloudTile {
+  float lb[3], ub[3], o[3];
+  int32_t n;
+};
+struct alignas(16) CloudTile64 {
+  double lb[3], ub[3];
+};
+
+// Where the FP32 validity pass leaves sphere-vs-cloud queries.
+struct CloudWork {
+  float* cen;                 // [n_ext][3][B] sphere centres
+  float* cap2[kMaxClouds];    // [n_ext][B] squared reach (r + rc + band)^2, < 0 when not queried
+  float* rr[kMaxClouds];      // [n_ext][B] r + rc
+  int64_t B;
 };
 
 struct alignas(16) DevSelfPair {
@@ -66,10 +86,13 @@ struct KinArgs {
 
 template <class T>
 struct SceneArgs {
-  const void* prims;  // DevPrim<T>*
+  const void* prims;            // DevPrim<T>*; clouds carry their tile range in cloud_begin / cloud_count
   int32_t n_prims;
-  int32_t n_cloud;    // number of cloud primitives (must be last in the list)
-  const T* cloud_xyz; // cloud points [n][4] (x, y, z, |p|^2) for all clouds, concatenated
+  int32_t n_cloud;              // number of cloud primitives
+  const CloudTile* tiles;       // all clouds' tiles
+  const CloudTile64* tiles64;
+  const float* tileB;           // [tile][4 MFMA][4 k][16 j] B operands in tile-local coordinates
+  const double* pts64;          // [tile][64][4] FP64 world points (padding points far away)
 };
 
 }  // namespace pmp

@@ -210,3 +210,19 @@ def test_kdtree_and_cloud():
     ev = rng.normal(size=(3, 300)) * 1.5
     ref = _oracle.Prim.cloud(obst, 0.1).evaluate_batch(ev)
     assert np.allclose(cs.evaluate_batch(ev), ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("n_configs", [256])
+def test_cloud_valid_and_clearance(n_configs):
+    wl = synth.workload("cloud_shelves", scale=n_configs / 16384)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref = oc.is_valid_batch(wl.Q, 8).astype(bool)
+    got, n_re = pc.is_valid_batch_stats(wl.Q)
+    _report("cloud_valid", B=len(ref), valid_frac=ref.mean(), rechecked=n_re)
+    assert np.array_equal(got, ref)
+    Q = wl.Q[: min(16, len(wl.Q))]
+    cref = oc.sphere_clearance_batch(Q, False, 8)
+    cgot = pc.sphere_clearance_batch(Q, False)
+    err = np.abs(cgot - cref).max()
+    _report("cloud_clearance", max_err=err)
+    assert err <= 5e-6
