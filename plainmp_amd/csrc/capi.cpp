@@ -116,13 +116,13 @@ struct pmp_context {
   int n_cu = 256;
   // robot
   bool has_robot = false;
-  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs;
+  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks;
   KinArgs<float> k32{};
   KinArgs<double> k64{};
   int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
   DevBuf rad_out;  // sphere radius by group-order index (FP32)
   // scene
-  DevBuf prims32, prims64, tiles32, tiles64, tileB, pts64;
+  DevBuf prims32, prims64, vprims32, tiles32, tiles64, tileB, pts64;
   SceneArgs<float> s32{};
   SceneArgs<double> s64{};
   int n_clouds = 0;
@@ -343,7 +343,8 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
-    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->rad_out,
+    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks,
+                      &ctx->vprims32, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
                       &ctx->ws_cloud, &ctx->ws_flags})
@@ -504,6 +505,59 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
       meta[k] = {s, ext ? n_ext : -1, r->sphere_node[s], g};
       n_ext += ext;
     }
+    // Sphere blocks: per node, its primitive-checked spheres sorted along
+    // their longest extent and cut into runs of at most kBlockSpheres, each
+    // with a bounding sphere (rounded up to FP32) in the link frame.
+    std::vector<DevSphereBlock> blocks;
+    std::vector<int> blk_begin(N, 0), blk_end(N, 0);
+    for (int i = 0; i < N; ++i) {
+      blk_begin[i] = (int)blocks.size();
+      std::vector<int> mem;
+      for (int k = 0; k < S; ++k)
+        if (meta[k].node == i && meta[k].ext >= 0) mem.push_back(k);
+      if (!mem.empty()) {
+        auto coord = [&](int k, int d) { return d == 0 ? sp64[k].x : (d == 1 ? sp64[k].y : sp64[k].z); };
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int k : mem)
+          for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], coord(k, d));
+            hi[d] = std::max(hi[d], coord(k, d));
+          }
+        int axis = 0;
+        for (int d = 1; d < 3; ++d)
+          if (hi[d] - lo[d] > hi[axis] - lo[axis]) axis = d;
+        std::stable_sort(mem.begin(), mem.end(), [&](int a, int b) { return coord(a, axis) < coord(b, axis); });
+        const int m = (int)mem.size(), nblk = (m + kBlockSpheres - 1) / kBlockSpheres;
+        for (int q = 0; q < nblk; ++q) {
+          const int a = m * q / nblk, z = m * (q + 1) / nblk;
+          DevSphereBlock bk{};
+          bk.n = z - a;
+          double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+          for (int t = a; t < z; ++t)
+            for (int d = 0; d < 3; ++d) {
+              blo[d] = std::min(blo[d], coord(mem[t], d) - sp64[mem[t]].r);
+              bhi[d] = std::max(bhi[d], coord(mem[t], d) + sp64[mem[t]].r);
+            }
+          for (int d = 0; d < 3; ++d) bk.g[d] = (float)(0.5 * (blo[d] + bhi[d]));
+          double G = 0.0;
+          for (int t = a; t < z; ++t) {
+            const int k = mem[t];
+            const double ex = coord(k, 0) - bk.g[0], ey = coord(k, 1) - bk.g[1], ez = coord(k, 2) - bk.g[2];
+            G = std::max(G, std::sqrt(ex * ex + ey * ey + ez * ez) + sp64[k].r);
+            bk.ext[t - a] = meta[k].ext;
+            bk.s[t - a][0] = sp32[k].x;
+            bk.s[t - a][1] = sp32[k].y;
+            bk.s[t - a][2] = sp32[k].z;
+            bk.s[t - a][3] = sp32[k].r;
+          }
+          bk.G = (float)G;
+          if ((double)bk.G < G) bk.G = std::nextafter(bk.G, INFINITY);
+          if (!(G == G)) bk.G = INFINITY;  // never cull around a NaN bound
+          blocks.push_back(bk);
+        }
+      }
+      blk_end[i] = (int)blocks.size();
+    }
     std::vector<DevNode<float>> n32(N);
     std::vector<DevNode<double>> n64(N);
     for (int i = 0; i < N; ++i) {
@@ -524,6 +578,8 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
         nd.sphere_end = e;
         nd.identity = r->node_identity[i] ? 1 : 0;
         nd.col_mask = mask[i];
+        nd.block_begin = blk_begin[i];
+        nd.block_end = blk_end[i];
         for (int k = 0; k < 4; ++k) nd.quat[k] = (T)r->node_quat[4 * i + k];
         for (int k = 0; k < 3; ++k) {
           nd.trans[k] = (T)r->node_trans[3 * i + k];
@@ -565,6 +621,7 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     d.spheres = ctx->sph64.upload(sp64, s0);
     a.meta = d.meta = ctx->meta.upload(meta, s0);
     a.pairs = d.pairs = ctx->pairs.upload(pairs, s0);
+    a.blocks = d.blocks = ctx->blocks.upload(blocks, s0);
     std::vector<float> rad(std::max(S, 1), 0.f);
     for (int s2 = 0; s2 < S; ++s2) rad[s2] = (float)r->sphere_radius[s2];
     ctx->rad_out.upload(rad, s0);
@@ -623,11 +680,24 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
       p64[i].flags |= oc << 4;
       clouds.push_back({begin, nt, (float)sc->prims[i].size[0]});
     }
+    // kind-sorted copy for the validity pass (its answer does not depend on
+    // the primitive order)
+    std::vector<DevPrim<float>> v32;
+    int vrange[kNumKinds + 1] = {0};
+    for (int k = 0; k < kNumKinds; ++k) {
+      vrange[k] = (int)v32.size();
+      for (const DevPrim<float>& p : p32)
+        if (prim_kind(p) == k) v32.push_back(p);
+    }
+    vrange[kNumKinds] = (int)v32.size();
     set_device(ctx);
     hipStream_t s0 = st(ctx);
     SceneArgs<float> a{};
     SceneArgs<double> d{};
     a.prims = ctx->prims32.upload(p32, s0);
+    a.vprims = ctx->vprims32.upload(v32, s0);
+    d.vprims = nullptr;
+    for (int k = 0; k <= kNumKinds; ++k) a.vrange[k] = d.vrange[k] = vrange[k];
     d.prims = ctx->prims64.upload(p64, s0);
     a.tiles = d.tiles = ctx->tiles32.upload(t32, s0);
     a.tiles64 = d.tiles64 = ctx->tiles64.upload(t64, s0);
@@ -799,9 +869,12 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
   ea.n_recheck = ((int32_t*)ctx->counter.ptr) + 1;
 }
 
-int edge_grid(pmp_context* ctx, int block) {
-  const int wpc = ctx->sched.waves_per_cu > 0 ? ctx->sched.waves_per_cu : 8;
-  return std::max(1, ctx->n_cu * wpc * 64 / block);
+// Persistent grid: waves_per_cu waves on every CU (default 16, four per
+// SIMD), never more waves than edges.
+int edge_grid(pmp_context* ctx, int block, int64_t E) {
+  const int wpc = ctx->sched.waves_per_cu > 0 ? ctx->sched.waves_per_cu : 16;
+  const int64_t by_edges = (E * 64 + block - 1) / block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * wpc * 64 / block, by_edges));
 }
 
 }  // namespace
@@ -815,7 +888,7 @@ pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const doubl
     prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ea);
     if (E == 0) return;
     const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
-    PMP_HIP(launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block), block,
+    PMP_HIP(launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block, E), block,
                          st(ctx)));
   });
 }
@@ -838,7 +911,7 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
     g->stream = cs;
     try {
       PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      hipError_t le = launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block),
+      hipError_t le = launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block, E),
                                    block, cs);
       hipError_t ce = hipStreamEndCapture(cs, &g->graph);
       PMP_HIP(le);

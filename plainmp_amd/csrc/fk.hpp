@@ -24,6 +24,18 @@ namespace pmp {
 
 __device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
 
+// Visitors that declare `static constexpr bool kBlocks = true` receive each
+// node's sphere blocks through blocks(node, pose) instead of one sphere() call
+// per sphere.
+template <class V, class = void>
+struct uses_blocks {
+  static constexpr bool value = false;
+};
+template <class V>
+struct uses_blocks<V, decltype((void)V::kBlocks)> {
+  static constexpr bool value = V::kBlocks;
+};
+
 // Register file of kept poses.  The slot index is wave-uniform, so the
 // switch compiles to scalar branches around plain register moves.
 template <class T>
@@ -100,6 +112,7 @@ __device__ __forceinline__ Pose<T> base_pose(const KinArgs<T>& ka, const QS& qs)
 //   void begin(const Pose<T>& base, const QS&)  base pose of this configuration
 //   void node(int n, const DevNode<T>&, const Pose<T>&)
 //   void sphere(int s, V3<T> c, T r)            s indexes the node-sorted table
+//   void blocks(const DevNode<T>&, const Pose<T>&)  instead of sphere(), with kBlocks
 //   void self(const DevSelfPair&, int i, int j, V3<T> ci, V3<T> cj, T rsum)
 // `taylor` selects the collision check's sin/cos (a runtime, wave-uniform flag).
 template <class T, class QS, class Vis>
@@ -151,7 +164,9 @@ __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool 
     if (nd.out_slot >= 0) slots.set(nd.out_slot, P);
     prev = P;
     vis.node(n, nd, P);
-    if (nd.sphere_begin < nd.sphere_end) {
+    if constexpr (uses_blocks<Vis>::value) {
+      vis.blocks(nd, P);
+    } else if (nd.sphere_begin < nd.sphere_end) {
       const M3<T> R = qmat(P.q);
       for (int s = nd.sphere_begin; s < nd.sphere_end; ++s) {
         const DevSphere<T> sp = ka.spheres[s];

@@ -379,58 +379,85 @@ PMP_HD bool is_outside_ref(const DevPrim<T>& p, V3<T> c, T r) {
   return false;
 }
 
-// FP32 collision test of a sphere (c, r) against a non-cloud primitive,
-// branch-free: returns true on collision (the reference's
-// !is_outside_aabb && !is_outside) and raises `inband` when the decision lies
-// within `band` metres of flipping.  For box, cylinder and sphere the AABB
-// test is implied by the exact test; for the ground it is part of the
-// decision (its AABB is z in [0, 0]).  A NaN centre collides, as every
-// comparison of the reference is false for NaN.
+// FP32 collision test of a sphere (c, r) against a non-cloud primitive of a
+// known kind: returns true on collision (the reference's !is_outside_aabb &&
+// !is_outside) and raises `inband` when the decision lies within `band`
+// metres of flipping.  For box, cylinder and sphere the AABB test is implied
+// by the exact test; for the ground it is part of the decision (its AABB is
+// z in [0, 0]).  NaN centres are the caller's business (see collide_fast).
+template <int K>
+PMP_HD bool collide_kind(const DevPrim<float>& p, V3<float> c, float r, float band, bool& inband) {
+  const float dx = c.x - p.pos[0], dy = c.y - p.pos[1], dz = c.z - p.pos[2];
+  if constexpr (K == kKindBoxAA || K == kKindBoxRot) {
+    V3<float> d = {dx, dy, dz};
+    if constexpr (K == kKindBoxRot) d = mv(load_m3(p.Rinv), d);
+    const float sx = fmaxf(fabsf(d.x) - p.p0, 0.0f);
+    const float sy = fmaxf(fabsf(d.y) - p.p1, 0.0f);
+    const float sz = fmaxf(fabsf(d.z) - p.p2, 0.0f);
+    const float m = sx * sx + sy * sy + sz * sz - r * r;
+    inband |= fabsf(m) <= band * (2.0f * r + band);
+    return m <= 0.0f;
+  } else if constexpr (K == kKindCylZ || K == kKindCylRot) {
+    float zs, xa, ya;
+    if constexpr (K == kKindCylZ) {
+      zs = fabsf(dz) - p.p1;
+      xa = dx;
+      ya = dy;
+    } else {
+      zs = fabsf(dx * p.R[2] + dy * p.R[5] + dz * p.R[8]) - p.p1;
+      xa = dx * p.R[0] + dy * p.R[3] + dz * p.R[6];
+      ya = dx * p.R[1] + dy * p.R[4] + dz * p.R[7];
+    }
+    const float rs = fmaxf(sqrtf(xa * xa + ya * ya) - p.p0, 0.0f);
+    zs = fmaxf(zs, 0.0f);
+    const float m = rs * rs + zs * zs - r * r;
+    inband |= fabsf(m) <= band * (2.0f * r + band);
+    return m <= 0.0f;
+  } else if constexpr (K == kKindSphere) {
+    const float R = p.p0 + r;
+    const float m = dx * dx + dy * dy + dz * dz - R * R;
+    inband |= fabsf(m) <= band * (2.0f * R + band);
+    return m <= 0.0f;
+  } else {  // ground
+    const float lo = c.z - (p.lb[2] - r), hi = (p.ub[2] + r) - c.z;  // inside the AABB z-slab iff both >= 0
+    const float m = c.z + p.p0 - r;                                    // outside iff m > 0
+    inband |= fabsf(lo) <= band || fabsf(hi) <= band || fabsf(m) <= band;
+    return lo >= 0.0f && hi >= 0.0f && m <= 0.0f;
+  }
+}
+
+template <class T>
+PMP_HD int prim_kind(const DevPrim<T>& p) {
+  switch (p.type) {
+    case PMP_PRIM_BOX:
+      return (p.flags & 1) ? kKindBoxAA : kKindBoxRot;
+    case PMP_PRIM_CYLINDER:
+      return (p.flags & 2) ? kKindCylZ : kKindCylRot;
+    case PMP_PRIM_SPHERE:
+      return kKindSphere;
+    case PMP_PRIM_GROUND:
+      return kKindGround;
+  }
+  return kKindCloud;
+}
+
+// Any non-cloud primitive.  A NaN centre collides, as every comparison of the
+// reference is false for NaN.
 PMP_HD bool collide_fast(const DevPrim<float>& p, V3<float> c, float r, float band, bool& inband) {
   if (is_nan3(c)) return true;
-  const float tau = band * (2.0f * r + band);
-  switch (p.type) {
-    case PMP_PRIM_BOX: {
-      V3<float> d = {c.x - p.pos[0], c.y - p.pos[1], c.z - p.pos[2]};
-      if (!(p.flags & 1)) d = mv(load_m3(p.Rinv), d);
-      float sx = fmaxf(fabsf(d.x) - p.p0, 0.0f);
-      float sy = fmaxf(fabsf(d.y) - p.p1, 0.0f);
-      float sz = fmaxf(fabsf(d.z) - p.p2, 0.0f);
-      float m = sx * sx + sy * sy + sz * sz - r * r;
-      inband |= fabsf(m) <= tau;
-      return m <= 0.0f;
-    }
-    case PMP_PRIM_CYLINDER: {
-      float zs, xa, ya;
-      float dx = c.x - p.pos[0], dy = c.y - p.pos[1], dz = c.z - p.pos[2];
-      if (p.flags & 2) {
-        zs = fabsf(dz) - p.p1;
-        xa = dx;
-        ya = dy;
-      } else {
-        zs = fabsf(dx * p.R[2] + dy * p.R[5] + dz * p.R[8]) - p.p1;
-        xa = dx * p.R[0] + dy * p.R[3] + dz * p.R[6];
-        ya = dx * p.R[1] + dy * p.R[4] + dz * p.R[7];
-      }
-      float rs = fmaxf(sqrtf(xa * xa + ya * ya) - p.p0, 0.0f);
-      zs = fmaxf(zs, 0.0f);
-      float m = rs * rs + zs * zs - r * r;
-      inband |= fabsf(m) <= tau;
-      return m <= 0.0f;
-    }
-    case PMP_PRIM_SPHERE: {
-      float dx = c.x - p.pos[0], dy = c.y - p.pos[1], dz = c.z - p.pos[2];
-      float R = p.p0 + r;
-      float m = dx * dx + dy * dy + dz * dz - R * R;
-      inband |= fabsf(m) <= band * (2.0f * R + band);
-      return m <= 0.0f;
-    }
-    case PMP_PRIM_GROUND: {
-      float lo = c.z - (p.lb[2] - r), hi = (p.ub[2] + r) - c.z;  // inside the AABB z-slab iff both >= 0
-      float m = c.z + p.p0 - r;                                    // outside iff m > 0
-      inband |= fabsf(lo) <= band || fabsf(hi) <= band || fabsf(m) <= band;
-      return lo >= 0.0f && hi >= 0.0f && m <= 0.0f;
-    }
+  switch (prim_kind(p)) {
+    case kKindBoxAA:
+      return collide_kind<kKindBoxAA>(p, c, r, band, inband);
+    case kKindBoxRot:
+      return collide_kind<kKindBoxRot>(p, c, r, band, inband);
+    case kKindCylZ:
+      return collide_kind<kKindCylZ>(p, c, r, band, inband);
+    case kKindCylRot:
+      return collide_kind<kKindCylRot>(p, c, r, band, inband);
+    case kKindSphere:
+      return collide_kind<kKindSphere>(p, c, r, band, inband);
+    case kKindGround:
+      return collide_kind<kKindGround>(p, c, r, band, inband);
   }
   return false;
 }

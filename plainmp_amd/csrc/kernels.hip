@@ -131,6 +131,7 @@ __global__ void __launch_bounds__(256) clearance_kernel(KinArgs<float> ka, Scene
 // FP64 re-decision ("uncertain").
 struct ValidVis32 {
   static constexpr bool kSelf = true;
+  static constexpr bool kBlocks = true;
   const KinArgs<float>& ka;
   const SceneArgs<float>& sc;
   float band;
@@ -143,35 +144,98 @@ struct ValidVis32 {
   template <class QS>
   __device__ void begin(const Pose<float>&, const QS&) {}
   __device__ void node(int, const DevNode<float>&, const Pose<float>&) {}
-  __device__ void sphere(int s, V3<float> c, float r) {
-    const int e = ka.meta[s].ext;
-    if (!alive() || e < 0) return;
-    if (cw) {
-      const int64_t B = cw->B;
-      cw->cen[((int64_t)e * 3 + 0) * B + b] = c.x;
-      cw->cen[((int64_t)e * 3 + 1) * B + b] = c.y;
-      cw->cen[((int64_t)e * 3 + 2) * B + b] = c.z;
-    }
-    for (int p = 0; p < sc.n_prims; ++p) {
-      const DevPrim<float>& P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) {
-        // queried by the matrix-core pass unless the AABB pre-test culls it
-        const int oc = (P.flags >> 4) & 15;
-        const bool qd = !outside_aabb(P, c, r);
-        const float R = r + P.p0;
-        cw->cap2[oc][(int64_t)e * cw->B + b] = qd ? (R + band) * (R + band) : -1.0f;
-        cw->rr[oc][(int64_t)e * cw->B + b] = qd ? R : -1.0f;
-        continue;
+  __device__ void sphere(int, V3<float>, float) {}
+
+  // One kind of primitive against the (up to eight) spheres of a block.  A
+  // primitive is skipped for the whole wave when no live lane's block
+  // bounding sphere, grown by `reach`, touches its AABB: every member sphere
+  // is then outside the AABB inflated by its radius plus the band, which is
+  // the reference's own AABB pre-test failing with room to spare, so the
+  // skipped pairs can neither collide nor fall in the band.
+  template <int K>
+  __device__ __forceinline__ void kind_loop(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
+                                            V3<float> g, float reach) {
+    const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
+    for (int p = sc.vrange[K]; p < sc.vrange[K + 1]; ++p) {
+      const DevPrim<float>& P = vp[p];
+      if (!wave_any(alive() && !outside_aabb(P, g, reach))) continue;
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        if (k < n) {
+          bool inb = false;
+          const bool col = collide_kind<K>(P, c[k], bk.s[k][3], band, inb);
+          definite |= col && !inb;
+          uncertain |= inb;
+        }
       }
-      bool inb = false;
-      const bool col = collide_fast(P, c, r, band, inb);
-      if (col && !inb) {
-        definite = true;
-        return;
-      }
-      uncertain |= inb;
     }
   }
+
+  __device__ void blocks(const DevNode<float>& nd, const Pose<float>& P) {
+    if (nd.block_begin >= nd.block_end) return;
+    const M3<float> R = qmat(P.q);
+    // |R v| <= (1 + 2 | |q|^2 - 1 |) |v| for the rotation matrix of a
+    // quaternion that is not exactly unit (Taylor sin/cos, URDF rounding)
+    const float n2 = P.q.x * P.q.x + P.q.y * P.q.y + P.q.z * P.q.z + P.q.w * P.q.w;
+    const float grow = 1.0f + 2.0f * fabsf(n2 - 1.0f);
+    const int n_solid = sc.vrange[kKindCloud];
+    for (int bi = nd.block_begin; bi < nd.block_end; ++bi) {
+      if (!wave_any(alive())) return;
+      const DevSphereBlock& bk = ka.blocks[bi];
+      const int n = bk.n;
+      V3<float> c[kBlockSpheres];
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        c[k] = P.t;
+        if (k < n) {
+          c[k] = add(mv(R, V3<float>{bk.s[k][0], bk.s[k][1], bk.s[k][2]}), P.t);
+          bad |= is_nan3(c[k]);
+        }
+      }
+      const V3<float> g = add(mv(R, V3<float>{bk.g[0], bk.g[1], bk.g[2]}), P.t);
+      // slack 1e-4 m covers the FP32 rounding of the centres and the AABB
+      const float reach = bk.G * grow + band + 1e-4f;
+      if (bad && n_solid > 0) definite = true;  // NaN centres collide with every primitive
+      if (cw && !idle) {
+        const int64_t B = cw->B;
+#pragma unroll
+        for (int k = 0; k < kBlockSpheres; ++k) {
+          if (k < n) {
+            const int64_t e = bk.ext[k];
+            cw->cen[(e * 3 + 0) * B + b] = c[k].x;
+            cw->cen[(e * 3 + 1) * B + b] = c[k].y;
+            cw->cen[(e * 3 + 2) * B + b] = c[k].z;
+          }
+        }
+        // clouds: queried by the matrix-core pass unless the AABB pre-test culls it
+        const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
+        for (int p = sc.vrange[kKindCloud]; p < sc.vrange[kKindCloud + 1]; ++p) {
+          const DevPrim<float>& Pc = vp[p];
+          const int oc = (Pc.flags >> 4) & 15;
+#pragma unroll
+          for (int k = 0; k < kBlockSpheres; ++k) {
+            if (k < n) {
+              const float r = bk.s[k][3];
+              const bool qd = alive() && !outside_aabb(Pc, c[k], r);
+              const float Rr = r + Pc.p0;
+              const int64_t at = (int64_t)bk.ext[k] * B + b;
+              cw->cap2[oc][at] = qd ? (Rr + band) * (Rr + band) : -1.0f;
+              cw->rr[oc][at] = qd ? Rr : -1.0f;
+            }
+          }
+        }
+      }
+      if (!wave_any(alive())) continue;
+      kind_loop<kKindBoxAA>(c, bk, n, g, reach);
+      kind_loop<kKindBoxRot>(c, bk, n, g, reach);
+      kind_loop<kKindCylZ>(c, bk, n, g, reach);
+      kind_loop<kKindCylRot>(c, bk, n, g, reach);
+      kind_loop<kKindSphere>(c, bk, n, g, reach);
+      kind_loop<kKindGround>(c, bk, n, g, reach);
+    }
+  }
+
   __device__ void self(const DevSelfPair&, int, int, V3<float> ci, V3<float> cj, float rs) {
     if (!alive()) return;
     const V3<float> d = sub(ci, cj);

@@ -18,6 +18,11 @@ constexpr int kMaxPrims = 512;
 constexpr int kMaxSelfPairs = 1024;
 constexpr int kMaxClouds = 4;
 constexpr int kCloudTile = 64;  // points per cloud tile
+constexpr int kBlockSpheres = 8;  // spheres of one link tested together against a primitive
+
+// Primitive kinds of the validity pass; the scene keeps a copy of its
+// primitives sorted by kind so that each kind is a loop without branches.
+enum PrimKind { kKindBoxAA = 0, kKindBoxRot, kKindCylZ, kKindCylRot, kKindSphere, kKindGround, kKindCloud, kNumKinds };
 
 template <class T>
 struct alignas(16) DevNode {
@@ -29,6 +34,9 @@ struct alignas(16) DevNode {
   int32_t sphere_end;
   int32_t identity;      // local rotation skipped on compose
   uint32_t col_mask;     // q columns that move this node (reference rptable_ row)
+  int32_t block_begin;   // sphere blocks of this node (primitive-checked spheres only)
+  int32_t block_end;
+  int32_t ipad[2];
   T quat[4];
   T trans[3];
   T axis[3];
@@ -38,6 +46,16 @@ struct alignas(16) DevNode {
 template <class T>
 struct alignas(16) DevSphere {
   T x, y, z, r;
+};
+
+// Up to kBlockSpheres primitive-checked spheres of one link with a bounding
+// sphere (g, G) in the link frame: G >= |s_k - g| + r_k for every member.
+struct alignas(16) DevSphereBlock {
+  int32_t n;
+  int32_t ext[kBlockSpheres];  // index among the primitive-checked spheres
+  int32_t pad[3];
+  float g[3], G;
+  float s[kBlockSpheres][4];   // link-frame centre and radius
 };
 
 struct alignas(16) DevSphereMeta {
@@ -77,6 +95,7 @@ struct KinArgs {
   const DevSphere<T>* spheres;
   const DevSphereMeta* meta;
   const DevSelfPair* pairs;
+  const DevSphereBlock* blocks;
   int32_t n_nodes, n_spheres, n_pairs, n_cols;
   int32_t n_q_cols, base_type, n_ext_spheres;
   uint32_t rev_mask;  // q columns that rotate (revolute joints, base yaw/roll/pitch)
@@ -93,6 +112,8 @@ struct SceneArgs {
   const CloudTile64* tiles64;
   const float* tileB;           // [tile][4 MFMA][4 k][16 j] B operands in tile-local coordinates
   const double* pts64;          // [tile][64][4] FP64 world points (padding points far away)
+  const void* vprims;           // DevPrim<T>* sorted by PrimKind (validity pass)
+  int32_t vrange[kNumKinds + 1];  // kind k occupies vprims[vrange[k], vrange[k + 1])
 };
 
 }  // namespace pmp

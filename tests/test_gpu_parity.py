@@ -226,3 +226,26 @@ def test_cloud_valid_and_clearance(n_configs):
     err = np.abs(cgot - cref).max()
     _report("cloud_clearance", max_err=err)
     assert err <= 5e-6
+
+
+@pytest.mark.parametrize("height", [0.0, 0.3])
+def test_valid_every_kind_with_ground(height):
+    """Boxes (axis-aligned and rotated), cylinders (z-aligned and rotated),
+    spheres and a ground plane in one scene: the kind-sorted validity pass
+    against the oracle, and the edge kernel on the same scene."""
+    import dataclasses
+
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    base = synth.workload("grad_mixed32", scale=1 / 16)  # 512 configs, 32 mixed primitives
+    wl = dataclasses.replace(base, prims=base.prims + [{"type": "ground", "height": height}])
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref = oc.is_valid_batch(wl.Q, 8).astype(bool)
+    got, n_re = pc.is_valid_batch_stats(wl.Q)
+    _report("every_kind", height=height, B=len(ref), valid_frac=ref.mean(), rechecked=n_re)
+    assert np.array_equal(got, ref)
+    S1, S2 = wl.Q[:128], wl.Q[128:256]
+    width = np.full(S1.shape[1], 0.1)
+    eref, _ = oc.check_motions(S1, S2, True, list(width), 0.0, 8)
+    egot, _ = BoxMotionValidator(pc, width).check_motions(S1, S2)
+    assert np.array_equal(egot, eref.astype(bool))
