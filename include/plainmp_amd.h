@@ -198,7 +198,10 @@ pmp_status pmp_pair_clearance_vjp(pmp_context* ctx, const double* q, int64_t ldq
 /* Edge validity: valid[e] for the motion s1[e] -> s2[e] (columns as q,
  * leading dimension lds).  Replaces CustomValidatorBase::checkMotion with the
  * box or euclidean step ratio (ompl/motion_validator.cpp:17-126).
- * n_checked (device, optional): waypoints evaluated per edge. */
+ * n_checked (device, optional): states checkMotion evaluates in its visiting
+ * order, i.e. up to and including the first invalid one (0 for an infinite
+ * step ratio, -1 for an edge needing more than 2^24 states, which is reported
+ * invalid). */
 pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
                                  const pmp_validator_desc* validator, uint8_t* valid, int32_t* n_checked);
 
@@ -211,6 +214,10 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
                                  pmp_graph** out);
 pmp_status pmp_graph_launch(pmp_graph* graph);
 pmp_status pmp_graph_destroy(pmp_graph* graph);
+/* Duration of the edge kernel in the most recent replay, from HIP events
+ * recorded around it inside the graph (waits for that replay).  No reference
+ * counterpart: it serves throughput measurement. */
+pmp_status pmp_graph_kernel_ms(pmp_graph* graph, float* ms);
 
 /* Primitive SDF of points (FP64): out[i] = min over the given primitives of
  * the signed distance at points[k * N + i] (mode 0), or is_outside(p, radius)

@@ -107,6 +107,20 @@ std::vector<uint8_t> sequence_table() {
 
 }  // namespace
 
+namespace {
+// Where a node's frame can be, for the static part of the cull table.
+// World rotation: |q|^2 in [n2_lo, n2_hi], so |R v| <= grow |v|.  Origin:
+// within rad of anchor (rad infinite when it depends on a prismatic joint
+// or a moving base).
+struct NodeReach {
+  double n2_lo = 1.0, n2_hi = 1.0, grow = 1.0;
+  bool rot_static = false;
+  double q[4] = {0, 0, 0, 1}, t[3] = {0, 0, 0};  // the exact pose when rot_static
+  double anchor[3] = {0, 0, 0};
+  double rad = INFINITY;
+};
+}  // namespace
+
 struct pmp_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -121,12 +135,19 @@ struct pmp_context {
   KinArgs<double> k64{};
   int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
   DevBuf rad_out;  // sphere radius by group-order index (FP32)
+  // host copies for the cull table (rebuilt whenever robot, scene or band change)
+  std::vector<DevSphereBlock> h_blocks;
+  std::vector<int> h_block_node;
+  std::vector<NodeReach> h_reach;
   // scene
   DevBuf prims32, prims64, vprims32, tiles32, tiles64, tileB, pts64;
   SceneArgs<float> s32{};
   SceneArgs<double> s64{};
   int n_clouds = 0;
   std::vector<CloudRange> clouds;
+  std::vector<DevPrim<double>> h_vprims;  // kind-sorted, FP64 bounds
+  int h_vrange[kNumKinds + 1] = {0};
+  DevBuf cull, cull_range;
   // misc
   DevBuf seq, counter, width, inv_width, tmp_prims, tmp_cloud, ws_cen, ws_cloud, ws_flags;
 };
@@ -135,12 +156,168 @@ struct pmp_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the edge kernel
 };
 
 namespace {
 
 void set_device(pmp_context* ctx) { PMP_HIP(hipSetDevice(ctx->device)); }
 hipStream_t st(pmp_context* ctx) { return ctx->stream; }
+
+// FP64 quaternion product (Eigen order) and q * v (Eigen's formula, which for
+// a quaternion s q' with unit q' gives (1 - s^2) v + s^2 R(q') v).
+void hq_mul(const double* a, const double* b, double* o) {
+  o[0] = a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1];
+  o[1] = a[3] * b[1] - a[0] * b[2] + a[1] * b[3] + a[2] * b[0];
+  o[2] = a[3] * b[2] + a[0] * b[1] - a[1] * b[0] + a[2] * b[3];
+  o[3] = a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2];
+}
+void hq_rot(const double* q, const double* v, double* o) {
+  const double t[3] = {2 * (q[1] * v[2] - q[2] * v[1]), 2 * (q[2] * v[0] - q[0] * v[2]),
+                       2 * (q[0] * v[1] - q[1] * v[0])};
+  o[0] = v[0] + q[3] * t[0] + (q[1] * t[2] - q[2] * t[1]);
+  o[1] = v[1] + q[3] * t[1] + (q[2] * t[0] - q[0] * t[2]);
+  o[2] = v[2] + q[3] * t[2] + (q[0] * t[1] - q[1] * t[0]);
+}
+
+// Bounds of every node frame for the cull table, walking the tree from the
+// base: the quaternion norm^2 multiplies along the chain (a unit-length
+// joint axis keeps the joint quaternion's norm in [min(1,|a|^2), max(1,|a|^2)]
+// up to the sin/cos error), and a node origin lies within the parent's
+// reach plus grow * |translation| of it.
+std::vector<NodeReach> node_reach(const pmp_robot_desc* r) {
+  constexpr double kSinCos = 1e-4;  // |s^2 + c^2 - 1| of the FP32 half-angle sin/cos (Taylor or exact)
+  constexpr double kRound = 1e-6;   // relative FP32 rounding of |q|^2 per composition
+  constexpr double kPad = 1e-5;     // margin between the bounds and what a lane may compute
+  const int N = r->n_nodes;
+  std::vector<NodeReach> R(N);
+  auto finish = [&](NodeReach& c) {
+    c.n2_lo *= 1.0 - kPad;
+    c.n2_hi *= 1.0 + kPad;
+    c.grow = 1.0 + 2.0 * std::max(0.0, std::max(c.n2_hi - 1.0, 1.0 - c.n2_lo));
+  };
+  NodeReach& b = R[0];
+  if (r->base_type == PMP_BASE_FIXED) {
+    double n2 = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      b.q[k] = r->base_pose[3 + k];
+      n2 += b.q[k] * b.q[k];
+    }
+    for (int k = 0; k < 3; ++k) b.t[k] = b.anchor[k] = r->base_pose[k];
+    b.n2_lo = n2 * (1.0 - kRound);
+    b.n2_hi = n2 * (1.0 + kRound);
+    b.rot_static = true;
+    b.rad = 0.0;
+  } else {
+    b.n2_lo = 1.0 - 10 * kRound;  // FP32 quaternion of the base roll / pitch / yaw
+    b.n2_hi = 1.0 + 10 * kRound;
+  }
+  finish(b);
+  for (int i = 1; i < N; ++i) {
+    const NodeReach& P = R[r->node_parent[i]];
+    NodeReach& C = R[i];
+    const int kind = r->node_kind[i];
+    const double* oq = r->node_quat + 4 * i;
+    const double* lt = r->node_trans + 3 * i;
+    const double* ax = r->node_axis + 3 * i;
+    const bool ident = r->node_identity[i] != 0;
+    const double oq2 = ident ? 1.0 : oq[0] * oq[0] + oq[1] * oq[1] + oq[2] * oq[2] + oq[3] * oq[3];
+    double lo = P.n2_lo, hi = P.n2_hi;
+    if (kind == PMP_NODE_FIXED) {
+      lo *= oq2;
+      hi *= oq2;
+    } else if (kind == PMP_NODE_REVOLUTE) {
+      const double a2 = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2];
+      lo *= oq2 * std::min(1.0, a2) * (1.0 - kSinCos);
+      hi *= oq2 * std::max(1.0, a2) * (1.0 + kSinCos);
+    }
+    C.n2_lo = lo * (1.0 - kRound);
+    C.n2_hi = hi * (1.0 + kRound);
+    const double ltn = std::sqrt(lt[0] * lt[0] + lt[1] * lt[1] + lt[2] * lt[2]);
+    if (kind == PMP_NODE_ROOT) {
+      C = R[0];
+      continue;
+    } else if (kind == PMP_NODE_PRISMATIC || !std::isfinite(P.rad)) {
+      C.rad = INFINITY;
+    } else if (P.rot_static) {
+      double tw[3];
+      hq_rot(P.q, lt, tw);
+      for (int k = 0; k < 3; ++k) C.t[k] = C.anchor[k] = P.t[k] + tw[k];
+      C.rad = 0.0;
+      if (kind == PMP_NODE_FIXED) {
+        C.rot_static = true;
+        if (ident)
+          for (int k = 0; k < 4; ++k) C.q[k] = P.q[k];
+        else
+          hq_mul(P.q, oq, C.q);
+      } else if (kind == PMP_NODE_TRANSLATION) {
+        C.rot_static = true;
+        for (int k = 0; k < 4; ++k) C.q[k] = P.q[k];
+      }
+    } else {
+      for (int k = 0; k < 3; ++k) C.anchor[k] = P.anchor[k];
+      C.rad = P.rad + P.grow * ltn;
+    }
+    finish(C);
+  }
+  return R;
+}
+
+float round_down(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+float round_up(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = std::nextafter(f, INFINITY);
+  return f;
+}
+
+// Candidates of every sphere block: each non-cloud primitive whose AABB,
+// grown by the block's reach, the block's bounding centre can enter; with a
+// fixed base a primitive is dropped when the grown box lies outside the ball
+// that the centre can ever reach.
+void rebuild_cull(pmp_context* ctx) {
+  const int nb = (int)ctx->h_blocks.size();
+  std::vector<CullEntry> ent;
+  std::vector<CullRange> rng(std::max(nb, 1));
+  for (int b = 0; b < nb; ++b) {
+    const NodeReach& nr = ctx->h_reach[ctx->h_block_node[b]];
+    const DevSphereBlock& bk = ctx->h_blocks[b];
+    const double reach = (double)bk.G * nr.grow + (double)ctx->band + 1e-4;
+    const double gn = std::sqrt((double)bk.g[0] * bk.g[0] + (double)bk.g[1] * bk.g[1] + (double)bk.g[2] * bk.g[2]);
+    const double ball = nr.rad + nr.grow * gn + 1e-4;
+    for (int k = 0; k < kNumKinds; ++k) {
+      rng[b].at[k] = (int)ent.size();
+      if (k == kKindCloud) continue;
+      for (int p = ctx->h_vrange[k]; p < ctx->h_vrange[k + 1]; ++p) {
+        const DevPrim<double>& P = ctx->h_vprims[p];
+        double lo[3], hi[3], d2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = P.lb[d] - reach;
+          hi[d] = P.ub[d] + reach;
+          const double e = std::max(std::max(lo[d] - nr.anchor[d], nr.anchor[d] - hi[d]), 0.0);
+          d2 += e * e;
+        }
+        if (std::isfinite(ball) && std::sqrt(d2) > ball) continue;  // NaN keeps the candidate
+        CullEntry ce{};
+        for (int d = 0; d < 3; ++d) {
+          ce.lb[d] = round_down(lo[d]);
+          ce.ub[d] = round_up(hi[d]);
+        }
+        ce.prim = p;
+        ent.push_back(ce);
+      }
+    }
+    rng[b].at[kNumKinds] = (int)ent.size();
+  }
+  hipStream_t s0 = st(ctx);
+  const CullEntry* c = ctx->cull.upload(ent, s0);
+  const CullRange* cr = ctx->cull_range.upload(rng, s0);
+  ctx->s32.cull = ctx->s64.cull = c;
+  ctx->s32.cull_range = ctx->s64.cull_range = cr;
+}
 
 template <class T>
 DevPrim<T> make_prim(const pmp_prim_desc& d, int cloud_begin) {
@@ -344,7 +521,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
     for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks,
-                      &ctx->vprims32, &ctx->rad_out,
+                      &ctx->vprims32, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
                       &ctx->ws_cloud, &ctx->ws_flags})
@@ -374,6 +551,10 @@ pmp_status pmp_set_band(pmp_context* ctx, double band) {
     PMP_REQUIRE(ctx != nullptr, "null context");
     PMP_REQUIRE(band >= 0.0 && band < 1.0, "band must be in [0, 1) m");
     ctx->band = (float)band;
+    if (ctx->has_robot) {
+      set_device(ctx);
+      rebuild_cull(ctx);
+    }
   });
 }
 
@@ -509,7 +690,8 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     // their longest extent and cut into runs of at most kBlockSpheres, each
     // with a bounding sphere (rounded up to FP32) in the link frame.
     std::vector<DevSphereBlock> blocks;
-    std::vector<int> blk_begin(N, 0), blk_end(N, 0);
+    std::vector<int> blk_begin(N, 0), blk_end(N, 0), block_node;
+    const std::vector<NodeReach> reach = node_reach(r);
     for (int i = 0; i < N; ++i) {
       blk_begin[i] = (int)blocks.size();
       std::vector<int> mem;
@@ -554,6 +736,7 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
           if ((double)bk.G < G) bk.G = std::nextafter(bk.G, INFINITY);
           if (!(G == G)) bk.G = INFINITY;  // never cull around a NaN bound
           blocks.push_back(bk);
+          block_node.push_back(i);
         }
       }
       blk_end[i] = (int)blocks.size();
@@ -580,6 +763,8 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
         nd.col_mask = mask[i];
         nd.block_begin = blk_begin[i];
         nd.block_end = blk_end[i];
+        nd.n2_lo = (T)reach[i].n2_lo;
+        nd.n2_hi = (T)reach[i].n2_hi;
         for (int k = 0; k < 4; ++k) nd.quat[k] = (T)r->node_quat[4 * i + k];
         for (int k = 0; k < 3; ++k) {
           nd.trans[k] = (T)r->node_trans[3 * i + k];
@@ -648,7 +833,11 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     ctx->n_groups = G;
     ctx->n_cols = n_cols;
     ctx->n_nodes = N;
+    ctx->h_blocks = blocks;
+    ctx->h_block_node = block_node;
+    ctx->h_reach = reach;
     ctx->has_robot = true;
+    rebuild_cull(ctx);
   });
 }
 
@@ -683,11 +872,15 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     // kind-sorted copy for the validity pass (its answer does not depend on
     // the primitive order)
     std::vector<DevPrim<float>> v32;
+    std::vector<DevPrim<double>> v64;
     int vrange[kNumKinds + 1] = {0};
     for (int k = 0; k < kNumKinds; ++k) {
       vrange[k] = (int)v32.size();
-      for (const DevPrim<float>& p : p32)
-        if (prim_kind(p) == k) v32.push_back(p);
+      for (size_t i = 0; i < p32.size(); ++i)
+        if (prim_kind(p32[i]) == k) {
+          v32.push_back(p32[i]);
+          v64.push_back(p64[i]);
+        }
     }
     vrange[kNumKinds] = (int)v32.size();
     set_device(ctx);
@@ -709,6 +902,9 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     ctx->s64 = d;
     ctx->n_clouds = n_clouds;
     ctx->clouds = clouds;
+    ctx->h_vprims = v64;
+    for (int k = 0; k <= kNumKinds; ++k) ctx->h_vrange[k] = vrange[k];
+    if (ctx->has_robot) rebuild_cull(ctx);
   });
 }
 
@@ -910,9 +1106,11 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
     auto* g = new pmp_graph();
     g->stream = cs;
     try {
+      PMP_HIP(hipEventCreate(&g->ev0));
+      PMP_HIP(hipEventCreate(&g->ev1));
       PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       hipError_t le = launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block, E),
-                                   block, cs);
+                                   block, cs, g->ev0, g->ev1);
       hipError_t ce = hipStreamEndCapture(cs, &g->graph);
       PMP_HIP(le);
       PMP_HIP(ce);
@@ -920,6 +1118,8 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
     } catch (...) {
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
       if (g->graph) (void)hipGraphDestroy(g->graph);
+      if (g->ev0) (void)hipEventDestroy(g->ev0);
+      if (g->ev1) (void)hipEventDestroy(g->ev1);
       delete g;
       throw;
     }
@@ -940,7 +1140,17 @@ pmp_status pmp_graph_destroy(pmp_graph* g) {
     (void)hipStreamSynchronize(g->stream);
     if (g->exec) PMP_HIP(hipGraphExecDestroy(g->exec));
     if (g->graph) PMP_HIP(hipGraphDestroy(g->graph));
+    if (g->ev0) PMP_HIP(hipEventDestroy(g->ev0));
+    if (g->ev1) PMP_HIP(hipEventDestroy(g->ev1));
     delete g;
+  });
+}
+
+pmp_status pmp_graph_kernel_ms(pmp_graph* g, float* ms) {
+  return guarded([&] {
+    PMP_REQUIRE(g != nullptr && ms != nullptr, "null argument");
+    PMP_HIP(hipEventSynchronize(g->ev1));
+    PMP_HIP(hipEventElapsedTime(ms, g->ev0, g->ev1));
   });
 }
 

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   float best[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   for (int t = 0; t < n_tiles; ++t) {
-    const CloudTile T = tiles[tile_begin + t];
+    const CloudTile T = cload(tiles, tile_begin + t);
     const float dx = fmaxf(fmaxf(T.lb[0] - cx, cx - T.ub[0]), 0.f);
     const float dy = fmaxf(fmaxf(T.lb[1] - cy, cy - T.ub[1]), 0.f);
     const float dz = fmaxf(fmaxf(T.lb[2] - cz, cz - T.ub[2]), 0.f);

@@ -80,7 +80,7 @@ struct EdgeQ {
   double t;
   bool at_end;
   __device__ __forceinline__ double get(int col) const {
-    double a = s1[(int64_t)col * lds + e], b = s2[(int64_t)col * lds + e];
+    double a = cload(s1, (int64_t)col * lds + e), b = cload(s2, (int64_t)col * lds + e);
     return at_end ? b : a + (b - a) * t;
   }
 };
@@ -123,7 +123,7 @@ __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool 
   Pose<T> prev = base;
   for (int n = 0; n < ka.n_nodes; ++n) {
     if (!wave_any(vis.alive())) return;
-    const DevNode<T>& nd = ka.nodes[n];
+    const DevNode<T> nd = cload(ka.nodes, n);
     // parent_slot: -2 the previous node, -1 the base pose, >= 0 a kept pose
     const Pose<T> par = nd.parent_slot == -2 ? prev : (nd.parent_slot == -1 ? base : slots.get(nd.parent_slot));
     const V3<T> lt{nd.trans[0], nd.trans[1], nd.trans[2]};
@@ -169,7 +169,7 @@ __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool 
     } else if (nd.sphere_begin < nd.sphere_end) {
       const M3<T> R = qmat(P.q);
       for (int s = nd.sphere_begin; s < nd.sphere_end; ++s) {
-        const DevSphere<T> sp = ka.spheres[s];
+        const DevSphere<T> sp = cload(ka.spheres, s);
         V3<T> c = add(mv(R, V3<T>{sp.x, sp.y, sp.z}), P.t);
         vis.sphere(s, c, sp.r);
       }
@@ -180,14 +180,14 @@ __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool 
   if constexpr (!Vis::kSelf) return;
   for (int k = 0; k < ka.n_pairs; ++k) {
     if (!wave_any(vis.alive())) return;
-    const DevSelfPair pr = ka.pairs[k];
+    const DevSelfPair pr = cload(ka.pairs, k);
     const Pose<T> P1 = slots.get(pr.slot1), P2 = slots.get(pr.slot2);
     const M3<T> R1 = qmat(P1.q), R2 = qmat(P2.q);
     for (int i = pr.begin1; i < pr.end1; ++i) {
-      const DevSphere<T> a = ka.spheres[i];
+      const DevSphere<T> a = cload(ka.spheres, i);
       const V3<T> ci = add(mv(R1, V3<T>{a.x, a.y, a.z}), P1.t);
       for (int j = pr.begin2; j < pr.end2; ++j) {
-        const DevSphere<T> bb = ka.spheres[j];
+        const DevSphere<T> bb = cload(ka.spheres, j);
         const V3<T> cj = add(mv(R2, V3<T>{bb.x, bb.y, bb.z}), P2.t);
         vis.self(pr, i, j, ci, cj, a.r + bb.r);
       }

@@ -22,6 +22,22 @@
 #define PMP_HD __host__ __device__ __forceinline__
 
 namespace pmp {
+// Read-only table access.  The robot, scene and edge tables are never written
+// while a kernel runs, so they are read through the constant address space
+// (by value, so the address space is kept up to the load): with a
+// wave-uniform index the compiler then issues scalar loads into SGPRs instead
+// of 64 identical vector loads, even in kernels that store to other buffers.
+template <class T>
+__device__ __forceinline__ T cload(const T* p, int64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+  return p[i];  // host pass: never executed, only type-checked
+#endif
+}
+}  // namespace pmp
+
+namespace pmp {
 
 template <class T>
 struct V3 {

@@ -451,6 +451,14 @@ PYBIND11_MODULE(_native, m) {
   };
   py::class_<Graph, std::shared_ptr<Graph>>(mo, "EdgeGraph")
       .def("launch", [](Graph& g) { check(pmp_graph_launch(g.g)); })
+      .def("kernel_ms", [](Graph& g) {
+        float ms = 0.f;
+        {
+          py::gil_scoped_release rel;
+          check(pmp_graph_kernel_ms(g.g, &ms));
+        }
+        return ms;
+      })
       .def("destroy", [](Graph& g) {
         if (g.g) check(pmp_graph_destroy(g.g));
         g.g = nullptr;

@@ -6,6 +6,7 @@
 // "batch index fastest" layout so that every store of a wave is one
 // coalesced 256-byte line.
 #include <float.h>
+#include <limits.h>
 
 #include "fk.hpp"
 #include "launch.h"
@@ -18,8 +19,8 @@ constexpr float kCutoff = 0.1f;  // SphereCollisionCst::cutoff_dist_ (primitive_
 constexpr double kMaxEdgeStates = 16777216.0;
 
 template <class T>
-__device__ __forceinline__ const DevPrim<T>& prim_at(const SceneArgs<T>& sc, int p) {
-  return reinterpret_cast<const DevPrim<T>*>(sc.prims)[p];
+__device__ __forceinline__ DevPrim<T> prim_at(const SceneArgs<T>& sc, int p) {
+  return cload(reinterpret_cast<const DevPrim<T>*>(sc.prims), p);
 }
 
 __device__ __forceinline__ int64_t gid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
@@ -38,7 +39,7 @@ struct CentersVis {
   __device__ void node(int, const DevNode<T>&, const Pose<T>&) {}
   __device__ void sphere(int s, V3<T> c, T) {
     if (!active) return;
-    const int o = ka.meta[s].out;
+    const int o = cload(ka.meta, s).out;
     out[(int64_t)(o * 3 + 0) * B + b] = (float)c.x;
     out[(int64_t)(o * 3 + 1) * B + b] = (float)c.y;
     out[(int64_t)(o * 3 + 2) * B + b] = (float)c.z;
@@ -105,7 +106,7 @@ struct ClearanceVis {
       best = fminf(best, sdf_eval(P, c) - r);
     }
     if (!active) return;
-    const int64_t o = ka.meta[s].out;
+    const int64_t o = cload(ka.meta, s).out;
     out[o * B + b] = best;
     if (cen) {
       cen[(o * 3 + 0) * B + b] = c.x;
@@ -146,19 +147,24 @@ struct ValidVis32 {
   __device__ void node(int, const DevNode<float>&, const Pose<float>&) {}
   __device__ void sphere(int, V3<float>, float) {}
 
-  // One kind of primitive against the (up to eight) spheres of a block.  A
-  // primitive is skipped for the whole wave when no live lane's block
-  // bounding sphere, grown by `reach`, touches its AABB: every member sphere
-  // is then outside the AABB inflated by its radius plus the band, which is
-  // the reference's own AABB pre-test failing with room to spare, so the
-  // skipped pairs can neither collide nor fall in the band.
+  // One kind of primitive against the (up to eight) spheres of a block.  The
+  // block's candidates carry the primitive's AABB grown by the block's reach
+  // (bounding radius times the rotation growth bound, plus band and slack).
+  // A candidate is skipped for the whole wave when no live lane's bounding
+  // centre lies in that grown box: every member sphere is then outside the
+  // AABB inflated by its radius plus the band, which is the reference's own
+  // AABB pre-test failing with room to spare, so the skipped pairs can
+  // neither collide nor fall in the band.
   template <int K>
   __device__ __forceinline__ void kind_loop(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
-                                            V3<float> g, float reach) {
+                                            V3<float> g, const CullRange& cr) {
     const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
-    for (int p = sc.vrange[K]; p < sc.vrange[K + 1]; ++p) {
-      const DevPrim<float>& P = vp[p];
-      if (!wave_any(alive() && !outside_aabb(P, g, reach))) continue;
+    for (int i = cr.at[K]; i < cr.at[K + 1]; ++i) {
+      const CullEntry ce = cload(sc.cull, i);
+      const bool out = g.x < ce.lb[0] || g.x > ce.ub[0] || g.y < ce.lb[1] || g.y > ce.ub[1] || g.z < ce.lb[2] ||
+                       g.z > ce.ub[2];
+      if (!wave_any(alive() && !out)) continue;
+      const DevPrim<float> P = cload(vp, ce.prim);
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         if (k < n) {
@@ -174,14 +180,14 @@ struct ValidVis32 {
   __device__ void blocks(const DevNode<float>& nd, const Pose<float>& P) {
     if (nd.block_begin >= nd.block_end) return;
     const M3<float> R = qmat(P.q);
-    // |R v| <= (1 + 2 | |q|^2 - 1 |) |v| for the rotation matrix of a
-    // quaternion that is not exactly unit (Taylor sin/cos, URDF rounding)
+    // The cull table assumes |q|^2 of this rotation within [n2_lo, n2_hi]
+    // (so that |R v| <= grow |v|); a lane outside is left to the FP64 pass.
     const float n2 = P.q.x * P.q.x + P.q.y * P.q.y + P.q.z * P.q.z + P.q.w * P.q.w;
-    const float grow = 1.0f + 2.0f * fabsf(n2 - 1.0f);
+    if (!(n2 >= nd.n2_lo && n2 <= nd.n2_hi)) uncertain = true;
     const int n_solid = sc.vrange[kKindCloud];
     for (int bi = nd.block_begin; bi < nd.block_end; ++bi) {
       if (!wave_any(alive())) return;
-      const DevSphereBlock& bk = ka.blocks[bi];
+      const DevSphereBlock bk = cload(ka.blocks, bi);
       const int n = bk.n;
       V3<float> c[kBlockSpheres];
       bool bad = false;
@@ -194,8 +200,6 @@ struct ValidVis32 {
         }
       }
       const V3<float> g = add(mv(R, V3<float>{bk.g[0], bk.g[1], bk.g[2]}), P.t);
-      // slack 1e-4 m covers the FP32 rounding of the centres and the AABB
-      const float reach = bk.G * grow + band + 1e-4f;
       if (bad && n_solid > 0) definite = true;  // NaN centres collide with every primitive
       if (cw && !idle) {
         const int64_t B = cw->B;
@@ -211,7 +215,7 @@ struct ValidVis32 {
         // clouds: queried by the matrix-core pass unless the AABB pre-test culls it
         const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
         for (int p = sc.vrange[kKindCloud]; p < sc.vrange[kKindCloud + 1]; ++p) {
-          const DevPrim<float>& Pc = vp[p];
+          const DevPrim<float> Pc = cload(vp, p);
           const int oc = (Pc.flags >> 4) & 15;
 #pragma unroll
           for (int k = 0; k < kBlockSpheres; ++k) {
@@ -227,12 +231,13 @@ struct ValidVis32 {
         }
       }
       if (!wave_any(alive())) continue;
-      kind_loop<kKindBoxAA>(c, bk, n, g, reach);
-      kind_loop<kKindBoxRot>(c, bk, n, g, reach);
-      kind_loop<kKindCylZ>(c, bk, n, g, reach);
-      kind_loop<kKindCylRot>(c, bk, n, g, reach);
-      kind_loop<kKindSphere>(c, bk, n, g, reach);
-      kind_loop<kKindGround>(c, bk, n, g, reach);
+      const CullRange cr = cload(sc.cull_range, bi);
+      kind_loop<kKindBoxAA>(c, bk, n, g, cr);
+      kind_loop<kKindBoxRot>(c, bk, n, g, cr);
+      kind_loop<kKindCylZ>(c, bk, n, g, cr);
+      kind_loop<kKindCylRot>(c, bk, n, g, cr);
+      kind_loop<kKindSphere>(c, bk, n, g, cr);
+      kind_loop<kKindGround>(c, bk, n, g, cr);
     }
   }
 
@@ -261,7 +266,7 @@ struct ValidVis64 {
   __device__ void begin(const Pose<double>&, const QS&) {}
   __device__ void node(int, const DevNode<double>&, const Pose<double>&) {}
   __device__ void sphere(int s, V3<double> c, double r) {
-    if (collided || ka.meta[s].ext < 0) return;
+    if (collided || cload(ka.meta, s).ext < 0) return;
     for (int p = 0; p < sc.n_prims; ++p) {
       const DevPrim<double>& P = prim_at(sc, p);
       if (P.type == PMP_PRIM_CLOUD) {
@@ -271,7 +276,7 @@ struct ValidVis64 {
         const double R = r + P.p0, lim = R * R * (1.0 + 1e-12) + 1e-300;
         double best = DBL_MAX;
         for (int t = 0; t < P.cloud_count; ++t) {
-          const CloudTile64 T = sc.tiles64[P.cloud_begin + t];
+          const CloudTile64 T = cload(sc.tiles64, P.cloud_begin + t);
           const double dx = fmax(fmax(T.lb[0] - c.x, c.x - T.ub[0]), 0.0);
           const double dy = fmax(fmax(T.lb[1] - c.y, c.y - T.ub[1]), 0.0);
           const double dz = fmax(fmax(T.lb[2] - c.z, c.z - T.ub[2]), 0.0);
@@ -342,19 +347,19 @@ __device__ __forceinline__ double edge_step_ratio(const EdgeArgs& ea, int D, int
     double max_diff = -INFINITY, diff_longest = 0.0;
     int longest = 0;
     for (int i = 0; i < D; ++i) {
-      const double diff = ea.s2[(int64_t)i * ea.lds + e] - ea.s1[(int64_t)i * ea.lds + e];
-      const double asd = fabs(diff) * ea.inv_width[i];
+      const double diff = cload(ea.s2, (int64_t)i * ea.lds + e) - cload(ea.s1, (int64_t)i * ea.lds + e);
+      const double asd = fabs(diff) * cload(ea.inv_width, i);
       if (asd > max_diff) {
         max_diff = asd;
         longest = i;
         diff_longest = diff;
       }
     }
-    return fabs(diff_longest) < 1e-6 ? INFINITY : ea.width[longest] / fabs(diff_longest);
+    return fabs(diff_longest) < 1e-6 ? INFINITY : cload(ea.width, longest) / fabs(diff_longest);
   }
   double d2 = 0.0;
   for (int i = 0; i < D; ++i) {
-    const double diff = ea.s1[(int64_t)i * ea.lds + e] - ea.s2[(int64_t)i * ea.lds + e];
+    const double diff = cload(ea.s1, (int64_t)i * ea.lds + e) - cload(ea.s2, (int64_t)i * ea.lds + e);
     d2 += diff * diff;
   }
   return ea.resolution / sqrt(d2);
@@ -371,7 +376,18 @@ __device__ __forceinline__ EdgeQ edge_waypoint(const EdgeArgs& ea, int64_t e, in
   return EdgeQ{ea.s1, ea.s2, ea.lds, e, t, k == 0};
 }
 
-// valid[e]: 0 invalid, 1 valid, 2 pending FP64 re-decision.
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) v = min(v, __shfl_xor(v, s, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// valid[e]: 0 invalid, 1 valid, 2 pending FP64 re-decision.  n_checked[e]:
+// the states the reference's checkMotion evaluates, i.e. the visiting-order
+// index of the first invalid state plus one, or every state of a valid edge
+// (0 for an edge whose step ratio is infinite, -1 for an edge that would need
+// more than kMaxEdgeStates states).  A count that depends on an in-band state
+// before the first certain collision is left to the FP64 pass as well.
 __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, EdgeArgs ea,
                                                     float band) {
   const int lane = threadIdx.x & 63;
@@ -392,21 +408,25 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
       } else {
         const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
         const int64_t n_wp = n_test - 1;  // s2 plus the interior waypoints
-        bool unc = false;
+        bool unc = false;                 // an in-band state seen so far
         for (int64_t base = 0; base < n_wp; base += 64) {
           const int64_t k = base + lane;
           const bool has = k < n_wp;
           ValidVis32 v{k32, s32, band};
           v.idle = !has;
           run_fk(k32, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), taylor, v);
-          checked = base + 64 < n_wp ? base + 64 : n_wp;
-          unc |= has && v.uncertain;
-          if (__any(has && v.definite)) {
-            state = 0;
+          const bool def = has && v.definite;
+          if (__any(def)) {
+            const int kmin = wave_min(def ? (int)k : INT_MAX);
+            const bool before = has && v.uncertain && (int)k < kmin;
+            state = (unc || __any(before)) ? 2 : 0;
+            checked = kmin + 1;
             break;
           }
+          unc |= __any(has && v.uncertain);
+          checked = base + 64 < n_wp ? base + 64 : n_wp;
         }
-        if (state == 1 && __any(unc)) state = 2;
+        if (state == 1 && unc) state = 2;
       }
     }
     if (lane == 0) {
@@ -426,19 +446,23 @@ __global__ void __launch_bounds__(64) edges_recheck_kernel(KinArgs<double> k64, 
   const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
   const int64_t n_wp = n_test - 1;
   bool ok = true;
+  int64_t checked = n_wp;
   for (int64_t base = 0; base < n_wp; base += 64) {
     const int64_t k = base + lane;
     const bool has = k < n_wp;
     ValidVis64 v{k64, s64};
     v.collided = !has;
     run_fk(k64, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), k64.base_type == PMP_BASE_FIXED, v);
-    if (__any(has && v.collided)) {
+    const bool hit = has && v.collided;
+    if (__any(hit)) {
       ok = false;
+      checked = wave_min(hit ? (int)k : INT_MAX) + 1;
       break;
     }
   }
   if (lane == 0) {
     ea.valid[e] = ok ? 1 : 0;
+    if (ea.n_checked) ea.n_checked[e] = (int32_t)checked;
     if (ea.n_recheck) atomicAdd(ea.n_recheck, 1);
   }
 }
@@ -524,7 +548,7 @@ struct EvalVis {
   // evaluate_ext_collision (constraints/primitive_sphere_collision.cpp:256-322): the minimum
   // over (group, primitive, sphere) with the reference's visiting order as tie-break key.
   __device__ void sphere(int s, V3<float> c, float r) {
-    const DevSphereMeta m = ka.meta[s];
+    const DevSphereMeta m = cload(ka.meta, s);
     if (m.ext < 0) return;
     for (int p = 0; p < sc.n_prims; ++p) {
       const DevPrim<float>& P = prim_at(sc, p);
@@ -572,7 +596,7 @@ __global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneA
     uint32_t mask = 0;
     if (v.bs >= 0) {
       g = sdf_grad(prim_at(sc, v.bp), v.bc);
-      mask = ka.nodes[ka.meta[v.bs].node].col_mask;
+      mask = cload(ka.nodes, cload(ka.meta, v.bs).node).col_mask;
     }
 #pragma unroll
     for (int j = 0; j < MAXC; ++j)
@@ -586,8 +610,8 @@ __global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneA
     if (v.si >= 0) {
       const V3<float> d = sub(v.sci, v.scj);
       n = scale(1.0f / sqrtf(dot(d, d)), d);
-      m1 = ka.nodes[ka.meta[v.si].node].col_mask;
-      m2 = ka.nodes[ka.meta[v.sj].node].col_mask;
+      m1 = cload(ka.nodes, cload(ka.meta, v.si).node).col_mask;
+      m2 = cload(ka.nodes, cload(ka.meta, v.sj).node).col_mask;
     }
 #pragma unroll
     for (int j = 0; j < MAXC; ++j)
@@ -616,8 +640,8 @@ struct PairVis {
     fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
   }
   __device__ void sphere(int s, V3<float> c, float r) {
-    const DevSphereMeta m = ka.meta[s];
-    const uint32_t mask = ka.nodes[m.node].col_mask;
+    const DevSphereMeta m = cload(ka.meta, s);
+    const uint32_t mask = cload(ka.nodes, m.node).col_mask;
     const int D = ka.n_cols, NP = sc.n_prims;
     V3<float> J[MAXC];
 #pragma unroll
@@ -669,8 +693,8 @@ struct VjpVis {
     fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
   }
   __device__ void sphere(int s, V3<float> c, float) {
-    const DevSphereMeta m = ka.meta[s];
-    const uint32_t mask = ka.nodes[m.node].col_mask;
+    const DevSphereMeta m = cload(ka.meta, s);
+    const uint32_t mask = cload(ka.nodes, m.node).col_mask;
     const int NP = sc.n_prims;
     V3<float> g{0.f, 0.f, 0.f};
     for (int p = 0; p < NP; ++p) {
@@ -810,9 +834,17 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
 }
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
-                        hipStream_t st) {
+                        hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after) {
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ea.counter, 2);
+  if (ev_before) {
+    hipError_t e = hipEventRecord(ev_before, st);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(edges_kernel, dim3(grid), dim3(block), 0, st, k32, s32, ea, band);
+  if (ev_after) {
+    hipError_t e = hipEventRecord(ev_after, st);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(edges_recheck_kernel, dim3((unsigned)ea.E), dim3(64), 0, st, k64, s64, ea);
   return hipGetLastError();
 }

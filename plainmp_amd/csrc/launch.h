@@ -47,9 +47,10 @@ hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, int tile
                             hipStream_t st);
 hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float* radius_out, float rc, int64_t B,
                                   int64_t S, hipStream_t st);
+// ev_before / ev_after (optional) are recorded around the FP32 edge kernel.
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
-                        hipStream_t st);
+                        hipStream_t st, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr);
 hipError_t launch_evaluate(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                            int64_t B, float* vals, float* jac, hipStream_t st);
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,

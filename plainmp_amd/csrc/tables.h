@@ -40,7 +40,7 @@ struct alignas(16) DevNode {
   T quat[4];
   T trans[3];
   T axis[3];
-  T pad[2];
+  T n2_lo, n2_hi;  // bounds of |q|^2 of this node's world rotation assumed by the cull table
 };
 
 template <class T>
@@ -57,6 +57,21 @@ struct alignas(16) DevSphereBlock {
   float g[3], G;
   float s[kBlockSpheres][4];   // link-frame centre and radius
 };
+
+// A candidate primitive of a sphere block: the primitive's AABB grown by the
+// block's reach (bounding radius times the rotation growth bound of its node,
+// plus the band and a rounding slack), rounded outwards to FP32.
+struct alignas(16) CullEntry {
+  float lb[3];
+  int32_t prim;  // index into the kind-sorted primitives
+  float ub[3];
+  int32_t pad;
+};
+// Per block: its candidates of kind k are cull[at[k], at[k + 1]).
+struct alignas(16) CullRange {
+  int32_t at[8];
+};
+static_assert(kNumKinds + 1 == 8, "CullRange holds one offset per kind plus the end");
 
 struct alignas(16) DevSphereMeta {
   int32_t out;    // index in SphereCollisionCst group order
@@ -114,6 +129,8 @@ struct SceneArgs {
   const double* pts64;          // [tile][64][4] FP64 world points (padding points far away)
   const void* vprims;           // DevPrim<T>* sorted by PrimKind (validity pass)
   int32_t vrange[kNumKinds + 1];  // kind k occupies vprims[vrange[k], vrange[k + 1])
+  const CullEntry* cull;        // candidates of every sphere block (FP32 validity pass)
+  const CullRange* cull_range;  // [n_blocks]
 };
 
 }  // namespace pmp

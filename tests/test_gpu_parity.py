@@ -62,6 +62,8 @@ def test_edges_mixed16_subset():
     got, n_got = mv.check_motions(wl.S1, wl.S2)
     _report("edges", E=len(ref), valid_frac=ref.mean(), mean_checked_ref=n_ref.mean(), mean_checked_gpu=n_got.mean())
     assert np.array_equal(got, ref.astype(bool))
+    # states evaluated in the reference's visiting order, up to the first invalid one
+    assert np.array_equal(n_got, n_ref)
     # single-edge entry point agrees with the batch
     for e in range(8):
         assert mv.checkMotion(wl.S1[e], wl.S2[e]) == bool(ref[e])
@@ -72,9 +74,10 @@ def test_edges_euclidean_subset():
 
     wl = synth.workload("edges_mixed16", scale=0.0625)
     oc, pc = oracle_cst(wl), product_cst(wl)
-    ref, _ = oc.check_motions(wl.S1, wl.S2, False, [], 0.1, 8)
-    got, _ = EuclideanMotionValidator(pc, 0.1).check_motions(wl.S1, wl.S2)
+    ref, n_ref = oc.check_motions(wl.S1, wl.S2, False, [], 0.1, 8)
+    got, n_got = EuclideanMotionValidator(pc, 0.1).check_motions(wl.S1, wl.S2)
     assert np.array_equal(got, ref.astype(bool))
+    assert np.array_equal(n_got, n_ref)
 
 
 def _second_gap(d):
