@@ -214,10 +214,14 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
                                  pmp_graph** out);
 pmp_status pmp_graph_launch(pmp_graph* graph);
 pmp_status pmp_graph_destroy(pmp_graph* graph);
-/* Duration of the edge kernel in the most recent replay, from HIP events
- * recorded around it inside the graph (waits for that replay).  No reference
- * counterpart: it serves throughput measurement. */
-pmp_status pmp_graph_kernel_ms(pmp_graph* graph, float* ms);
+/* Duration of the FP32 edge kernel of the most recent pmp_check_edges_batch
+ * on this context, from HIP events recorded around it on the context stream
+ * (waits for that kernel).  No reference counterpart: it serves throughput
+ * measurement. */
+pmp_status pmp_edge_kernel_ms(pmp_context* ctx, float* ms);
+/* Edges the most recent edge batch on this context re-decided in FP64
+ * (waits for it).  No reference counterpart: a diagnostic of the band. */
+pmp_status pmp_edge_recheck_count(pmp_context* ctx, int32_t* n);
 
 /* Primitive SDF of points (FP64): out[i] = min over the given primitives of
  * the signed distance at points[k * N + i] (mode 0), or is_outside(p, radius)

@@ -9,7 +9,7 @@
 // Eigen expression order the reference compiles to (Eigen built with
 // EIGEN_DONT_VECTORIZE, CMakeLists.txt:9-10).  Quirks are kept on purpose.
 //
-// Pinning (what checks each part; the tests live in tests/test_oracle_*.py):
+// Pinning (what checks each part; the tests live in tests/test_oracle.py):
 //   taylor_sincos        closed form + error bounds vs std::sin/cos + special cases
 //   KinematicModel FK    independent numpy 4x4 homogeneous FK (exact mode, 1e-12),
 //                        known answers (q=0, single-joint rotation), quirk URDFs

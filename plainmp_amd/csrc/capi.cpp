@@ -125,6 +125,7 @@ struct pmp_context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // used for graph capture when no stream is set
+  hipEvent_t edge_ev0 = nullptr, edge_ev1 = nullptr;  // around the last direct edge kernel
   float band = 1e-5f;
   pmp_schedule sched{0, 0, 0};
   int n_cu = 256;
@@ -140,7 +141,7 @@ struct pmp_context {
   std::vector<int> h_block_node;
   std::vector<NodeReach> h_reach;
   // scene
-  DevBuf prims32, prims64, vprims32, tiles32, tiles64, tileB, pts64;
+  DevBuf prims32, prims64, vprims32, vprims64, tiles32, tiles64, tileB, pts64;
   SceneArgs<float> s32{};
   SceneArgs<double> s64{};
   int n_clouds = 0;
@@ -156,7 +157,6 @@ struct pmp_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the edge kernel
 };
 
 namespace {
@@ -521,12 +521,14 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
     for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks,
-                      &ctx->vprims32, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
+                      &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
                       &ctx->ws_cloud, &ctx->ws_flags})
       b->release();
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    if (ctx->edge_ev0) (void)hipEventDestroy(ctx->edge_ev0);
+    if (ctx->edge_ev1) (void)hipEventDestroy(ctx->edge_ev1);
     delete ctx;
   });
 }
@@ -727,6 +729,7 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
             const double ex = coord(k, 0) - bk.g[0], ey = coord(k, 1) - bk.g[1], ez = coord(k, 2) - bk.g[2];
             G = std::max(G, std::sqrt(ex * ex + ey * ey + ez * ez) + sp64[k].r);
             bk.ext[t - a] = meta[k].ext;
+            bk.sph[t - a] = k;
             bk.s[t - a][0] = sp32[k].x;
             bk.s[t - a][1] = sp32[k].y;
             bk.s[t - a][2] = sp32[k].z;
@@ -889,7 +892,7 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     SceneArgs<double> d{};
     a.prims = ctx->prims32.upload(p32, s0);
     a.vprims = ctx->vprims32.upload(v32, s0);
-    d.vprims = nullptr;
+    d.vprims = ctx->vprims64.upload(v64, s0);
     for (int k = 0; k <= kNumKinds; ++k) a.vrange[k] = d.vrange[k] = vrange[k];
     d.prims = ctx->prims64.upload(p64, s0);
     a.tiles = d.tiles = ctx->tiles32.upload(t32, s0);
@@ -1084,8 +1087,12 @@ pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const doubl
     prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ea);
     if (E == 0) return;
     const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
+    if (!ctx->edge_ev0) {
+      PMP_HIP(hipEventCreate(&ctx->edge_ev0));
+      PMP_HIP(hipEventCreate(&ctx->edge_ev1));
+    }
     PMP_HIP(launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block, E), block,
-                         st(ctx)));
+                         st(ctx), ctx->edge_ev0, ctx->edge_ev1));
   });
 }
 
@@ -1106,11 +1113,9 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
     auto* g = new pmp_graph();
     g->stream = cs;
     try {
-      PMP_HIP(hipEventCreate(&g->ev0));
-      PMP_HIP(hipEventCreate(&g->ev1));
       PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       hipError_t le = launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block, E),
-                                   block, cs, g->ev0, g->ev1);
+                                   block, cs);
       hipError_t ce = hipStreamEndCapture(cs, &g->graph);
       PMP_HIP(le);
       PMP_HIP(ce);
@@ -1118,8 +1123,6 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
     } catch (...) {
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
       if (g->graph) (void)hipGraphDestroy(g->graph);
-      if (g->ev0) (void)hipEventDestroy(g->ev0);
-      if (g->ev1) (void)hipEventDestroy(g->ev1);
       delete g;
       throw;
     }
@@ -1140,17 +1143,27 @@ pmp_status pmp_graph_destroy(pmp_graph* g) {
     (void)hipStreamSynchronize(g->stream);
     if (g->exec) PMP_HIP(hipGraphExecDestroy(g->exec));
     if (g->graph) PMP_HIP(hipGraphDestroy(g->graph));
-    if (g->ev0) PMP_HIP(hipEventDestroy(g->ev0));
-    if (g->ev1) PMP_HIP(hipEventDestroy(g->ev1));
     delete g;
   });
 }
 
-pmp_status pmp_graph_kernel_ms(pmp_graph* g, float* ms) {
+pmp_status pmp_edge_recheck_count(pmp_context* ctx, int32_t* n) {
   return guarded([&] {
-    PMP_REQUIRE(g != nullptr && ms != nullptr, "null argument");
-    PMP_HIP(hipEventSynchronize(g->ev1));
-    PMP_HIP(hipEventElapsedTime(ms, g->ev0, g->ev1));
+    PMP_REQUIRE(ctx != nullptr && n != nullptr, "null argument");
+    PMP_REQUIRE(ctx->counter.ptr != nullptr, "no edge batch has run on this context");
+    set_device(ctx);
+    PMP_HIP(hipMemcpyAsync(n, (const int32_t*)ctx->counter.ptr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st(ctx)));
+    PMP_HIP(hipStreamSynchronize(st(ctx)));
+  });
+}
+
+pmp_status pmp_edge_kernel_ms(pmp_context* ctx, float* ms) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr && ms != nullptr, "null argument");
+    PMP_REQUIRE(ctx->edge_ev0 != nullptr, "no edge batch has run on this context");
+    set_device(ctx);
+    PMP_HIP(hipEventSynchronize(ctx->edge_ev1));
+    PMP_HIP(hipEventElapsedTime(ms, ctx->edge_ev0, ctx->edge_ev1));
   });
 }
 

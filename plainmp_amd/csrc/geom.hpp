@@ -320,20 +320,30 @@ PMP_HD V3<T> sdf_grad(const DevPrim<T>& p, V3<T> c) {
 
 // Reference is_outside (primitive_sdf.hpp:313-398, 464-498, 546-552, 246-248)
 // in its own branch structure; used in FP64 for the in-band re-decision.
-template <class T>
+// K >= 0 fixes the primitive kind at compile time (kind-sorted loops).
+template <class T, int K = -1>
 PMP_HD bool is_outside_ref(const DevPrim<T>& p, V3<T> c, T r) {
   V3<T> pos{p.pos[0], p.pos[1], p.pos[2]};
-  switch (p.type) {
+  const int type = K < 0 ? p.type
+                         : (K <= kKindBoxRot ? PMP_PRIM_BOX
+                                             : (K <= kKindCylRot ? PMP_PRIM_CYLINDER
+                                                                 : (K == kKindSphere ? PMP_PRIM_SPHERE : PMP_PRIM_GROUND)));
+  const int flags = K == kKindBoxAA    ? 1
+                    : K == kKindBoxRot ? (p.flags & ~1)
+                    : K == kKindCylZ   ? 2
+                    : K == kKindCylRot ? (p.flags & ~2)
+                                       : p.flags;
+  switch (type) {
     case PMP_PRIM_BOX: {
       T xs, ys, zs;
-      if (p.flags & 1) {
+      if (flags & 1) {
         xs = fabs(c.x - pos.x) - p.p0;
         if (xs > r) return true;
         ys = fabs(c.y - pos.y) - p.p1;
         if (ys > r) return true;
         zs = fabs(c.z - pos.z) - p.p2;
         if (zs > r) return true;
-      } else if (p.flags & 2) {
+      } else if (flags & 2) {
         zs = fabs(c.z - pos.z) - p.p2;
         if (zs > r) return true;
         V3<T> pc = sub(c, pos);
@@ -362,7 +372,7 @@ PMP_HD bool is_outside_ref(const DevPrim<T>& p, V3<T> c, T r) {
     }
     case PMP_PRIM_CYLINDER: {
       T zs, xa, ya;
-      if (p.flags & 2) {
+      if (flags & 2) {
         zs = fabs(c.z - pos.z) - p.p1;
         if (zs > r) return true;
         xa = fabs(c.x - pos.x);

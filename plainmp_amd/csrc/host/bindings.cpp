@@ -427,6 +427,9 @@ PYBIND11_MODULE(_native, m) {
              check(pmp_evaluate_batch(c.device_context((void*)stream), (const double*)q, ldq, B, (float*)vals,
                                       (float*)jac));
            })
+      .def("set_band",
+           [](SphereCollisionCst& c, double band) { check(pmp_set_band(c.device_context(nullptr), band)); },
+           "Width (m) of the band around zero clearance whose FP32 decisions are re-made in FP64 (default 1e-5).")
       .def("_dev_set_schedule", [](SphereCollisionCst& c, int block, int waves_per_cu, int lanes) {
         pmp_schedule s{block, waves_per_cu, lanes};
         check(pmp_set_schedule(c.device_context(nullptr), &s));
@@ -451,14 +454,6 @@ PYBIND11_MODULE(_native, m) {
   };
   py::class_<Graph, std::shared_ptr<Graph>>(mo, "EdgeGraph")
       .def("launch", [](Graph& g) { check(pmp_graph_launch(g.g)); })
-      .def("kernel_ms", [](Graph& g) {
-        float ms = 0.f;
-        {
-          py::gil_scoped_release rel;
-          check(pmp_graph_kernel_ms(g.g, &ms));
-        }
-        return ms;
-      })
       .def("destroy", [](Graph& g) {
         if (g.g) check(pmp_graph_destroy(g.g));
         g.g = nullptr;
@@ -497,6 +492,20 @@ PYBIND11_MODULE(_native, m) {
              pmp_validator_desc d = v.desc();
              check(pmp_check_edges_batch(ctx, (const double*)s1, (const double*)s2, lds, E, &d, (uint8_t*)valid,
                                          (int32_t*)nchk));
+           })
+      .def("last_recheck_count", [](MotionValidator& v) {
+        py::gil_scoped_release rel;
+        return v.last_recheck_count();
+      })
+      .def("_dev_edge_kernel_ms",
+           [](MotionValidator& v, uintptr_t stream) {
+             pmp_context* ctx = v.constraint()->device_context((void*)stream);
+             float ms = 0.f;
+             {
+               py::gil_scoped_release rel;
+               check(pmp_edge_kernel_ms(ctx, &ms));
+             }
+             return ms;
            })
       .def("_dev_capture_graph",
            [](MotionValidator& v, uintptr_t s1, uintptr_t s2, int64_t lds, int64_t E, uintptr_t valid, uintptr_t nchk,

@@ -488,6 +488,12 @@ class MotionValidator {
     return v;
   }
   std::shared_ptr<SphereCollisionCst> constraint() const { return cst_; }
+  // edges of the last batch re-decided in FP64
+  int32_t last_recheck_count() {
+    int32_t n = 0;
+    check(pmp_edge_recheck_count(cst_->device_context(nullptr), &n));
+    return n;
+  }
 
  private:
   std::shared_ptr<SphereCollisionCst> cst_;

@@ -256,52 +256,112 @@ struct ValidVis32 {
 // pre-test and is_outside (check_ext_collision), per self pair the strict
 // squared-distance test (check_self_collision).  The group-sphere broad
 // phase of the reference only culls, so it is not needed for the result.
+// Spheres are visited by block with the FP32 pass's cull table: its grown
+// boxes carry a 1e-4 m slack, far above the FP32/FP64 centre difference, so a
+// candidate it drops is outside the reference's AABB pre-test in FP64 too.
+// A wave may take a share of the work: candidates i with i % n_split ==
+// split (self pairs with split 0).
 struct ValidVis64 {
   static constexpr bool kSelf = true;
+  static constexpr bool kBlocks = true;
   const KinArgs<double>& ka;
   const SceneArgs<double>& sc;
+  int split = 0, n_split = 1;
   bool collided = false;
   __device__ bool alive() const { return !collided; }
   template <class QS>
   __device__ void begin(const Pose<double>&, const QS&) {}
   __device__ void node(int, const DevNode<double>&, const Pose<double>&) {}
-  __device__ void sphere(int s, V3<double> c, double r) {
-    if (collided || cload(ka.meta, s).ext < 0) return;
-    for (int p = 0; p < sc.n_prims; ++p) {
-      const DevPrim<double>& P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) {
-        // CloudSDF::is_outside: sqrt(min |p - c|^2) - rc > r (primitive_sdf.hpp:597-603),
-        // the minimum taken over the tiles that can reach within r + rc.
-        if (outside_aabb(P, c, r)) continue;
-        const double R = r + P.p0, lim = R * R * (1.0 + 1e-12) + 1e-300;
-        double best = DBL_MAX;
-        for (int t = 0; t < P.cloud_count; ++t) {
-          const CloudTile64 T = cload(sc.tiles64, P.cloud_begin + t);
-          const double dx = fmax(fmax(T.lb[0] - c.x, c.x - T.ub[0]), 0.0);
-          const double dy = fmax(fmax(T.lb[1] - c.y, c.y - T.ub[1]), 0.0);
-          const double dz = fmax(fmax(T.lb[2] - c.z, c.z - T.ub[2]), 0.0);
-          if (dx * dx + dy * dy + dz * dz > lim) continue;
-          const double* pt = sc.pts64 + (int64_t)(P.cloud_begin + t) * kCloudTile * 4;
-          for (int j = 0; j < kCloudTile; ++j) {
-            const double ex = pt[4 * j] - c.x, ey = pt[4 * j + 1] - c.y, ez = pt[4 * j + 2] - c.z;
-            const double d2 = ex * ex + ey * ey + ez * ez;
-            if (d2 < best) best = d2;
-          }
-        }
-        if (!(sqrt(best) - P.p0 > r)) {
-          collided = true;
-          return;
-        }
-        continue;
+  __device__ void sphere(int, V3<double>, double) {}
+
+  template <int K>
+  __device__ __forceinline__ void kind_loop(const V3<double> (&c)[kBlockSpheres], const double (&r)[kBlockSpheres],
+                                            int n, V3<double> g, const CullRange& cr, bool nocull) {
+    const DevPrim<double>* vp = reinterpret_cast<const DevPrim<double>*>(sc.vprims);
+    const int lo = nocull ? sc.vrange[K] : cr.at[K], hi = nocull ? sc.vrange[K + 1] : cr.at[K + 1];
+    for (int i = lo + split; i < hi; i += n_split) {
+      int p = i;
+      if (!nocull) {
+        const CullEntry ce = cload(sc.cull, i);
+        const bool out = g.x < ce.lb[0] || g.x > ce.ub[0] || g.y < ce.lb[1] || g.y > ce.ub[1] ||
+                         g.z < ce.lb[2] || g.z > ce.ub[2];
+        if (!wave_any(alive() && !out)) continue;
+        p = ce.prim;
       }
-      if (!outside_aabb(P, c, r) && !is_outside_ref(P, c, r)) {
-        collided = true;
-        return;
+      const DevPrim<double> P = cload(vp, p);
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k)
+        if (k < n && !outside_aabb(P, c[k], r[k]) && !is_outside_ref<double, K>(P, c[k], r[k])) collided = true;
+    }
+  }
+
+  // CloudSDF::is_outside: sqrt(min |p - c|^2) - rc > r (primitive_sdf.hpp:597-603),
+  // the minimum taken over the tiles that can reach within r + rc.
+  __device__ void cloud(const DevPrim<double>& P, V3<double> c, double r) {
+    if (outside_aabb(P, c, r)) return;
+    const double R = r + P.p0, lim = R * R * (1.0 + 1e-12) + 1e-300;
+    double best = DBL_MAX;
+    for (int t = 0; t < P.cloud_count; ++t) {
+      const CloudTile64 T = cload(sc.tiles64, P.cloud_begin + t);
+      const double dx = fmax(fmax(T.lb[0] - c.x, c.x - T.ub[0]), 0.0);
+      const double dy = fmax(fmax(T.lb[1] - c.y, c.y - T.ub[1]), 0.0);
+      const double dz = fmax(fmax(T.lb[2] - c.z, c.z - T.ub[2]), 0.0);
+      if (dx * dx + dy * dy + dz * dz > lim) continue;
+      const double* pt = sc.pts64 + (int64_t)(P.cloud_begin + t) * kCloudTile * 4;
+      for (int j = 0; j < kCloudTile; ++j) {
+        const double ex = pt[4 * j] - c.x, ey = pt[4 * j + 1] - c.y, ez = pt[4 * j + 2] - c.z;
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        if (d2 < best) best = d2;
+      }
+    }
+    if (!(sqrt(best) - P.p0 > r)) collided = true;
+  }
+
+  __device__ void blocks(const DevNode<double>& nd, const Pose<double>& P) {
+    if (nd.block_begin >= nd.block_end) return;
+    const M3<double> R = qmat(P.q);
+    // outside the norm bounds the cull table assumed: test every primitive
+    const double n2 = P.q.x * P.q.x + P.q.y * P.q.y + P.q.z * P.q.z + P.q.w * P.q.w;
+    const bool nocull = wave_any(alive() && !(n2 >= nd.n2_lo && n2 <= nd.n2_hi));
+    const int n_solid = sc.vrange[kKindCloud];
+    for (int bi = nd.block_begin; bi < nd.block_end; ++bi) {
+      if (!wave_any(alive())) return;
+      const DevSphereBlock bk = cload(ka.blocks, bi);
+      const int n = bk.n;
+      V3<double> c[kBlockSpheres];
+      double r[kBlockSpheres];
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        c[k] = P.t;
+        r[k] = 0.0;
+        if (k < n) {
+          const DevSphere<double> sp = cload(ka.spheres, bk.sph[k]);
+          c[k] = add(mv(R, V3<double>{sp.x, sp.y, sp.z}), P.t);
+          r[k] = sp.r;
+          bad |= is_nan3(c[k]);
+        }
+      }
+      if (bad && n_solid > 0) collided = true;  // NaN centres collide with every primitive
+      const V3<double> g = add(mv(R, V3<double>{bk.g[0], bk.g[1], bk.g[2]}), P.t);
+      const CullRange cr = cload(sc.cull_range, bi);
+      kind_loop<kKindBoxAA>(c, r, n, g, cr, nocull);
+      kind_loop<kKindBoxRot>(c, r, n, g, cr, nocull);
+      kind_loop<kKindCylZ>(c, r, n, g, cr, nocull);
+      kind_loop<kKindCylRot>(c, r, n, g, cr, nocull);
+      kind_loop<kKindSphere>(c, r, n, g, cr, nocull);
+      kind_loop<kKindGround>(c, r, n, g, cr, nocull);
+      const DevPrim<double>* vp = reinterpret_cast<const DevPrim<double>*>(sc.vprims);
+      for (int p = sc.vrange[kKindCloud] + split; p < sc.vrange[kKindCloud + 1]; p += n_split) {
+        const DevPrim<double> Pc = cload(vp, p);
+#pragma unroll
+        for (int k = 0; k < kBlockSpheres; ++k)
+          if (k < n && !collided) cloud(Pc, c[k], r[k]);
       }
     }
   }
   __device__ void self(const DevSelfPair&, int, int, V3<double> ci, V3<double> cj, double rs) {
-    if (collided) return;
+    if (collided || split != 0) return;
     const V3<double> d = sub(ci, cj);
     if (dot(d, d) < rs * rs) collided = true;
   }
@@ -436,33 +496,51 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
   }
 }
 
-// FP64 walk of the edges left pending by edges_kernel, one wave per edge.
-__global__ void __launch_bounds__(64) edges_recheck_kernel(KinArgs<double> k64, SceneArgs<double> s64,
-                                                           EdgeArgs ea) {
+// FP64 walk of the edges left pending by edges_kernel: one workgroup of
+// kRecheckWaves waves per edge.  A round evaluates up to kRecheckWaves chunks
+// of 64 states at once, each chunk's primitive candidates shared among
+// kRecheckWaves / chunks waves; the first colliding state in visiting order is
+// the minimum colliding index over the round, and a round that finds one
+// ends the walk.
+constexpr int kRecheckWaves = 4;
+__global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinArgs<double> k64,
+                                                                           SceneArgs<double> s64, EdgeArgs ea) {
   const int64_t e = blockIdx.x;
-  if (e >= ea.E || ea.valid[e] != 2) return;
-  const int lane = threadIdx.x & 63;
+  if (e >= ea.E || ea.valid[e] != 2) return;  // uniform over the workgroup
+  __shared__ int first;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) first = INT_MAX;
+  __syncthreads();
   const double ratio = edge_step_ratio(ea, k64.n_cols, e);
   const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
   const int64_t n_wp = n_test - 1;
-  bool ok = true;
-  int64_t checked = n_wp;
-  for (int64_t base = 0; base < n_wp; base += 64) {
-    const int64_t k = base + lane;
-    const bool has = k < n_wp;
-    ValidVis64 v{k64, s64};
-    v.collided = !has;
-    run_fk(k64, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), k64.base_type == PMP_BASE_FIXED, v);
-    const bool hit = has && v.collided;
-    if (__any(hit)) {
-      ok = false;
-      checked = wave_min(hit ? (int)k : INT_MAX) + 1;
-      break;
+  const int64_t n_chunks = (n_wp + 63) / 64;
+  const int per_round = n_chunks < kRecheckWaves ? (int)n_chunks : kRecheckWaves;
+  const int n_split = kRecheckWaves / per_round;
+  for (int64_t c0 = 0; c0 < n_chunks; c0 += per_round) {
+    const int64_t chunk = c0 + w / n_split;
+    if (w / n_split < per_round && chunk < n_chunks) {
+      const int64_t k = chunk * 64 + lane;
+      const bool has = k < n_wp;
+      ValidVis64 v{k64, s64};
+      v.split = w % n_split;
+      v.n_split = n_split;
+      v.collided = !has;
+      run_fk(k64, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), k64.base_type == PMP_BASE_FIXED, v);
+      const bool hit = has && v.collided;
+      if (__any(hit)) {
+        const int kmin = wave_min(hit ? (int)k : INT_MAX);
+        if (lane == 0) atomicMin(&first, kmin);
+      }
     }
+    __syncthreads();
+    const int f = first;
+    __syncthreads();
+    if (f != INT_MAX) break;  // uniform: every wave read the same value between the barriers
   }
-  if (lane == 0) {
-    ea.valid[e] = ok ? 1 : 0;
-    if (ea.n_checked) ea.n_checked[e] = (int32_t)checked;
+  if (threadIdx.x == 0) {
+    ea.valid[e] = first == INT_MAX ? 1 : 0;
+    if (ea.n_checked) ea.n_checked[e] = (int32_t)(first == INT_MAX ? n_wp : (int64_t)first + 1);
     if (ea.n_recheck) atomicAdd(ea.n_recheck, 1);
   }
 }
@@ -845,7 +923,7 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
     hipError_t e = hipEventRecord(ev_after, st);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(edges_recheck_kernel, dim3((unsigned)ea.E), dim3(64), 0, st, k64, s64, ea);
+  hipLaunchKernelGGL(edges_recheck_kernel, dim3((unsigned)ea.E), dim3(64 * kRecheckWaves), 0, st, k64, s64, ea);
   return hipGetLastError();
 }
 

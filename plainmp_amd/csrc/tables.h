@@ -53,6 +53,7 @@ struct alignas(16) DevSphere {
 struct alignas(16) DevSphereBlock {
   int32_t n;
   int32_t ext[kBlockSpheres];  // index among the primitive-checked spheres
+  int32_t sph[kBlockSpheres];  // index in the node-sorted sphere tables
   int32_t pad[3];
   float g[3], G;
   float s[kBlockSpheres][4];   // link-frame centre and radius

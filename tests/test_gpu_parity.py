@@ -60,7 +60,8 @@ def test_edges_mixed16_subset():
     ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(wl.box_width), 0.0, 8)
     mv = BoxMotionValidator(pc, wl.box_width)
     got, n_got = mv.check_motions(wl.S1, wl.S2)
-    _report("edges", E=len(ref), valid_frac=ref.mean(), mean_checked_ref=n_ref.mean(), mean_checked_gpu=n_got.mean())
+    _report("edges", E=len(ref), valid_frac=ref.mean(), mean_checked_ref=n_ref.mean(), mean_checked_gpu=n_got.mean(),
+            rechecked=mv.last_recheck_count())
     assert np.array_equal(got, ref.astype(bool))
     # states evaluated in the reference's visiting order, up to the first invalid one
     assert np.array_equal(n_got, n_ref)
@@ -252,3 +253,45 @@ def test_valid_every_kind_with_ground(height):
     eref, _ = oc.check_motions(S1, S2, True, list(width), 0.0, 8)
     egot, _ = BoxMotionValidator(pc, width).check_motions(S1, S2)
     assert np.array_equal(egot, eref.astype(bool))
+
+
+@pytest.mark.parametrize("name", ["plain_two_bars", "every_kind", "dual_arm", "cloud", "edges"])
+def test_fp64_redecision_wide_band(name):
+    """A 5 cm band sends a large share of the work to the FP64 pass (block
+    visitor with the cull table, multi-wave edge walk with split candidate
+    lists); its decisions must still equal the oracle's."""
+    import dataclasses
+
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    if name == "every_kind":
+        base = synth.workload("grad_mixed32", scale=1 / 32)
+        wl = dataclasses.replace(base, prims=base.prims + [{"type": "ground", "height": 0.0}])
+        Q = wl.Q
+    elif name == "dual_arm":
+        wl = synth.workload("traj_dual_arm", scale=0.005)
+        Q = wl.traj_X[:256]
+    elif name == "cloud":
+        wl = synth.workload("cloud_shelves", scale=64 / 16384)
+        Q = wl.Q
+    elif name == "edges":
+        wl = synth.workload("edges_mixed16", scale=1 / 16)
+    else:
+        wl = synth.workload(name, scale=0.25)
+        Q = wl.Q
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    pc.set_band(0.05)
+    if name == "edges":
+        ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(wl.box_width), 0.0, 8)
+        mv = BoxMotionValidator(pc, wl.box_width)
+        got, n_got = mv.check_motions(wl.S1, wl.S2)
+        _report("wide_band_edges", E=len(ref), rechecked=mv.last_recheck_count())
+        assert mv.last_recheck_count() > 0
+        assert np.array_equal(got, ref.astype(bool))
+        assert np.array_equal(n_got, n_ref)
+        return
+    ref = oc.is_valid_batch(Q, 8).astype(bool)
+    got, n_re = pc.is_valid_batch_stats(Q)
+    _report("wide_band_" + name, B=len(ref), rechecked=n_re, valid_frac=ref.mean())
+    assert n_re > 0
+    assert np.array_equal(got, ref)
