@@ -243,6 +243,23 @@ def _extra(torch, stream, dev):
     ms = _time_stream(torch, stream,
                       lambda: cst._dev_pair_clearance_vjp(q.data_ptr(), B, B, gd.data_ptr(), gq.data_ptr(), sh))
     res["grad_mixed32_vjp"] = {"configs": B, "ms": ms, "vjps_per_s": B / (ms * 1e-3)}
+    # traj_dual_arm: 2,048 trajectories of 20-100 waypoints, one launch: evaluate rows
+    # of every waypoint + smoothness cost and gradient
+    wl = synth.workload("traj_dual_arm")
+    cst = product_cst(wl)
+    off, X = wl.traj_offsets, wl.traj_X
+    N, D, C, T = len(X), cst.q_dim(), cst.cst_dim(), len(off) - 1
+    offl = [int(v) for v in off]
+    with torch.cuda.stream(stream):
+        x = _soa(torch, X)
+        vals = torch.empty((C, N), dtype=torch.float32, device=dev)
+        jac = torch.empty((C, D, N), dtype=torch.float32, device=dev)
+        cost = torch.empty(T, dtype=torch.float32, device=dev)
+        grad = torch.empty((D, N), dtype=torch.float32, device=dev)
+    ms = _time_stream(torch, stream, lambda: cst._dev_traj_cost_grad(
+        x.data_ptr(), N, N, offl, [], vals.data_ptr(), jac.data_ptr(), cost.data_ptr(), grad.data_ptr(), sh))
+    res["traj_dual_arm_cost_grad"] = {"trajectories": T, "waypoints": N, "ms": ms,
+                                      "waypoints_per_s": N / (ms * 1e-3), "trajectories_per_s": T / (ms * 1e-3)}
     return res
 
 

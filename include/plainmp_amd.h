@@ -182,6 +182,24 @@ pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, in
 pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, float* vals,
                               float* jac);
 
+/* Trajectories: X holds the waypoints of n_traj trajectories back to back
+ * (columns as q, leading dimension ldx, N waypoints in all); trajectory i
+ * spans waypoints [offsets[i], offsets[i+1]) (HOST array of n_traj + 1,
+ * offsets[0] = 0, non-decreasing, offsets[n_traj] = N).  One launch writes
+ *   vals / jac  the evaluate rows of every waypoint, laid out as
+ *               pmp_evaluate_batch with B = N;
+ *   cost[i]     sum over interior t of sum_j w_j^2 (x_{t-1,j} - 2 x_{t,j} + x_{t+1,j})^2
+ *               (FP32, device, n_traj);
+ *   grad        grad[j * N + n] = d cost / d x_{n,j} (FP32, device).
+ * weights: HOST [q_dim], NULL for ones.  Replaces SequentialCst::evaluate for
+ * the collision constraint (constraints/sequential_constraint.cpp:63-91, without
+ * its 1e-12 sparsity marker) and the smoothness term x^T P x with
+ * P = smoothcost_fullmat (src/plainmp/nlp_solver/sqp_based_solver.py:74-96),
+ * gradient 2 P x. */
+pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
+                                    int32_t n_traj, const double* weights, float* vals, float* jac, float* cost,
+                                    float* grad);
+
 /* Signed distance of every (sphere, primitive) pair with exact FK and its
  * derivative with respect to q: dist[(s*P + p) * B + b],
  * grad[((s*P + p) * q_dim + j) * B + b].  grad may be NULL.  No reference

@@ -150,6 +150,9 @@ struct pmp_context {
   int h_vrange[kNumKinds + 1] = {0};
   DevBuf cull, cull_range;
   // misc
+  DevBuf traj_off, traj_w2;
+  std::vector<int64_t> h_traj_off;
+  std::vector<double> h_traj_w2;
   DevBuf seq, counter, width, inv_width, tmp_prims, tmp_cloud, ws_cen, ws_cloud, ws_flags;
 };
 
@@ -524,7 +527,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
                       &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
-                      &ctx->ws_cloud, &ctx->ws_flags})
+                      &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2})
       b->release();
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     if (ctx->edge_ev0) (void)hipEventDestroy(ctx->edge_ev0);
@@ -1001,6 +1004,37 @@ pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, in
     if (B == 0) return;
     set_device(ctx);
     PMP_HIP(launch_evaluate(ctx->k32, ctx->s32, q, ldq, B, vals, jac, st(ctx)));
+  });
+}
+
+pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
+                                    int32_t n_traj, const double* weights, float* vals, float* jac, float* cost,
+                                    float* grad) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(X, ldx, N);
+    PMP_REQUIRE(n_traj >= 0 && offsets != nullptr, "missing trajectory offsets");
+    PMP_REQUIRE(offsets[0] == 0 && offsets[n_traj] == N, "offsets must start at 0 and end at N");
+    for (int32_t i = 0; i < n_traj; ++i) PMP_REQUIRE(offsets[i + 1] >= offsets[i], "offsets must not decrease");
+    PMP_REQUIRE(N == 0 || (vals && jac && grad), "null output");
+    PMP_REQUIRE(n_traj == 0 || cost, "null output");
+    require_no_cloud(ctx, "pmp_traj_cost_grad_batch");
+    if (n_traj == 0) return;
+    set_device(ctx);
+    std::vector<int64_t> off(offsets, offsets + n_traj + 1);
+    std::vector<double> w2(std::max(ctx->n_cols, 1), 1.0);
+    for (int j = 0; j < ctx->n_cols; ++j) w2[j] = weights ? weights[j] * weights[j] : 1.0;
+    // the tables are re-sent only when they change (repeated solver iterations reuse them)
+    if (off != ctx->h_traj_off) {
+      ctx->traj_off.upload(off, st(ctx));
+      ctx->h_traj_off = off;
+    }
+    if (w2 != ctx->h_traj_w2) {
+      ctx->traj_w2.upload(w2, st(ctx));
+      ctx->h_traj_w2 = w2;
+    }
+    const int64_t* doff = (const int64_t*)ctx->traj_off.ptr;
+    const double* dw = (const double*)ctx->traj_w2.ptr;
+    PMP_HIP(launch_traj(ctx->k32, ctx->s32, X, ldx, N, doff, n_traj, dw, vals, jac, cost, grad, st(ctx)));
   });
 }
 

@@ -332,6 +332,36 @@ PYBIND11_MODULE(_native, m) {
              for (int64_t i = 0; i < B; ++i) out.mutable_data()[i] = v[i] != 0;
              return std::make_pair(out, n);
            })
+      .def(
+          "traj_cost_grad_batch",
+          [](SphereCollisionCst& c, const DArr& X, const std::vector<int64_t>& offsets,
+             const std::vector<double>& weights) {
+            int64_t N;
+            const double* x = rows(X, c.q_dim(), N);
+            std::vector<double> vals, jac, cost, grad;
+            {
+              py::gil_scoped_release rel;
+              c.traj_cost_grad_batch(x, N, offsets, weights, vals, jac, cost, grad);
+            }
+            const int64_t C = (int64_t)c.cst_dim(), D = (int64_t)c.q_dim(), T = (int64_t)cost.size();
+            DArr v({N, C}), j({N, C, D}), co(T), g({N, D});
+            std::copy(vals.begin(), vals.end(), v.mutable_data());
+            std::copy(jac.begin(), jac.end(), j.mutable_data());
+            std::copy(cost.begin(), cost.end(), co.mutable_data());
+            std::copy(grad.begin(), grad.end(), g.mutable_data());
+            return py::make_tuple(v, j, co, g);
+          },
+          py::arg("X"), py::arg("offsets"), py::arg("weights") = std::vector<double>{},
+          "Per-waypoint evaluate rows, smoothness cost per trajectory and its gradient, one launch.")
+      .def("_dev_traj_cost_grad",
+           [](SphereCollisionCst& c, uintptr_t X, int64_t ldx, int64_t N, const std::vector<int64_t>& offsets,
+              const std::vector<double>& weights, uintptr_t vals, uintptr_t jac, uintptr_t cost, uintptr_t grad,
+              uintptr_t stream) {
+             pmp_context* ctx = c.device_context((void*)stream);
+             check(pmp_traj_cost_grad_batch(ctx, (const double*)X, ldx, N, offsets.data(), (int32_t)offsets.size() - 1,
+                                            weights.empty() ? nullptr : weights.data(), (float*)vals, (float*)jac,
+                                            (float*)cost, (float*)grad));
+           })
       .def("evaluate_batch",
            [](SphereCollisionCst& c, const DArr& Q) {
              int64_t B;

@@ -212,6 +212,44 @@ class SphereCollisionCst {
         for (int j = 0; j < D; ++j) jac[((size_t)b * C + k) * D + j] = hj[((size_t)k * D + j) * B + b];
       }
   }
+  // Trajectories back to back in X (N, q_dim) row-major, trajectory i =
+  // rows [offsets[i], offsets[i+1]): per-waypoint evaluate rows, smoothness
+  // cost per trajectory and its gradient (N, q_dim).
+  void traj_cost_grad_batch(const double* X, int64_t N, const std::vector<int64_t>& offsets,
+                            const std::vector<double>& weights, std::vector<double>& vals, std::vector<double>& jac,
+                            std::vector<double>& cost, std::vector<double>& grad) {
+    sync_batch_tables();
+    const int D = (int)q_dim(), C = (int)cst_dim();
+    if (offsets.empty()) throw std::invalid_argument("offsets must hold n_traj + 1 entries");
+    if (!weights.empty() && (int)weights.size() != D) throw std::invalid_argument("weights must have q_dim entries");
+    const int32_t n_traj = (int32_t)offsets.size() - 1;
+    vals.assign((size_t)N * C, 0.0);
+    jac.assign((size_t)N * C * D, 0.0);
+    cost.assign(n_traj, 0.0);
+    grad.assign((size_t)N * D, 0.0);
+    double* dx = N ? batch_.upload_soa(0, X, N, D) : nullptr;
+    float* dv = (float*)batch_.buf(1, sizeof(float) * std::max<int64_t>(N * C, 1));
+    float* dj = (float*)batch_.buf(2, sizeof(float) * std::max<int64_t>(N * C * D, 1));
+    float* dc = (float*)batch_.buf(3, sizeof(float) * std::max<int32_t>(n_traj, 1));
+    float* dg = (float*)batch_.buf(4, sizeof(float) * std::max<int64_t>(N * D, 1));
+    check(pmp_traj_cost_grad_batch(batch_.ctx(), dx, N, N, offsets.data(), n_traj,
+                                   weights.empty() ? nullptr : weights.data(), dv, dj, dc, dg));
+    if (n_traj == 0) return;
+    std::vector<float> hv((size_t)N * C), hj((size_t)N * C * D), hc(n_traj), hg((size_t)N * D);
+    batch_.d2h(hc.data(), dc, hc.size() * sizeof(float));
+    for (int32_t i = 0; i < n_traj; ++i) cost[i] = hc[i];
+    if (N == 0) return;
+    batch_.d2h(hv.data(), dv, hv.size() * sizeof(float));
+    batch_.d2h(hj.data(), dj, hj.size() * sizeof(float));
+    batch_.d2h(hg.data(), dg, hg.size() * sizeof(float));
+    for (int64_t n = 0; n < N; ++n) {
+      for (int k = 0; k < C; ++k) {
+        vals[n * C + k] = hv[(size_t)k * N + n];
+        for (int j = 0; j < D; ++j) jac[((size_t)n * C + k) * D + j] = hj[((size_t)k * D + j) * N + n];
+      }
+      for (int j = 0; j < D; ++j) grad[(size_t)n * D + j] = hg[(size_t)j * N + n];
+    }
+  }
   // (B, S) per-sphere minimum clearance
   std::vector<float> sphere_clearance_batch(const double* Q, int64_t B, bool accurate) {
     sync_batch_tables();

@@ -657,12 +657,11 @@ struct EvalVis {
   }
 };
 
-template <int MAXC>
-__global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
-                                                       int64_t ldq, int64_t B, float* vals, float* jac) {
-  const int64_t b = gid();
-  const bool active = b < B;
-  BatchQ qs{q, ldq, active ? b : B - 1};
+// Reference-semantics value and Jacobian rows of one configuration (exact
+// FK), written at column b of outputs with leading dimension ld.
+template <int MAXC, class QS>
+__device__ __forceinline__ void evaluate_rows(const KinArgs<float>& ka, const SceneArgs<float>& sc, const QS& qs,
+                                              bool active, int64_t b, int64_t ld, float* vals, float* jac) {
   EvalVis<MAXC> v{ka, sc};
   run_fk(ka, qs, false, v);
   if (!active) return;
@@ -678,11 +677,11 @@ __global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneA
     }
 #pragma unroll
     for (int j = 0; j < MAXC; ++j)
-      if (j < D) jac[(int64_t)j * B + b] = dot(g, v.fr.col(j, mask, ka.rev_mask, v.bc));
+      if (j < D) jac[(int64_t)j * ld + b] = dot(g, v.fr.col(j, mask, ka.rev_mask, v.bc));
     row = 1;
   }
   if (ka.n_pairs > 0) {
-    vals[(int64_t)row * B + b] = v.sbest;
+    vals[(int64_t)row * ld + b] = v.sbest;
     V3<float> n{0.f, 0.f, 0.f};
     uint32_t m1 = 0, m2 = 0;
     if (v.si >= 0) {
@@ -694,8 +693,69 @@ __global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneA
 #pragma unroll
     for (int j = 0; j < MAXC; ++j)
       if (j < D)
-        jac[((int64_t)row * D + j) * B + b] =
+        jac[((int64_t)row * D + j) * ld + b] =
             dot(n, sub(v.fr.col(j, m1, ka.rev_mask, v.sci), v.fr.col(j, m2, ka.rev_mask, v.scj)));
+  }
+}
+
+template <int MAXC>
+__global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
+                                                       int64_t ldq, int64_t B, float* vals, float* jac) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  evaluate_rows<MAXC>(ka, sc, qs, active, b, B, vals, jac);
+}
+
+// ------------------------------------------------------------ trajectories
+// One launch over every waypoint of a set of trajectories of uneven length
+// (lane per waypoint, trajectories back to back): the evaluate rows of each
+// waypoint (SequentialCst::evaluate, constraints/sequential_constraint.cpp:63-91)
+// and the smoothness term of smoothcost_fullmat
+// (nlp_solver/sqp_based_solver.py:74-96),
+//   cost_i = sum_{interior t} sum_j w_j^2 a_{t,j}^2,  a_t = x_{t-1} - 2 x_t + x_{t+1},
+//   d cost / d x_t = 2 w^2 (a_{t-1} - 2 a_t + a_{t+1})   (terms of interior indices only).
+// Second differences are formed in FP64 (they cancel), terms accumulate in
+// FP32; the first waypoint's lane sums its trajectory's cost in order, so the
+// result does not depend on scheduling.
+template <int MAXC>
+__global__ void __launch_bounds__(256) traj_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* X,
+                                                   int64_t ldx, int64_t N, const int64_t* offsets, int n_traj,
+                                                   const double* w2, float* vals, float* jac, float* cost,
+                                                   float* grad) {
+  const int64_t n = gid();
+  const bool active = n < N;
+  BatchQ qs{X, ldx, active ? n : N - 1};
+  evaluate_rows<MAXC>(ka, sc, qs, active, n, N, vals, jac);
+  if (!active) return;
+  int lo = 0, hi = n_traj;  // offsets[lo] <= n < offsets[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offsets[mid] <= n) lo = mid;
+    else hi = mid;
+  }
+  const int64_t o = offsets[lo], T = offsets[lo + 1] - o, t = n - o;
+  const int D = ka.n_cols;
+  auto x = [&](int64_t s, int j) { return X[(int64_t)j * ldx + o + s]; };
+  auto acc = [&](int64_t s, int j) { return x(s - 1, j) - 2.0 * x(s, j) + x(s + 1, j); };
+#pragma unroll
+  for (int j = 0; j < MAXC; ++j) {
+    if (j < D) {
+      double g = 0.0;
+      if (t - 1 >= 1 && t - 1 <= T - 2) g += acc(t - 1, j);
+      if (t >= 1 && t <= T - 2) g -= 2.0 * acc(t, j);
+      if (t + 1 >= 1 && t + 1 <= T - 2) g += acc(t + 1, j);
+      grad[(int64_t)j * N + n] = (float)(2.0 * w2[j] * g);
+    }
+  }
+  if (t == 0) {
+    float c = 0.f;
+    for (int64_t s = 1; s + 1 < T; ++s)
+      for (int j = 0; j < D; ++j) {
+        const double a = acc(s, j);
+        c += (float)(w2[j] * a * a);
+      }
+    cost[lo] = c;
   }
 }
 
@@ -941,6 +1001,20 @@ hipError_t launch_evaluate(const KinArgs<float>& ka, const SceneArgs<float>& sc,
                            int64_t B, float* vals, float* jac, hipStream_t st) {
   PMP_DISPATCH_MAXC(ka.n_cols, evaluate_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, vals,
                     jac);
+  return hipGetLastError();
+}
+__global__ void fill_f32_kernel(float* p, float v, int64_t n) {
+  const int64_t i = gid();
+  if (i < n) p[i] = v;
+}
+hipError_t launch_traj(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* X, int64_t ldx, int64_t N,
+                       const int64_t* offsets, int n_traj, const double* w2, float* vals, float* jac, float* cost,
+                       float* grad, hipStream_t st) {
+  // empty trajectories have no lane of their own
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks_for(n_traj, 256)), dim3(256), 0, st, cost, 0.0f, (int64_t)n_traj);
+  if (N == 0) return hipGetLastError();
+  PMP_DISPATCH_MAXC(ka.n_cols, traj_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, ka, sc, X, ldx, N, offsets,
+                    n_traj, w2, vals, jac, cost, grad);
   return hipGetLastError();
 }
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,

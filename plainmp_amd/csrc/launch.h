@@ -53,6 +53,9 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                         hipStream_t st, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr);
 hipError_t launch_evaluate(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                            int64_t B, float* vals, float* jac, hipStream_t st);
+hipError_t launch_traj(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* X, int64_t ldx, int64_t N,
+                       const int64_t* offsets, int n_traj, const double* w2, float* vals, float* jac, float* cost,
+                       float* grad, hipStream_t st);
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                         int64_t B, float* dist, float* grad, hipStream_t st);
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,

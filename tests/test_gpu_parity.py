@@ -295,3 +295,47 @@ def test_fp64_redecision_wide_band(name):
     _report("wide_band_" + name, B=len(ref), rechecked=n_re, valid_frac=ref.mean())
     assert n_re > 0
     assert np.array_equal(got, ref)
+
+
+def test_trajectory_rows_and_smoothness_dual_arm():
+    """One launch over uneven trajectories: per-waypoint evaluate rows against
+    the oracle's evaluate, smoothness cost and gradient against the oracle's
+    smooth_cost (x^T P x and 2 P x of smoothcost_fullmat)."""
+    wl = synth.workload("traj_dual_arm", scale=0.01)  # ~20 trajectories, 20-100 waypoints each
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    off, X = wl.traj_offsets, wl.traj_X
+    D = X.shape[1]
+    w = np.linspace(0.5, 2.0, D)
+    vals, jac, cost, grad = pc.traj_cost_grad_batch(X, list(off), list(w))
+    vref, jref = oc.evaluate_batch(X, 8)
+    assert vals.shape == vref.shape and jac.shape == jref.shape
+    assert np.abs(vals - vref).max() <= 5e-6
+    pd = oc.pair_clearance_batch(X, True, 8)
+    clear = (_second_gap(pd) > 1e-5) & (vref[:, 0] < 0.1)  # unambiguous ext argmin below the cutoff
+    assert clear.sum() > 0
+    scale = np.maximum(np.abs(jref[clear, 0]), 1.0)
+    assert (np.abs(jac[clear, 0] - jref[clear, 0]) <= 1e-4 * scale).all()
+    for i in range(len(off) - 1):
+        a, b = off[i], off[i + 1]
+        cref, gref = _oracle.smooth_cost(X[a:b].ravel(), b - a, D, list(w))
+        assert abs(cost[i] - cref) <= 1e-5 * max(1.0, abs(cref))
+        assert np.allclose(grad[a:b].ravel(), gref, rtol=1e-5, atol=1e-6)
+    _report("traj", N=len(X), T=len(off) - 1, val_err=np.abs(vals - vref).max())
+
+
+def test_trajectory_edge_cases():
+    """Empty, one- and two-waypoint trajectories have zero smoothness cost and
+    gradient; bad offsets are rejected as ValueError."""
+    wl = synth.workload("grad_mixed32", scale=1 / 256)
+    pc = product_cst(wl)
+    X = wl.Q[:7]
+    off = [0, 0, 1, 3, 3, 7]
+    vals, jac, cost, grad = pc.traj_cost_grad_batch(X, off)
+    assert cost[0] == 0 and cost[1] == 0 and cost[2] == 0 and cost[3] == 0
+    assert np.all(grad[:3] == 0)
+    cref, gref = _oracle.smooth_cost(X[3:7].ravel(), 4, X.shape[1], [1.0] * X.shape[1])
+    assert abs(cost[4] - cref) <= 1e-5 * max(1.0, cref)
+    with pytest.raises(ValueError):
+        pc.traj_cost_grad_batch(X, [0, 3, 2, 7])
+    with pytest.raises(ValueError):
+        pc.traj_cost_grad_batch(X, [0, 3, 6])
