@@ -131,7 +131,7 @@ struct pmp_context {
   int n_cu = 256;
   // robot
   bool has_robot = false;
-  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks;
+  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, ws_vjp;
   KinArgs<float> k32{};
   KinArgs<double> k64{};
   int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
@@ -523,7 +523,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
-    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks,
+    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->ws_vjp,
                       &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
@@ -813,6 +813,12 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     a.meta = d.meta = ctx->meta.upload(meta, s0);
     a.pairs = d.pairs = ctx->pairs.upload(pairs, s0);
     a.blocks = d.blocks = ctx->blocks.upload(blocks, s0);
+    std::vector<DevSphereItem> items;
+    for (int i = 0; i < N; ++i)
+      for (int b0 = n32[i].sphere_begin; b0 < n32[i].sphere_end; b0 += kItemSpheres)
+        items.push_back({i, b0, std::min(b0 + kItemSpheres, n32[i].sphere_end), 0});
+    a.items = d.items = ctx->items.upload(items, s0);
+    a.n_items = d.n_items = (int32_t)items.size();
     std::vector<float> rad(std::max(S, 1), 0.f);
     for (int s2 = 0; s2 < S; ++s2) rad[s2] = (float)r->sphere_radius[s2];
     ctx->rad_out.upload(rad, s0);
@@ -1058,7 +1064,8 @@ pmp_status pmp_pair_clearance_vjp(pmp_context* ctx, const double* q, int64_t ldq
     require_no_cloud(ctx, "pmp_pair_clearance_vjp");
     if (B == 0) return;
     set_device(ctx);
-    PMP_HIP(launch_vjp(ctx->k32, ctx->s32, q, ldq, B, grad_dist, grad_q, st(ctx)));
+    float* part = (float*)ctx->ws_vjp.get(sizeof(float) * std::max<int64_t>(1, (int64_t)ctx->k32.n_items * ctx->n_cols * B));
+    PMP_HIP(launch_vjp(ctx->k32, ctx->s32, q, ldq, B, grad_dist, part, grad_q, st(ctx)));
   });
 }
 

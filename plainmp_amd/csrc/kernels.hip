@@ -760,6 +760,10 @@ __global__ void __launch_bounds__(256) traj_kernel(KinArgs<float> ka, SceneArgs<
 }
 
 // ------------------------------------------------ per-pair values and adjoint
+// The pair and adjoint kernels give each (configuration, sphere item) its
+// own lane: the wave's item is uniform (grid y), FK runs only down to the
+// item's node, and the item's spheres meet every primitive.  Lanes b of a
+// wave are consecutive, so every store and gradient load is coalesced.
 template <int MAXC>
 struct PairVis {
   static constexpr bool kSelf = false;
@@ -770,14 +774,18 @@ struct PairVis {
   float* grad;
   int64_t B, b;
   bool active;
-  __device__ bool alive() const { return true; }
+  DevSphereItem it;
+  bool done = false;
+  __device__ bool alive() const { return !done; }
   template <class QS>
   __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
-  __device__ void node(int, const DevNode<float>& nd, const Pose<float>& P) {
+  __device__ void node(int n, const DevNode<float>& nd, const Pose<float>& P) {
+    if (n == it.node) done = true;  // its spheres follow in this same step
     if (nd.q_col < 0) return;
     fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
   }
   __device__ void sphere(int s, V3<float> c, float r) {
+    if (s < it.begin || s >= it.end) return;
     const DevSphereMeta m = cload(ka.meta, s);
     const uint32_t mask = cload(ka.nodes, m.node).col_mask;
     const int D = ka.n_cols, NP = sc.n_prims;
@@ -785,7 +793,7 @@ struct PairVis {
 #pragma unroll
     for (int j = 0; j < MAXC; ++j) J[j] = fr.col(j, mask, ka.rev_mask, c);
     for (int p = 0; p < NP; ++p) {
-      const DevPrim<float>& P = prim_at(sc, p);
+      const DevPrim<float> P = prim_at(sc, p);
       if (P.type == PMP_PRIM_CLOUD) continue;
       const float d = sdf_eval(P, c) - r;
       const int64_t row = (int64_t)m.out * NP + p;
@@ -807,13 +815,15 @@ __global__ void __launch_bounds__(256) pair_kernel(KinArgs<float> ka, SceneArgs<
   const int64_t b = gid();
   const bool active = b < B;
   BatchQ qs{q, ldq, active ? b : B - 1};
-  PairVis<MAXC> v{ka, sc, {}, dist, grad, B, b, active};
+  PairVis<MAXC> v{ka, sc, {}, dist, grad, B, b, active, cload(ka.items, blockIdx.y)};
   run_fk(ka, qs, false, v);
 }
 
 // Reverse mode: per sphere the upstream weights of its primitive distances
 // are folded into one world-space force g = sum_p w_p grad sdf_p(c); each
-// q column then receives g . (a_j x (c - o_j)) (or g . a_j).
+// q column then receives g . (a_j x (c - o_j)) (or g . a_j).  Each
+// (configuration, item) lane writes its partial column sums; a second pass
+// adds the items in a fixed order.
 template <int MAXC>
 struct VjpVis {
   static constexpr bool kSelf = false;
@@ -822,21 +832,25 @@ struct VjpVis {
   Frames<MAXC> fr;
   const float* gd;
   int64_t B, b;
+  DevSphereItem it;
   float acc[MAXC];
-  __device__ bool alive() const { return true; }
+  bool done = false;
+  __device__ bool alive() const { return !done; }
   template <class QS>
   __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
-  __device__ void node(int, const DevNode<float>& nd, const Pose<float>& P) {
+  __device__ void node(int n, const DevNode<float>& nd, const Pose<float>& P) {
+    if (n == it.node) done = true;
     if (nd.q_col < 0) return;
     fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
   }
   __device__ void sphere(int s, V3<float> c, float) {
+    if (s < it.begin || s >= it.end) return;
     const DevSphereMeta m = cload(ka.meta, s);
     const uint32_t mask = cload(ka.nodes, m.node).col_mask;
     const int NP = sc.n_prims;
     V3<float> g{0.f, 0.f, 0.f};
     for (int p = 0; p < NP; ++p) {
-      const DevPrim<float>& P = prim_at(sc, p);
+      const DevPrim<float> P = prim_at(sc, p);
       if (P.type == PMP_PRIM_CLOUD) continue;
       const float w = gd[((int64_t)m.out * NP + p) * B + b];
       const V3<float> gp = sdf_grad(P, c);
@@ -850,19 +864,29 @@ struct VjpVis {
 
 template <int MAXC>
 __global__ void __launch_bounds__(256) vjp_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
-                                                  int64_t ldq, int64_t B, const float* gd, float* grad_q) {
+                                                  int64_t ldq, int64_t B, const float* gd, float* part) {
   const int64_t b = gid();
   const bool active = b < B;
   const int64_t bb = active ? b : B - 1;
   BatchQ qs{q, ldq, bb};
-  VjpVis<MAXC> v{ka, sc, {}, gd, B, bb};
+  VjpVis<MAXC> v{ka, sc, {}, gd, B, bb, cload(ka.items, blockIdx.y)};
 #pragma unroll
   for (int j = 0; j < MAXC; ++j) v.acc[j] = 0.f;
   run_fk(ka, qs, false, v);
   if (!active) return;
+  const int D = ka.n_cols;
 #pragma unroll
   for (int j = 0; j < MAXC; ++j)
-    if (j < ka.n_cols) grad_q[(int64_t)j * B + b] = v.acc[j];
+    if (j < D) part[((int64_t)blockIdx.y * D + j) * B + b] = v.acc[j];
+}
+
+__global__ void __launch_bounds__(256) vjp_reduce_kernel(const float* part, int n_items, int D, int64_t B,
+                                                         float* grad_q) {
+  const int64_t i = gid();  // j * B + b
+  if (i >= (int64_t)D * B) return;
+  float s = 0.f;
+  for (int it = 0; it < n_items; ++it) s += part[(int64_t)it * D * B + i];
+  grad_q[i] = s;
 }
 
 // ---------------------------------------------------- point queries (FP64)
@@ -1019,14 +1043,18 @@ hipError_t launch_traj(const KinArgs<float>& ka, const SceneArgs<float>& sc, con
 }
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                         int64_t B, float* dist, float* grad, hipStream_t st) {
-  PMP_DISPATCH_MAXC(ka.n_cols, pair_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, dist,
-                    grad);
+  if (ka.n_items == 0) return hipSuccess;
+  PMP_DISPATCH_MAXC(ka.n_cols, pair_kernel, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256), 0, st, ka, sc,
+                    q, ldq, B, dist, grad);
   return hipGetLastError();
 }
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
-                      const float* gd, float* grad_q, hipStream_t st) {
-  PMP_DISPATCH_MAXC(ka.n_cols, vjp_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, gd,
-                    grad_q);
+                      const float* gd, float* part, float* grad_q, hipStream_t st) {
+  if (ka.n_items > 0)
+    PMP_DISPATCH_MAXC(ka.n_cols, vjp_kernel, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256), 0, st, ka,
+                      sc, q, ldq, B, gd, part);
+  hipLaunchKernelGGL(vjp_reduce_kernel, dim3(blocks_for((int64_t)ka.n_cols * B, 256)), dim3(256), 0, st, part,
+                     ka.n_items, ka.n_cols, B, grad_q);
   return hipGetLastError();
 }
 hipError_t launch_prims_eval(const DevPrim<double>* prims, int n, const double* cloud_xyz, const double* pts,

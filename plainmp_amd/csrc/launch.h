@@ -58,8 +58,9 @@ hipError_t launch_traj(const KinArgs<float>& ka, const SceneArgs<float>& sc, con
                        float* grad, hipStream_t st);
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                         int64_t B, float* dist, float* grad, hipStream_t st);
+// part: workspace of n_items * n_cols * B floats
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
-                      const float* gd, float* grad_q, hipStream_t st);
+                      const float* gd, float* part, float* grad_q, hipStream_t st);
 hipError_t launch_prims_eval(const DevPrim<double>* prims, int n, const double* cloud_xyz, const double* pts,
                              int64_t N, int mode, double radius, double* out, hipStream_t st);
 hipError_t launch_nearest(const double* pts, int64_t N, const double* qry, int64_t M, double* sqd, int32_t* idx,

@@ -74,6 +74,14 @@ struct alignas(16) CullRange {
 };
 static_assert(kNumKinds + 1 == 8, "CullRange holds one offset per kind plus the end");
 
+// A run of at most kItemSpheres consecutive node-sorted spheres of one node
+// (every sphere, primitive-checked or not): the unit of work of the pair and
+// adjoint kernels, which give each (configuration, item) its own lane.
+constexpr int kItemSpheres = 8;
+struct alignas(16) DevSphereItem {
+  int32_t node, begin, end, pad;
+};
+
 struct alignas(16) DevSphereMeta {
   int32_t out;    // index in SphereCollisionCst group order
   int32_t ext;    // index among the spheres checked against primitives, -1 for only-self groups
@@ -112,6 +120,8 @@ struct KinArgs {
   const DevSphereMeta* meta;
   const DevSelfPair* pairs;
   const DevSphereBlock* blocks;
+  const DevSphereItem* items;
+  int32_t n_items;
   int32_t n_nodes, n_spheres, n_pairs, n_cols;
   int32_t n_q_cols, base_type, n_ext_spheres;
   uint32_t rev_mask;  // q columns that rotate (revolute joints, base yaw/roll/pitch)
