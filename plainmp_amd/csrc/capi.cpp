@@ -131,7 +131,7 @@ struct pmp_context {
   int n_cu = 256;
   // robot
   bool has_robot = false;
-  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, ws_vjp;
+  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, ws_vjp, ws_eval;
   KinArgs<float> k32{};
   KinArgs<double> k64{};
   int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
@@ -480,6 +480,18 @@ void require_robot(pmp_context* ctx) {
   PMP_REQUIRE(ctx != nullptr, "null context");
   PMP_REQUIRE(ctx->has_robot, "no robot tables uploaded (call pmp_upload_robot first)");
 }
+// Unit values / winners ([units][B]) and per-chunk counters of the evaluate kernels.
+EvalArgs eval_workspace(pmp_context* ctx, int64_t B) {
+  const int64_t units = std::max<int64_t>(1, (int64_t)ctx->k32.n_items + ctx->k32.n_pairs);
+  const int64_t chunks = std::max<int64_t>(1, (B + 63) / 64);
+  char* w = (char*)ctx->ws_eval.get(units * B * 8 + chunks * 4 + 256);
+  EvalArgs e{};
+  e.pval = (float*)w;
+  e.pidx = (int32_t*)(w + units * B * 4);
+  e.count = (int32_t*)(w + units * B * 8);
+  return e;
+}
+
 void require_no_cloud(pmp_context* ctx, const char* what) {
   if (ctx->n_clouds > 0)
     throw Error(PMP_ERR_INVALID, std::string(what) + ": point-cloud primitives are supported by is_valid and "
@@ -523,7 +535,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
-    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->ws_vjp,
+    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->ws_vjp, &ctx->ws_eval,
                       &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
@@ -1009,7 +1021,10 @@ pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, in
     require_no_cloud(ctx, "pmp_evaluate_batch");
     if (B == 0) return;
     set_device(ctx);
-    PMP_HIP(launch_evaluate(ctx->k32, ctx->s32, q, ldq, B, vals, jac, st(ctx)));
+    EvalArgs e = eval_workspace(ctx, B);
+    e.vals = vals;
+    e.jac = jac;
+    PMP_HIP(launch_eval_units(ctx->k32, ctx->s32, q, ldq, B, e, st(ctx)));
   });
 }
 
@@ -1040,7 +1055,15 @@ pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t l
     }
     const int64_t* doff = (const int64_t*)ctx->traj_off.ptr;
     const double* dw = (const double*)ctx->traj_w2.ptr;
-    PMP_HIP(launch_traj(ctx->k32, ctx->s32, X, ldx, N, doff, n_traj, dw, vals, jac, cost, grad, st(ctx)));
+    EvalArgs e = eval_workspace(ctx, N);
+    e.vals = vals;
+    e.jac = jac;
+    e.offsets = doff;
+    e.n_traj = n_traj;
+    e.w2 = dw;
+    e.cost = cost;
+    e.grad = grad;
+    PMP_HIP(launch_eval_units(ctx->k32, ctx->s32, X, ldx, N, e, st(ctx)));
   });
 }
 

@@ -604,18 +604,86 @@ struct Frames {
   }
 };
 
+// ------------------------------------------------------------ evaluate rows
+// evaluate_ext_collision / evaluate_self_collision
+// (constraints/primitive_sphere_collision.cpp:256-383) split into units: a
+// unit is a sphere item (the external minimum over its spheres x every
+// primitive, visited in the reference's (group, primitive, sphere) key order)
+// or a self group pair (minimum over its sphere pairs in visiting order).  A
+// wave evaluates one unit for 64 consecutive configurations and leaves
+// (value, winner) per lane; the last wave to finish a chunk (device-scope
+// counter after a release fence) reduces the units in visiting order and
+// recomputes the winning Jacobian rows with one more FK walk.
 template <int MAXC>
-struct EvalVis {
-  static constexpr bool kSelf = true;
+struct EvalUnitVis {
+  static constexpr bool kSelf = false;
   const KinArgs<float>& ka;
   const SceneArgs<float>& sc;
-  Frames<MAXC> fr;
+  int mode = 0;  // 0: sphere item, 1: self group pair
+  DevSphereItem it{};
+  DevSelfPair pr{};
+  int stop = 0;
+  bool done = false;
+  Pose<float> P1{}, P2{};
   float best = kCutoff;
-  int bkey = 0x7fffffff, bs = -1, bp = -1;
-  V3<float> bc{0.f, 0.f, 0.f};
-  float sbest = kCutoff;
-  int si = -1, sj = -1;
-  V3<float> sci{0.f, 0.f, 0.f}, scj{0.f, 0.f, 0.f};
+  int bkey = 0x7fffffff, bidx = -1;
+  __device__ bool alive() const { return !done; }
+  template <class QS>
+  __device__ void begin(const Pose<float>&, const QS&) {}
+  __device__ void node(int n, const DevNode<float>&, const Pose<float>& P) {
+    if (mode == 1) {
+      if (n == pr.node1) P1 = P;
+      if (n == pr.node2) P2 = P;
+    }
+    if (n == stop) done = true;  // a sphere item's spheres follow in this same step
+  }
+  __device__ void sphere(int s, V3<float> c, float r) {
+    if (mode != 0 || s < it.begin || s >= it.end) return;
+    const DevSphereMeta m = cload(ka.meta, s);
+    if (m.ext < 0) return;
+    for (int p = 0; p < sc.n_prims; ++p) {
+      const DevPrim<float> P = prim_at(sc, p);
+      if (P.type == PMP_PRIM_CLOUD) continue;
+      if (outside_aabb(P, c, r + kCutoff)) continue;
+      const float v = sdf_eval(P, c) - r;
+      const int key = (m.group * sc.n_prims + p) * ka.n_spheres + m.out;
+      if (v < best || (bidx >= 0 && v == best && key < bkey)) {
+        best = v;
+        bkey = key;
+        bidx = s * kMaxPrims + p;
+      }
+    }
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
+  // the self group pair, after the walk has reached both of its nodes
+  __device__ void self_pair() {
+    const M3<float> R1 = qmat(P1.q), R2 = qmat(P2.q);
+    for (int i = pr.begin1; i < pr.end1; ++i) {
+      const DevSphere<float> a = cload(ka.spheres, i);
+      const V3<float> ci = add(mv(R1, V3<float>{a.x, a.y, a.z}), P1.t);
+      for (int j = pr.begin2; j < pr.end2; ++j) {
+        const DevSphere<float> bb = cload(ka.spheres, j);
+        const V3<float> cj = add(mv(R2, V3<float>{bb.x, bb.y, bb.z}), P2.t);
+        const V3<float> d = sub(ci, cj);
+        const float dist = sqrtf(dot(d, d)) - (a.r + bb.r);
+        if (dist < best) {
+          best = dist;
+          bidx = i * kMaxSpheres + j;
+        }
+      }
+    }
+  }
+};
+
+// Full FK walk that keeps the joint frames and the world centres of up to
+// three spheres (the winners whose Jacobian rows are written).
+template <int MAXC>
+struct EvalRowVis {
+  static constexpr bool kSelf = false;
+  const KinArgs<float>& ka;
+  Frames<MAXC> fr;
+  int s0 = -1, s1 = -1, s2 = -1;
+  V3<float> c0{0.f, 0.f, 0.f}, c1{0.f, 0.f, 0.f}, c2{0.f, 0.f, 0.f};
   __device__ bool alive() const { return true; }
   template <class QS>
   __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
@@ -623,120 +691,46 @@ struct EvalVis {
     if (nd.q_col < 0) return;
     fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
   }
-  // evaluate_ext_collision (constraints/primitive_sphere_collision.cpp:256-322): the minimum
-  // over (group, primitive, sphere) with the reference's visiting order as tie-break key.
-  __device__ void sphere(int s, V3<float> c, float r) {
-    const DevSphereMeta m = cload(ka.meta, s);
-    if (m.ext < 0) return;
-    for (int p = 0; p < sc.n_prims; ++p) {
-      const DevPrim<float>& P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) continue;
-      if (outside_aabb(P, c, r + kCutoff)) continue;
-      const float v = sdf_eval(P, c) - r;
-      const int key = (m.group * sc.n_prims + p) * ka.n_spheres + m.out;
-      if (v < best || (bs >= 0 && v == best && key < bkey)) {
-        best = v;
-        bkey = key;
-        bs = s;
-        bp = p;
-        bc = c;
-      }
-    }
+  __device__ void sphere(int s, V3<float> c, float) {
+    if (s == s0) c0 = c;
+    if (s == s1) c1 = c;
+    if (s == s2) c2 = c;
   }
-  // evaluate_self_collision (:324-383): pairs are visited in the reference order.
-  __device__ void self(const DevSelfPair&, int i, int j, V3<float> ci, V3<float> cj, float rs) {
-    const V3<float> d = sub(ci, cj);
-    const float dist = sqrtf(dot(d, d)) - rs;
-    if (dist < sbest) {
-      sbest = dist;
-      si = i;
-      sj = j;
-      sci = ci;
-      scj = cj;
-    }
-  }
+  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
 };
 
-// Reference-semantics value and Jacobian rows of one configuration (exact
-// FK), written at column b of outputs with leading dimension ld.
-template <int MAXC, class QS>
-__device__ __forceinline__ void evaluate_rows(const KinArgs<float>& ka, const SceneArgs<float>& sc, const QS& qs,
-                                              bool active, int64_t b, int64_t ld, float* vals, float* jac) {
-  EvalVis<MAXC> v{ka, sc};
-  run_fk(ka, qs, false, v);
-  if (!active) return;
-  const int D = ka.n_cols;
-  int row = 0;
-  if (sc.n_prims > 0) {
-    vals[b] = v.best;
-    V3<float> g{0.f, 0.f, 0.f};
-    uint32_t mask = 0;
-    if (v.bs >= 0) {
-      g = sdf_grad(prim_at(sc, v.bp), v.bc);
-      mask = cload(ka.nodes, cload(ka.meta, v.bs).node).col_mask;
-    }
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j)
-      if (j < D) jac[(int64_t)j * ld + b] = dot(g, v.fr.col(j, mask, ka.rev_mask, v.bc));
-    row = 1;
-  }
-  if (ka.n_pairs > 0) {
-    vals[(int64_t)row * ld + b] = v.sbest;
-    V3<float> n{0.f, 0.f, 0.f};
-    uint32_t m1 = 0, m2 = 0;
-    if (v.si >= 0) {
-      const V3<float> d = sub(v.sci, v.scj);
-      n = scale(1.0f / sqrtf(dot(d, d)), d);
-      m1 = cload(ka.nodes, cload(ka.meta, v.si).node).col_mask;
-      m2 = cload(ka.nodes, cload(ka.meta, v.sj).node).col_mask;
-    }
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j)
-      if (j < D)
-        jac[((int64_t)row * D + j) * ld + b] =
-            dot(n, sub(v.fr.col(j, m1, ka.rev_mask, v.sci), v.fr.col(j, m2, ka.rev_mask, v.scj)));
-  }
-}
+struct EvalOut {
+  float* vals;  // [cst_dim][ld]
+  float* jac;   // [cst_dim][D][ld]
+  int64_t ld;
+  float* pval;    // [units][ld] unit minimum
+  int32_t* pidx;  // [units][ld] winner (sphere * kMaxPrims + prim, or sphere_i * kMaxSpheres + sphere_j), -1 none
+  int32_t* count; // [chunks] finished units, zeroed before the launch
+  int n_ext_units, n_units;
+  // trajectories (offsets == nullptr: plain evaluate)
+  const int64_t* offsets;
+  int n_traj;
+  const double* w2;
+  float* cost;
+  float* grad;
+};
 
-template <int MAXC>
-__global__ void __launch_bounds__(256) evaluate_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
-                                                       int64_t ldq, int64_t B, float* vals, float* jac) {
-  const int64_t b = gid();
-  const bool active = b < B;
-  BatchQ qs{q, ldq, active ? b : B - 1};
-  evaluate_rows<MAXC>(ka, sc, qs, active, b, B, vals, jac);
-}
-
-// ------------------------------------------------------------ trajectories
-// One launch over every waypoint of a set of trajectories of uneven length
-// (lane per waypoint, trajectories back to back): the evaluate rows of each
-// waypoint (SequentialCst::evaluate, constraints/sequential_constraint.cpp:63-91)
-// and the smoothness term of smoothcost_fullmat
-// (nlp_solver/sqp_based_solver.py:74-96),
+// Smoothness gradient of waypoint n and, from the trajectory's first lane,
+// its cost (smoothcost_fullmat, nlp_solver/sqp_based_solver.py:74-96):
 //   cost_i = sum_{interior t} sum_j w_j^2 a_{t,j}^2,  a_t = x_{t-1} - 2 x_t + x_{t+1},
 //   d cost / d x_t = 2 w^2 (a_{t-1} - 2 a_t + a_{t+1})   (terms of interior indices only).
 // Second differences are formed in FP64 (they cancel), terms accumulate in
-// FP32; the first waypoint's lane sums its trajectory's cost in order, so the
-// result does not depend on scheduling.
+// FP32 in a fixed order.
 template <int MAXC>
-__global__ void __launch_bounds__(256) traj_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* X,
-                                                   int64_t ldx, int64_t N, const int64_t* offsets, int n_traj,
-                                                   const double* w2, float* vals, float* jac, float* cost,
-                                                   float* grad) {
-  const int64_t n = gid();
-  const bool active = n < N;
-  BatchQ qs{X, ldx, active ? n : N - 1};
-  evaluate_rows<MAXC>(ka, sc, qs, active, n, N, vals, jac);
-  if (!active) return;
-  int lo = 0, hi = n_traj;  // offsets[lo] <= n < offsets[hi]
+__device__ void smooth_terms(const double* X, int64_t ldx, int64_t n, int D, const EvalOut& o) {
+  int lo = 0, hi = o.n_traj;  // offsets[lo] <= n < offsets[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (offsets[mid] <= n) lo = mid;
+    if (o.offsets[mid] <= n) lo = mid;
     else hi = mid;
   }
-  const int64_t o = offsets[lo], T = offsets[lo + 1] - o, t = n - o;
-  const int D = ka.n_cols;
-  auto x = [&](int64_t s, int j) { return X[(int64_t)j * ldx + o + s]; };
+  const int64_t b0 = o.offsets[lo], T = o.offsets[lo + 1] - b0, t = n - b0;
+  auto x = [&](int64_t s, int j) { return X[(int64_t)j * ldx + b0 + s]; };
   auto acc = [&](int64_t s, int j) { return x(s - 1, j) - 2.0 * x(s, j) + x(s + 1, j); };
 #pragma unroll
   for (int j = 0; j < MAXC; ++j) {
@@ -745,7 +739,7 @@ __global__ void __launch_bounds__(256) traj_kernel(KinArgs<float> ka, SceneArgs<
       if (t - 1 >= 1 && t - 1 <= T - 2) g += acc(t - 1, j);
       if (t >= 1 && t <= T - 2) g -= 2.0 * acc(t, j);
       if (t + 1 >= 1 && t + 1 <= T - 2) g += acc(t + 1, j);
-      grad[(int64_t)j * N + n] = (float)(2.0 * w2[j] * g);
+      o.grad[(int64_t)j * o.ld + n] = (float)(2.0 * o.w2[j] * g);
     }
   }
   if (t == 0) {
@@ -753,13 +747,109 @@ __global__ void __launch_bounds__(256) traj_kernel(KinArgs<float> ka, SceneArgs<
     for (int64_t s = 1; s + 1 < T; ++s)
       for (int j = 0; j < D; ++j) {
         const double a = acc(s, j);
-        c += (float)(w2[j] * a * a);
+        c += (float)(o.w2[j] * a * a);
       }
-    cost[lo] = c;
+    o.cost[lo] = c;
   }
 }
 
-// ------------------------------------------------ per-pair values and adjoint
+template <int MAXC>
+__global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
+                                                        int64_t ldq, int64_t B, EvalOut o) {
+  const int lane = threadIdx.x;
+  const int64_t chunk = blockIdx.x;
+  const int64_t b = chunk * 64 + lane;
+  const bool active = b < B;
+  const int u = blockIdx.y;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  if (u < o.n_units) {
+    EvalUnitVis<MAXC> v{ka, sc};
+    if (u < o.n_ext_units) {
+      v.it = cload(ka.items, u);
+      v.stop = v.it.node;
+    } else {
+      v.mode = 1;
+      v.pr = cload(ka.pairs, u - o.n_ext_units);
+      v.stop = max(v.pr.node1, v.pr.node2);
+    }
+    run_fk(ka, qs, false, v);
+    if (v.mode == 1) v.self_pair();
+    if (active) {
+      o.pval[(int64_t)u * o.ld + b] = v.best;
+      o.pidx[(int64_t)u * o.ld + b] = v.bidx;
+    }
+  }
+  // the last unit of this chunk reduces it
+  __threadfence();
+  int prev = 0;
+  if (lane == 0) prev = atomicAdd(&o.count[chunk], 1);
+  prev = __shfl(prev, 0, 64);
+  if (prev != max(o.n_units, 1) - 1) return;
+  __threadfence();
+  if (!active) return;
+  const int D = ka.n_cols, NP = sc.n_prims;
+  float best = kCutoff, sbest = kCutoff;
+  int bkey = 0x7fffffff, bidx = -1, sidx = -1;
+  for (int k = 0; k < o.n_ext_units; ++k) {
+    const float val = o.pval[(int64_t)k * o.ld + b];
+    const int idx = o.pidx[(int64_t)k * o.ld + b];
+    if (idx < 0) continue;
+    const DevSphereMeta m = cload(ka.meta, idx / kMaxPrims);
+    const int key = (m.group * NP + idx % kMaxPrims) * ka.n_spheres + m.out;
+    if (val < best || (bidx >= 0 && val == best && key < bkey)) {
+      best = val;
+      bkey = key;
+      bidx = idx;
+    }
+  }
+  for (int k = o.n_ext_units; k < o.n_units; ++k) {
+    const float val = o.pval[(int64_t)k * o.ld + b];
+    const int idx = o.pidx[(int64_t)k * o.ld + b];
+    if (idx >= 0 && val < sbest) {
+      sbest = val;
+      sidx = idx;
+    }
+  }
+  EvalRowVis<MAXC> rv{ka};
+  if (bidx >= 0) rv.s0 = bidx / kMaxPrims;
+  if (sidx >= 0) {
+    rv.s1 = sidx / kMaxSpheres;
+    rv.s2 = sidx % kMaxSpheres;
+  }
+  run_fk(ka, qs, false, rv);
+  int row = 0;
+  if (NP > 0) {
+    o.vals[b] = best;
+    V3<float> g{0.f, 0.f, 0.f};
+    uint32_t mask = 0;
+    if (bidx >= 0) {
+      g = sdf_grad(prim_at(sc, bidx % kMaxPrims), rv.c0);
+      mask = cload(ka.nodes, cload(ka.meta, rv.s0).node).col_mask;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j < D) o.jac[(int64_t)j * o.ld + b] = dot(g, rv.fr.col(j, mask, ka.rev_mask, rv.c0));
+    row = 1;
+  }
+  if (ka.n_pairs > 0) {
+    o.vals[(int64_t)row * o.ld + b] = sbest;
+    V3<float> n{0.f, 0.f, 0.f};
+    uint32_t m1 = 0, m2 = 0;
+    if (sidx >= 0) {
+      const V3<float> d = sub(rv.c1, rv.c2);
+      n = scale(1.0f / sqrtf(dot(d, d)), d);
+      m1 = cload(ka.nodes, cload(ka.meta, rv.s1).node).col_mask;
+      m2 = cload(ka.nodes, cload(ka.meta, rv.s2).node).col_mask;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j < D)
+        o.jac[((int64_t)row * D + j) * o.ld + b] =
+            dot(n, sub(rv.fr.col(j, m1, ka.rev_mask, rv.c1), rv.fr.col(j, m2, ka.rev_mask, rv.c2)));
+  }
+  if (o.offsets) smooth_terms<MAXC>(q, ldq, b, D, o);
+}
+
 // The pair and adjoint kernels give each (configuration, sphere item) its
 // own lane: the wave's item is uniform (grid y), FK runs only down to the
 // item's node, and the item's spheres meet every primitive.  Lanes b of a
@@ -1021,24 +1111,38 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
       hipLaunchKernelGGL(KERNEL<32>, __VA_ARGS__);                                            \
   } while (0)
 
-hipError_t launch_evaluate(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
-                           int64_t B, float* vals, float* jac, hipStream_t st) {
-  PMP_DISPATCH_MAXC(ka.n_cols, evaluate_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, vals,
-                    jac);
-  return hipGetLastError();
-}
 __global__ void fill_f32_kernel(float* p, float v, int64_t n) {
   const int64_t i = gid();
   if (i < n) p[i] = v;
 }
-hipError_t launch_traj(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* X, int64_t ldx, int64_t N,
-                       const int64_t* offsets, int n_traj, const double* w2, float* vals, float* jac, float* cost,
-                       float* grad, hipStream_t st) {
-  // empty trajectories have no lane of their own
-  hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks_for(n_traj, 256)), dim3(256), 0, st, cost, 0.0f, (int64_t)n_traj);
-  if (N == 0) return hipGetLastError();
-  PMP_DISPATCH_MAXC(ka.n_cols, traj_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, ka, sc, X, ldx, N, offsets,
-                    n_traj, w2, vals, jac, cost, grad);
+__global__ void fill_counts_kernel(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = gid();
+  if (i < n) p[i] = v;
+}
+hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                             int64_t B, const EvalArgs& e, hipStream_t st) {
+  EvalOut o{};
+  o.vals = e.vals;
+  o.jac = e.jac;
+  o.ld = B;
+  o.pval = e.pval;
+  o.pidx = e.pidx;
+  o.count = e.count;
+  o.n_ext_units = sc.n_prims > 0 ? ka.n_items : 0;
+  o.n_units = o.n_ext_units + ka.n_pairs;
+  o.offsets = e.offsets;
+  o.n_traj = e.n_traj;
+  o.w2 = e.w2;
+  o.cost = e.cost;
+  o.grad = e.grad;
+  if (e.offsets)  // empty trajectories have no lane of their own
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks_for(e.n_traj, 256)), dim3(256), 0, st, e.cost, 0.0f,
+                       (int64_t)e.n_traj);
+  if (B == 0) return hipGetLastError();
+  const int64_t chunks = (B + 63) / 64;
+  hipLaunchKernelGGL(fill_counts_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0, st, e.count, 0, chunks);
+  PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks, (unsigned)max(o.n_units, 1)), dim3(64), 0,
+                    st, ka, sc, q, ldq, B, o);
   return hipGetLastError();
 }
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,

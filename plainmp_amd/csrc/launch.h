@@ -51,11 +51,23 @@ hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
                         hipStream_t st, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr);
-hipError_t launch_evaluate(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
-                           int64_t B, float* vals, float* jac, hipStream_t st);
-hipError_t launch_traj(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* X, int64_t ldx, int64_t N,
-                       const int64_t* offsets, int n_traj, const double* w2, float* vals, float* jac, float* cost,
-                       float* grad, hipStream_t st);
+// Evaluate rows (and, with offsets, trajectory smoothness) by units; the
+// workspace holds units * B values and winners and one counter per 64
+// configurations.
+struct EvalArgs {
+  float* vals;
+  float* jac;
+  float* pval;
+  int32_t* pidx;
+  int32_t* count;
+  const int64_t* offsets;  // device, n_traj + 1; nullptr for plain evaluate
+  int n_traj;
+  const double* w2;
+  float* cost;
+  float* grad;
+};
+hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
+                             int64_t B, const EvalArgs& e, hipStream_t st);
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                         int64_t B, float* dist, float* grad, hipStream_t st);
 // part: workspace of n_items * n_cols * B floats
