@@ -127,11 +127,11 @@ struct pmp_context {
   hipStream_t capture_stream = nullptr;  // used for graph capture when no stream is set
   hipEvent_t edge_ev0 = nullptr, edge_ev1 = nullptr;  // around the last direct edge kernel
   float band = 1e-5f;
-  pmp_schedule sched{0, 0, 0};
+  pmp_schedule sched{0, 0, 0, 0};
   int n_cu = 256;
   // robot
   bool has_robot = false;
-  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, ws_vjp, ws_eval;
+  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, units, unit_all, ws_vjp, ws_eval;
   KinArgs<float> k32{};
   KinArgs<double> k64{};
   int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
@@ -482,13 +482,13 @@ void require_robot(pmp_context* ctx) {
 }
 // Unit values / winners ([units][B]) and per-chunk counters of the evaluate kernels.
 EvalArgs eval_workspace(pmp_context* ctx, int64_t B) {
-  const int64_t units = std::max<int64_t>(1, (int64_t)ctx->k32.n_items + ctx->k32.n_pairs);
+  const int64_t units = std::max<int64_t>(1, ctx->k32.n_units);
   const int64_t chunks = std::max<int64_t>(1, (B + 63) / 64);
-  char* w = (char*)ctx->ws_eval.get(units * B * 8 + chunks * 4 + 256);
+  char* w = (char*)ctx->ws_eval.get(units * B * 16 + chunks * 4 + 256);
   EvalArgs e{};
   e.pval = (float*)w;
-  e.pidx = (int32_t*)(w + units * B * 4);
-  e.count = (int32_t*)(w + units * B * 8);
+  e.pidx = (int32_t*)(w + units * B * 8);
+  e.count = (int32_t*)(w + units * B * 16);
   return e;
 }
 
@@ -535,7 +535,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
-    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->ws_vjp, &ctx->ws_eval,
+    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->units, &ctx->unit_all, &ctx->ws_vjp, &ctx->ws_eval,
                       &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
@@ -580,6 +580,7 @@ pmp_status pmp_set_schedule(pmp_context* ctx, const pmp_schedule* s) {
     PMP_REQUIRE(ctx != nullptr && s != nullptr, "null argument");
     PMP_REQUIRE(s->block == 0 || (s->block % 64 == 0 && s->block <= 1024), "block must be a multiple of 64, <= 1024");
     PMP_REQUIRE(s->waves_per_cu >= 0 && s->waves_per_cu <= 32, "waves_per_cu must be in [0, 32]");
+    PMP_REQUIRE(s->eval_whole_chunks >= 0, "eval_whole_chunks must be >= 0");
     PMP_REQUIRE(s->lanes_per_item == 0 || s->lanes_per_item == 64, "lanes_per_item must be 64");
     ctx->sched = *s;
   });
@@ -831,6 +832,15 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
         items.push_back({i, b0, std::min(b0 + kItemSpheres, n32[i].sphere_end), 0});
     a.items = d.items = ctx->items.upload(items, s0);
     a.n_items = d.n_items = (int32_t)items.size();
+    // evaluate units: one per sphere item, then one per self pair; and the whole configuration
+    std::vector<DevEvalUnit> units;
+    for (const DevSphereItem& it : items) units.push_back({it.begin, it.end, 0, 0, it.node, {0, 0, 0}});
+    for (int k = 0; k < (int)pairs.size(); ++k)
+      units.push_back({0, 0, k, k + 1, std::max(pairs[k].node1, pairs[k].node2), {0, 0, 0}});
+    std::vector<DevEvalUnit> whole{{0, S, 0, (int32_t)pairs.size(), N - 1, {0, 0, 0}}};
+    a.units = d.units = ctx->units.upload(units, s0);
+    a.n_units = d.n_units = (int32_t)units.size();
+    a.unit_all = d.unit_all = ctx->unit_all.upload(whole, s0);
     std::vector<float> rad(std::max(S, 1), 0.f);
     for (int s2 = 0; s2 < S; ++s2) rad[s2] = (float)r->sphere_radius[s2];
     ctx->rad_out.upload(rad, s0);
@@ -1024,6 +1034,7 @@ pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, in
     EvalArgs e = eval_workspace(ctx, B);
     e.vals = vals;
     e.jac = jac;
+    e.whole_chunks = ctx->sched.eval_whole_chunks;
     PMP_HIP(launch_eval_units(ctx->k32, ctx->s32, q, ldq, B, e, st(ctx)));
   });
 }
@@ -1063,6 +1074,7 @@ pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t l
     e.w2 = dw;
     e.cost = cost;
     e.grad = grad;
+    e.whole_chunks = ctx->sched.eval_whole_chunks;
     PMP_HIP(launch_eval_units(ctx->k32, ctx->s32, X, ldx, N, e, st(ctx)));
   });
 }

@@ -36,6 +36,18 @@ struct uses_blocks<V, decltype((void)V::kBlocks)> {
   static constexpr bool value = V::kBlocks;
 };
 
+// Visitors that declare `static constexpr bool kRanged = true` walk nodes
+// only up to vis.last_node() and visit only the self pairs
+// [vis.pair_lo(), vis.pair_hi()).
+template <class V, class = void>
+struct is_ranged {
+  static constexpr bool value = false;
+};
+template <class V>
+struct is_ranged<V, decltype((void)V::kRanged)> {
+  static constexpr bool value = V::kRanged;
+};
+
 // Register file of kept poses.  The slot index is wave-uniform, so the
 // switch compiles to scalar branches around plain register moves.
 template <class T>
@@ -121,7 +133,9 @@ __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool 
   const Pose<T> base = base_pose(ka, qs);
   vis.begin(base, qs);
   Pose<T> prev = base;
-  for (int n = 0; n < ka.n_nodes; ++n) {
+  int n_walk = ka.n_nodes;
+  if constexpr (is_ranged<Vis>::value) n_walk = vis.last_node() + 1 < n_walk ? vis.last_node() + 1 : n_walk;
+  for (int n = 0; n < n_walk; ++n) {
     if (!wave_any(vis.alive())) return;
     const DevNode<T> nd = cload(ka.nodes, n);
     // parent_slot: -2 the previous node, -1 the base pose, >= 0 a kept pose
@@ -178,7 +192,12 @@ __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool 
   // Self-collision sphere pairs, group pair by group pair
   // (constraints/primitive_sphere_collision.cpp:222-254, 324-383).
   if constexpr (!Vis::kSelf) return;
-  for (int k = 0; k < ka.n_pairs; ++k) {
+  int k0 = 0, k1 = ka.n_pairs;
+  if constexpr (is_ranged<Vis>::value) {
+    k0 = vis.pair_lo();
+    k1 = vis.pair_hi();
+  }
+  for (int k = k0; k < k1; ++k) {
     if (!wave_any(vis.alive())) return;
     const DevSelfPair pr = cload(ka.pairs, k);
     const Pose<T> P1 = slots.get(pr.slot1), P2 = slots.get(pr.slot2);

@@ -14,6 +14,9 @@
 namespace pmp {
 
 constexpr float kCutoff = 0.1f;  // SphereCollisionCst::cutoff_dist_ (primitive_sphere_collision.hpp:115)
+// Evaluate batches of at least this many 64-configuration chunks walk each
+// configuration as one unit (the launch is large enough without splitting).
+constexpr int64_t kWholeUnitChunks = 512;
 // Edges are walked 64 states at a time; an edge needing more states than this
 // (a step ratio below 2^-24) is reported invalid with n_checked = -1 instead.
 constexpr double kMaxEdgeStates = 16777216.0;
@@ -616,29 +619,22 @@ struct Frames {
 // recomputes the winning Jacobian rows with one more FK walk.
 template <int MAXC>
 struct EvalUnitVis {
-  static constexpr bool kSelf = false;
+  static constexpr bool kSelf = true;
+  static constexpr bool kRanged = true;
   const KinArgs<float>& ka;
   const SceneArgs<float>& sc;
-  int mode = 0;  // 0: sphere item, 1: self group pair
-  DevSphereItem it{};
-  DevSelfPair pr{};
-  int stop = 0;
-  bool done = false;
-  Pose<float> P1{}, P2{};
-  float best = kCutoff;
-  int bkey = 0x7fffffff, bidx = -1;
-  __device__ bool alive() const { return !done; }
+  DevEvalUnit u{};
+  float best = kCutoff, sbest = kCutoff;
+  int bkey = 0x7fffffff, bidx = -1, sidx = -1;
+  __device__ bool alive() const { return true; }
+  __device__ int last_node() const { return u.last; }
+  __device__ int pair_lo() const { return u.k_lo; }
+  __device__ int pair_hi() const { return u.k_hi; }
   template <class QS>
   __device__ void begin(const Pose<float>&, const QS&) {}
-  __device__ void node(int n, const DevNode<float>&, const Pose<float>& P) {
-    if (mode == 1) {
-      if (n == pr.node1) P1 = P;
-      if (n == pr.node2) P2 = P;
-    }
-    if (n == stop) done = true;  // a sphere item's spheres follow in this same step
-  }
+  __device__ void node(int, const DevNode<float>&, const Pose<float>&) {}
   __device__ void sphere(int s, V3<float> c, float r) {
-    if (mode != 0 || s < it.begin || s >= it.end) return;
+    if (s < u.s_lo || s >= u.s_hi) return;
     const DevSphereMeta m = cload(ka.meta, s);
     if (m.ext < 0) return;
     for (int p = 0; p < sc.n_prims; ++p) {
@@ -654,23 +650,12 @@ struct EvalUnitVis {
       }
     }
   }
-  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
-  // the self group pair, after the walk has reached both of its nodes
-  __device__ void self_pair() {
-    const M3<float> R1 = qmat(P1.q), R2 = qmat(P2.q);
-    for (int i = pr.begin1; i < pr.end1; ++i) {
-      const DevSphere<float> a = cload(ka.spheres, i);
-      const V3<float> ci = add(mv(R1, V3<float>{a.x, a.y, a.z}), P1.t);
-      for (int j = pr.begin2; j < pr.end2; ++j) {
-        const DevSphere<float> bb = cload(ka.spheres, j);
-        const V3<float> cj = add(mv(R2, V3<float>{bb.x, bb.y, bb.z}), P2.t);
-        const V3<float> d = sub(ci, cj);
-        const float dist = sqrtf(dot(d, d)) - (a.r + bb.r);
-        if (dist < best) {
-          best = dist;
-          bidx = i * kMaxSpheres + j;
-        }
-      }
+  __device__ void self(const DevSelfPair&, int i, int j, V3<float> ci, V3<float> cj, float rs) {
+    const V3<float> d = sub(ci, cj);
+    const float dist = sqrtf(dot(d, d)) - rs;
+    if (dist < sbest) {
+      sbest = dist;
+      sidx = i * kMaxSpheres + j;
     }
   }
 };
@@ -703,10 +688,11 @@ struct EvalOut {
   float* vals;  // [cst_dim][ld]
   float* jac;   // [cst_dim][D][ld]
   int64_t ld;
-  float* pval;    // [units][ld] unit minimum
-  int32_t* pidx;  // [units][ld] winner (sphere * kMaxPrims + prim, or sphere_i * kMaxSpheres + sphere_j), -1 none
+  const DevEvalUnit* units;
+  int n_units;
+  float* pval;    // [2][units][ld] unit minima: external, then self
+  int32_t* pidx;  // [2][units][ld] winners (sphere * kMaxPrims + prim; sphere_i * kMaxSpheres + sphere_j), -1 none
   int32_t* count; // [chunks] finished units, zeroed before the launch
-  int n_ext_units, n_units;
   // trajectories (offsets == nullptr: plain evaluate)
   const int64_t* offsets;
   int n_traj;
@@ -762,21 +748,15 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
   const bool active = b < B;
   const int u = blockIdx.y;
   BatchQ qs{q, ldq, active ? b : B - 1};
-  if (u < o.n_units) {
-    EvalUnitVis<MAXC> v{ka, sc};
-    if (u < o.n_ext_units) {
-      v.it = cload(ka.items, u);
-      v.stop = v.it.node;
-    } else {
-      v.mode = 1;
-      v.pr = cload(ka.pairs, u - o.n_ext_units);
-      v.stop = max(v.pr.node1, v.pr.node2);
-    }
+  const int64_t UB = (int64_t)o.n_units * o.ld;
+  {
+    EvalUnitVis<MAXC> v{ka, sc, cload(o.units, u)};
     run_fk(ka, qs, false, v);
-    if (v.mode == 1) v.self_pair();
     if (active) {
       o.pval[(int64_t)u * o.ld + b] = v.best;
       o.pidx[(int64_t)u * o.ld + b] = v.bidx;
+      o.pval[UB + (int64_t)u * o.ld + b] = v.sbest;
+      o.pidx[UB + (int64_t)u * o.ld + b] = v.sidx;
     }
   }
   // the last unit of this chunk reduces it
@@ -784,30 +764,30 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
   int prev = 0;
   if (lane == 0) prev = atomicAdd(&o.count[chunk], 1);
   prev = __shfl(prev, 0, 64);
-  if (prev != max(o.n_units, 1) - 1) return;
+  if (prev != o.n_units - 1) return;
   __threadfence();
   if (!active) return;
   const int D = ka.n_cols, NP = sc.n_prims;
   float best = kCutoff, sbest = kCutoff;
   int bkey = 0x7fffffff, bidx = -1, sidx = -1;
-  for (int k = 0; k < o.n_ext_units; ++k) {
+  for (int k = 0; k < o.n_units; ++k) {
     const float val = o.pval[(int64_t)k * o.ld + b];
     const int idx = o.pidx[(int64_t)k * o.ld + b];
-    if (idx < 0) continue;
-    const DevSphereMeta m = cload(ka.meta, idx / kMaxPrims);
-    const int key = (m.group * NP + idx % kMaxPrims) * ka.n_spheres + m.out;
-    if (val < best || (bidx >= 0 && val == best && key < bkey)) {
-      best = val;
-      bkey = key;
-      bidx = idx;
+    if (idx >= 0) {
+      const DevSphereMeta m = cload(ka.meta, idx / kMaxPrims);
+      const int key = (m.group * NP + idx % kMaxPrims) * ka.n_spheres + m.out;
+      if (val < best || (bidx >= 0 && val == best && key < bkey)) {
+        best = val;
+        bkey = key;
+        bidx = idx;
+      }
     }
-  }
-  for (int k = o.n_ext_units; k < o.n_units; ++k) {
-    const float val = o.pval[(int64_t)k * o.ld + b];
-    const int idx = o.pidx[(int64_t)k * o.ld + b];
-    if (idx >= 0 && val < sbest) {
-      sbest = val;
-      sidx = idx;
+    // self pairs: units hold increasing pair ranges, so unit order is visiting order
+    const float sval = o.pval[UB + (int64_t)k * o.ld + b];
+    const int sid = o.pidx[UB + (int64_t)k * o.ld + b];
+    if (sid >= 0 && sval < sbest) {
+      sbest = sval;
+      sidx = sid;
     }
   }
   EvalRowVis<MAXC> rv{ka};
@@ -1121,15 +1101,21 @@ __global__ void fill_counts_kernel(int32_t* p, int32_t v, int64_t n) {
 }
 hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                              int64_t B, const EvalArgs& e, hipStream_t st) {
+  const int64_t chunks = (B + 63) / 64;
   EvalOut o{};
   o.vals = e.vals;
   o.jac = e.jac;
   o.ld = B;
+  // Small batches give each configuration many units (one per sphere item and
+  // per self pair) so that the launch fills the GPU; from kWholeUnitChunks
+  // chunks on, one unit walks the whole configuration once.
+  const int64_t whole = e.whole_chunks > 0 ? e.whole_chunks : kWholeUnitChunks;
+  const bool split = chunks < whole && ka.n_units > 1;
+  o.units = split ? ka.units : ka.unit_all;
+  o.n_units = split ? ka.n_units : 1;
   o.pval = e.pval;
   o.pidx = e.pidx;
   o.count = e.count;
-  o.n_ext_units = sc.n_prims > 0 ? ka.n_items : 0;
-  o.n_units = o.n_ext_units + ka.n_pairs;
   o.offsets = e.offsets;
   o.n_traj = e.n_traj;
   o.w2 = e.w2;
@@ -1139,10 +1125,9 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
     hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks_for(e.n_traj, 256)), dim3(256), 0, st, e.cost, 0.0f,
                        (int64_t)e.n_traj);
   if (B == 0) return hipGetLastError();
-  const int64_t chunks = (B + 63) / 64;
   hipLaunchKernelGGL(fill_counts_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0, st, e.count, 0, chunks);
-  PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks, (unsigned)max(o.n_units, 1)), dim3(64), 0,
-                    st, ka, sc, q, ldq, B, o);
+  PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks, (unsigned)o.n_units), dim3(64), 0, st, ka,
+                    sc, q, ldq, B, o);
   return hipGetLastError();
 }
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,

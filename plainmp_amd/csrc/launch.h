@@ -65,6 +65,7 @@ struct EvalArgs {
   const double* w2;
   float* cost;
   float* grad;
+  int whole_chunks;  // 0: default
 };
 hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                              int64_t B, const EvalArgs& e, hipStream_t st);

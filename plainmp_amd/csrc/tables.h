@@ -82,6 +82,13 @@ struct alignas(16) DevSphereItem {
   int32_t node, begin, end, pad;
 };
 
+// A unit of the evaluate kernels: the spheres [s_lo, s_hi) against every
+// primitive and the self pairs [k_lo, k_hi), walking FK to node `last`.
+struct alignas(16) DevEvalUnit {
+  int32_t s_lo, s_hi, k_lo, k_hi;
+  int32_t last, pad[3];
+};
+
 struct alignas(16) DevSphereMeta {
   int32_t out;    // index in SphereCollisionCst group order
   int32_t ext;    // index among the spheres checked against primitives, -1 for only-self groups
@@ -122,6 +129,9 @@ struct KinArgs {
   const DevSphereBlock* blocks;
   const DevSphereItem* items;
   int32_t n_items;
+  const DevEvalUnit* units;  // one unit per sphere item, then one per self pair
+  int32_t n_units;
+  const DevEvalUnit* unit_all;  // one unit: every sphere and self pair
   int32_t n_nodes, n_spheres, n_pairs, n_cols;
   int32_t n_q_cols, base_type, n_ext_spheres;
   uint32_t rev_mask;  // q columns that rotate (revolute joints, base yaw/roll/pitch)

@@ -339,3 +339,26 @@ def test_trajectory_edge_cases():
         pc.traj_cost_grad_batch(X, [0, 3, 2, 7])
     with pytest.raises(ValueError):
         pc.traj_cost_grad_batch(X, [0, 3, 6])
+
+
+@pytest.mark.parametrize("whole_chunks", [1, 0])
+def test_evaluate_unit_modes(whole_chunks):
+    """Evaluate rows and trajectories with each configuration walked as one
+    unit (whole_chunks = 1) and split per sphere run and self pair (the
+    default for these batch sizes) agree with the oracle."""
+    wl = synth.workload("traj_dual_arm", scale=0.004)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    pc._dev_set_schedule(0, 0, 0, whole_chunks)
+    X, off = wl.traj_X, wl.traj_offsets
+    vals, jac, cost, grad = pc.traj_cost_grad_batch(X, [int(v) for v in off])
+    vref, jref = oc.evaluate_batch(X, 8)
+    assert np.abs(vals - vref).max() <= 5e-6
+    pd = oc.pair_clearance_batch(X, True, 8)
+    clear = (_second_gap(pd) > 1e-5) & (vref[:, 0] < 0.1)
+    assert (np.abs(jac[clear, 0] - jref[clear, 0]) <= 1e-4 * np.maximum(np.abs(jref[clear, 0]), 1.0)).all()
+    D = X.shape[1]
+    for i in range(len(off) - 1):
+        cr, gr = _oracle.smooth_cost(X[off[i]:off[i + 1]].ravel(), int(off[i + 1] - off[i]), D, [1.0] * D)
+        assert abs(cost[i] - cr) <= 1e-5 * max(1.0, abs(cr))
+    v2, _ = pc.evaluate_batch(X[:300])
+    assert np.abs(v2 - vref[:300]).max() <= 5e-6
