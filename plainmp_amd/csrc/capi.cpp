@@ -148,7 +148,8 @@ struct pmp_context {
   std::vector<CloudRange> clouds;
   std::vector<DevPrim<double>> h_vprims;  // kind-sorted, FP64 bounds
   int h_vrange[kNumKinds + 1] = {0};
-  DevBuf cull, cull_range;
+  DevBuf cull, cull_range, ecull, ecull_range, vorig;
+  std::vector<int32_t> h_vorig;
   // misc
   DevBuf traj_off, traj_w2;
   std::vector<int64_t> h_traj_off;
@@ -281,14 +282,14 @@ float round_up(double x) {
 // grown by the block's reach, the block's bounding centre can enter; with a
 // fixed base a primitive is dropped when the grown box lies outside the ball
 // that the centre can ever reach.
-void rebuild_cull(pmp_context* ctx) {
+void build_cull(pmp_context* ctx, double margin, std::vector<CullEntry>& ent, std::vector<CullRange>& rng) {
   const int nb = (int)ctx->h_blocks.size();
-  std::vector<CullEntry> ent;
-  std::vector<CullRange> rng(std::max(nb, 1));
+  ent.clear();
+  rng.assign(std::max(nb, 1), CullRange{});
   for (int b = 0; b < nb; ++b) {
     const NodeReach& nr = ctx->h_reach[ctx->h_block_node[b]];
     const DevSphereBlock& bk = ctx->h_blocks[b];
-    const double reach = (double)bk.G * nr.grow + (double)ctx->band + 1e-4;
+    const double reach = (double)bk.G * nr.grow + margin + 1e-4;
     const double gn = std::sqrt((double)bk.g[0] * bk.g[0] + (double)bk.g[1] * bk.g[1] + (double)bk.g[2] * bk.g[2]);
     const double ball = nr.rad + nr.grow * gn + 1e-4;
     for (int k = 0; k < kNumKinds; ++k) {
@@ -310,16 +311,30 @@ void rebuild_cull(pmp_context* ctx) {
           ce.ub[d] = round_up(hi[d]);
         }
         ce.prim = p;
+        ce.orig = ctx->h_vorig[p];
         ent.push_back(ce);
       }
     }
     rng[b].at[kNumKinds] = (int)ent.size();
   }
+}
+
+// The validity table (boxes grown by the band) and the evaluate table (by
+// the 0.1 m cutoff of evaluate_ext_collision's prefilter).
+void rebuild_cull(pmp_context* ctx) {
+  std::vector<CullEntry> ent;
+  std::vector<CullRange> rng;
   hipStream_t s0 = st(ctx);
+  build_cull(ctx, (double)ctx->band, ent, rng);
   const CullEntry* c = ctx->cull.upload(ent, s0);
   const CullRange* cr = ctx->cull_range.upload(rng, s0);
   ctx->s32.cull = ctx->s64.cull = c;
   ctx->s32.cull_range = ctx->s64.cull_range = cr;
+  build_cull(ctx, 0.1, ent, rng);
+  c = ctx->ecull.upload(ent, s0);
+  cr = ctx->ecull_range.upload(rng, s0);
+  ctx->s32.ecull = ctx->s64.ecull = c;
+  ctx->s32.ecull_range = ctx->s64.ecull_range = cr;
 }
 
 template <class T>
@@ -536,7 +551,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
     for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->units, &ctx->unit_all, &ctx->ws_vjp, &ctx->ws_eval,
-                      &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->cull_range, &ctx->rad_out,
+                      &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->ecull, &ctx->ecull_range, &ctx->vorig, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
                       &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2})
@@ -834,10 +849,10 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     a.n_items = d.n_items = (int32_t)items.size();
     // evaluate units: one per sphere item, then one per self pair; and the whole configuration
     std::vector<DevEvalUnit> units;
-    for (const DevSphereItem& it : items) units.push_back({it.begin, it.end, 0, 0, it.node, {0, 0, 0}});
+    for (int b = 0; b < (int)blocks.size(); ++b) units.push_back({b, b + 1, 0, 0, block_node[b], {0, 0, 0}});
     for (int k = 0; k < (int)pairs.size(); ++k)
       units.push_back({0, 0, k, k + 1, std::max(pairs[k].node1, pairs[k].node2), {0, 0, 0}});
-    std::vector<DevEvalUnit> whole{{0, S, 0, (int32_t)pairs.size(), N - 1, {0, 0, 0}}};
+    std::vector<DevEvalUnit> whole{{0, (int32_t)blocks.size(), 0, (int32_t)pairs.size(), N - 1, {0, 0, 0}}};
     a.units = d.units = ctx->units.upload(units, s0);
     a.n_units = d.n_units = (int32_t)units.size();
     a.unit_all = d.unit_all = ctx->unit_all.upload(whole, s0);
@@ -907,6 +922,7 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     // the primitive order)
     std::vector<DevPrim<float>> v32;
     std::vector<DevPrim<double>> v64;
+    std::vector<int32_t> vorig;
     int vrange[kNumKinds + 1] = {0};
     for (int k = 0; k < kNumKinds; ++k) {
       vrange[k] = (int)v32.size();
@@ -914,6 +930,7 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
         if (prim_kind(p32[i]) == k) {
           v32.push_back(p32[i]);
           v64.push_back(p64[i]);
+          vorig.push_back((int32_t)i);
         }
     }
     vrange[kNumKinds] = (int)v32.size();
@@ -924,6 +941,7 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     a.prims = ctx->prims32.upload(p32, s0);
     a.vprims = ctx->vprims32.upload(v32, s0);
     d.vprims = ctx->vprims64.upload(v64, s0);
+    a.vorig = d.vorig = ctx->vorig.upload(vorig, s0);
     for (int k = 0; k <= kNumKinds; ++k) a.vrange[k] = d.vrange[k] = vrange[k];
     d.prims = ctx->prims64.upload(p64, s0);
     a.tiles = d.tiles = ctx->tiles32.upload(t32, s0);
@@ -937,6 +955,7 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     ctx->n_clouds = n_clouds;
     ctx->clouds = clouds;
     ctx->h_vprims = v64;
+    ctx->h_vorig = vorig;
     for (int k = 0; k <= kNumKinds; ++k) ctx->h_vrange[k] = vrange[k];
     if (ctx->has_robot) rebuild_cull(ctx);
   });

@@ -621,6 +621,7 @@ template <int MAXC>
 struct EvalUnitVis {
   static constexpr bool kSelf = true;
   static constexpr bool kRanged = true;
+  static constexpr bool kBlocks = true;
   const KinArgs<float>& ka;
   const SceneArgs<float>& sc;
   DevEvalUnit u{};
@@ -633,23 +634,81 @@ struct EvalUnitVis {
   template <class QS>
   __device__ void begin(const Pose<float>&, const QS&) {}
   __device__ void node(int, const DevNode<float>&, const Pose<float>&) {}
-  __device__ void sphere(int s, V3<float> c, float r) {
-    if (s < u.s_lo || s >= u.s_hi) return;
-    const DevSphereMeta m = cload(ka.meta, s);
-    if (m.ext < 0) return;
-    for (int p = 0; p < sc.n_prims; ++p) {
-      const DevPrim<float> P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) continue;
-      if (outside_aabb(P, c, r + kCutoff)) continue;
-      const float v = sdf_eval(P, c) - r;
-      const int key = (m.group * sc.n_prims + p) * ka.n_spheres + m.out;
-      if (v < best || (bidx >= 0 && v == best && key < bkey)) {
-        best = v;
-        bkey = key;
-        bidx = s * kMaxPrims + p;
+  __device__ void sphere(int, V3<float>, float) {}
+
+  // Candidates of one kind (all primitives of that kind when `all`): the
+  // reference skips a pair whose sphere, grown by the cutoff, is outside the
+  // primitive's AABB; the evaluate cull table grows the boxes by the block's
+  // radius on top, so a candidate the wave skips is one every lane skips.
+  template <int K>
+  __device__ __forceinline__ void kind_loop(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
+                                            const DevSphereMeta (&m)[kBlockSpheres], V3<float> g,
+                                            const CullRange& cr, bool all) {
+    const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
+    const int lo = all ? sc.vrange[K] : cr.at[K], hi = all ? sc.vrange[K + 1] : cr.at[K + 1];
+    for (int i = lo; i < hi; ++i) {
+      int p, orig;
+      if (all) {
+        p = i;
+        orig = cload(sc.vorig, i);
+      } else {
+        const CullEntry ce = cload(sc.ecull, i);
+        const bool out = g.x < ce.lb[0] || g.x > ce.ub[0] || g.y < ce.lb[1] || g.y > ce.ub[1] ||
+                         g.z < ce.lb[2] || g.z > ce.ub[2];
+        if (!wave_any(!out)) continue;
+        p = ce.prim;
+        orig = ce.orig;
+      }
+      const DevPrim<float> P = cload(vp, p);
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        if (k < n) {
+          const float r = bk.s[k][3];
+          if (outside_aabb(P, c[k], r + kCutoff)) continue;
+          const float v = sdf_eval(P, c[k]) - r;
+          const int key = (m[k].group * sc.n_prims + orig) * ka.n_spheres + m[k].out;
+          if (v < best || (bidx >= 0 && v == best && key < bkey)) {
+            best = v;
+            bkey = key;
+            bidx = bk.sph[k] * kMaxPrims + orig;
+          }
+        }
       }
     }
   }
+
+  __device__ void blocks(const DevNode<float>& nd, const Pose<float>& P) {
+    const int lo = max(nd.block_begin, u.b_lo), hi = min(nd.block_end, u.b_hi);
+    if (lo >= hi || sc.n_prims == 0) return;
+    const M3<float> R = qmat(P.q);
+    // outside the norm bounds the cull table assumed: every primitive is a candidate
+    const float n2 = P.q.x * P.q.x + P.q.y * P.q.y + P.q.z * P.q.z + P.q.w * P.q.w;
+    const bool all = wave_any(!(n2 >= nd.n2_lo && n2 <= nd.n2_hi));
+    for (int bi = lo; bi < hi; ++bi) {
+      const DevSphereBlock bk = cload(ka.blocks, bi);
+      const int n = bk.n;
+      V3<float> c[kBlockSpheres];
+      DevSphereMeta m[kBlockSpheres];
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        c[k] = P.t;
+        m[k] = DevSphereMeta{0, 0, 0, 0};
+        if (k < n) {
+          c[k] = add(mv(R, V3<float>{bk.s[k][0], bk.s[k][1], bk.s[k][2]}), P.t);
+          m[k] = cload(ka.meta, bk.sph[k]);
+        }
+      }
+      const V3<float> g = add(mv(R, V3<float>{bk.g[0], bk.g[1], bk.g[2]}), P.t);
+      const CullRange cr = cload(sc.ecull_range, bi);
+      kind_loop<kKindBoxAA>(c, bk, n, m, g, cr, all);
+      kind_loop<kKindBoxRot>(c, bk, n, m, g, cr, all);
+      kind_loop<kKindCylZ>(c, bk, n, m, g, cr, all);
+      kind_loop<kKindCylRot>(c, bk, n, m, g, cr, all);
+      kind_loop<kKindSphere>(c, bk, n, m, g, cr, all);
+      kind_loop<kKindGround>(c, bk, n, m, g, cr, all);
+    }
+  }
+
   __device__ void self(const DevSelfPair&, int i, int j, V3<float> ci, V3<float> cj, float rs) {
     const V3<float> d = sub(ci, cj);
     const float dist = sqrtf(dot(d, d)) - rs;

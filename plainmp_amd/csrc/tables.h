@@ -66,7 +66,7 @@ struct alignas(16) CullEntry {
   float lb[3];
   int32_t prim;  // index into the kind-sorted primitives
   float ub[3];
-  int32_t pad;
+  int32_t orig;  // index in the scene's own primitive order
 };
 // Per block: its candidates of kind k are cull[at[k], at[k + 1]).
 struct alignas(16) CullRange {
@@ -82,10 +82,11 @@ struct alignas(16) DevSphereItem {
   int32_t node, begin, end, pad;
 };
 
-// A unit of the evaluate kernels: the spheres [s_lo, s_hi) against every
-// primitive and the self pairs [k_lo, k_hi), walking FK to node `last`.
+// A unit of the evaluate kernels: the sphere blocks [b_lo, b_hi) against
+// their candidate primitives and the self pairs [k_lo, k_hi), walking FK to
+// node `last`.
 struct alignas(16) DevEvalUnit {
-  int32_t s_lo, s_hi, k_lo, k_hi;
+  int32_t b_lo, b_hi, k_lo, k_hi;
   int32_t last, pad[3];
 };
 
@@ -129,7 +130,7 @@ struct KinArgs {
   const DevSphereBlock* blocks;
   const DevSphereItem* items;
   int32_t n_items;
-  const DevEvalUnit* units;  // one unit per sphere item, then one per self pair
+  const DevEvalUnit* units;  // one unit per sphere block, then one per self pair
   int32_t n_units;
   const DevEvalUnit* unit_all;  // one unit: every sphere and self pair
   int32_t n_nodes, n_spheres, n_pairs, n_cols;
@@ -152,6 +153,9 @@ struct SceneArgs {
   int32_t vrange[kNumKinds + 1];  // kind k occupies vprims[vrange[k], vrange[k + 1])
   const CullEntry* cull;        // candidates of every sphere block (FP32 validity pass)
   const CullRange* cull_range;  // [n_blocks]
+  const CullEntry* ecull;       // the same for evaluate: boxes grown by the 0.1 m cutoff
+  const CullRange* ecull_range;
+  const int32_t* vorig;         // kind-sorted index -> scene order
 };
 
 }  // namespace pmp
