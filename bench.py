@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="run one device copy of this size (counter calibration)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file (default: none, i.e. tune in this run)")
     return ap.parse_args()
 
 
@@ -57,7 +59,7 @@ def main():
     import torch
 
     import plainmp_amd  # noqa: F401
-    from plainmp_amd import roofline, synth
+    from plainmp_amd import autotune, roofline, synth
     from plainmp_amd.ompl import BoxMotionValidator
     from plainmp_amd.synth import build_constraint as product_cst
 
@@ -81,6 +83,9 @@ def main():
             b = torch.empty_like(a)
             b.copy_(a)
             del a, b
+        # schedule: the autotuner's choice for this device and workload (timed outside the measured region)
+        schedule, tune_report = autotune.tune_edges(mv, cst, s1.data_ptr(), s2.data_ptr(), E, valid.data_ptr(),
+                                                    nchk.data_ptr(), sh, key="edges_mixed16", cache=args.tune_cache)
         graph = mv._dev_capture_graph(s1.data_ptr(), s2.data_ptr(), E, E, valid.data_ptr(), nchk.data_ptr(), sh)
         for _ in range(args.warmup):
             graph.launch()
@@ -140,6 +145,7 @@ def main():
             "edges_rechecked_fp64": n_recheck,
         },
         "edges_per_s": edges_per_s,
+        "schedule": {**schedule.as_dict(), "autotune": tune_report},
         "covered_configs_per_s": n_wp_total / (ms_step * 1e-3),
         "roofline": {
             "bound": "valu",
@@ -162,6 +168,10 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(wl)
         out["speedup_vs_cpu_configs"] = configs_per_s / out["cpu_baseline"]["value"]
+        if not args.no_extra:
+            out["cpu_baseline_evaluate"] = _cpu_evaluate_baseline()
+            out["speedup_vs_cpu_jacobians"] = (out["extra"]["grad_mixed32_evaluate"]["jacobians_per_s"]
+                                               / out["cpu_baseline_evaluate"]["value"])
     print(json.dumps(out), flush=True)
 
 
@@ -198,7 +208,7 @@ def _time_stream(torch, stream, fn, steps=20, warmup=3):
 
 
 def _extra(torch, stream, dev):
-    from plainmp_amd import roofline, synth
+    from plainmp_amd import autotune, roofline, synth
     from plainmp_amd.synth import build_constraint as product_cst
 
     res = {}
@@ -234,8 +244,11 @@ def _extra(torch, stream, dev):
     with torch.cuda.stream(stream):
         vals = torch.empty((C, B), dtype=torch.float32, device=dev)
         jac = torch.empty((C, D, B), dtype=torch.float32, device=dev)
+    _, tune_eval = autotune.tune_evaluate(cst, q.data_ptr(), B, vals.data_ptr(), jac.data_ptr(), stream,
+                                          key="grad_mixed32", cache=None)
     ms = _time_stream(torch, stream, lambda: cst._dev_evaluate(q.data_ptr(), B, B, vals.data_ptr(), jac.data_ptr(), sh))
-    res["grad_mixed32_evaluate"] = {"configs": B, "ms": ms, "jacobians_per_s": B / (ms * 1e-3)}
+    res["grad_mixed32_evaluate"] = {"configs": B, "ms": ms, "jacobians_per_s": B / (ms * 1e-3),
+                                    "autotune": tune_eval}
     # adjoint (VJP) used by the autograd backward
     with torch.cuda.stream(stream):
         gd = torch.ones((S, P, B), dtype=torch.float32, device=dev)
@@ -260,6 +273,21 @@ def _extra(torch, stream, dev):
         x.data_ptr(), N, N, offl, [], vals.data_ptr(), jac.data_ptr(), cost.data_ptr(), grad.data_ptr(), sh))
     res["traj_dual_arm_cost_grad"] = {"trajectories": T, "waypoints": N, "ms": ms,
                                       "waypoints_per_s": N / (ms * 1e-3), "trajectories_per_s": T / (ms * 1e-3)}
+    # cloud_shelves: 16,384 configurations of an 8-DOF arm with 90 spheres against a
+    # 65,536-point cloud (sphere-vs-cloud distances on the matrix cores)
+    wl = synth.workload("cloud_shelves")
+    cst = product_cst(wl)
+    B = len(wl.Q)
+    with torch.cuda.stream(stream):
+        q = _soa(torch, wl.Q)
+        valid = torch.empty(B, dtype=torch.uint8, device=dev)
+    ms = _time_stream(torch, stream, lambda: cst._dev_is_valid(q.data_ptr(), B, B, valid.data_ptr(), 0, sh), steps=10)
+    n_pts = len(wl.cloud)
+    res["cloud_shelves_is_valid"] = {
+        "configs": B, "cloud_points": n_pts, "spheres": cst.n_spheres(), "ms": ms, "configs_per_s": B / (ms * 1e-3),
+        "valid_frac": float(valid.float().mean().item()),
+        "dense_pair_tflops": B * cst.n_spheres() * n_pts * 8 / (ms * 1e-3) / 1e12,
+    }
     return res
 
 
@@ -294,6 +322,29 @@ def _cpu_baseline(wl, budget_s=2.0):
             "edges_per_s": edges / dt,
             "sample": f"{edges} edges of edges_mixed16 ({checked} waypoints checked in reference order), "
                       f"{dt:.1f} s wall on {threads} threads"}
+
+
+def _cpu_evaluate_baseline(budget_s=2.0):
+    """Reference-semantics evaluate (value + Jacobian row) of grad_mixed32
+    configurations by the FP64 oracle on host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from plainmp_amd import synth
+    from support import oracle_cst
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    wl = synth.workload("grad_mixed32")
+    oc = oracle_cst(wl)
+    done = 0
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < budget_s:
+        sl = slice((k * 2048) % len(wl.Q), (k * 2048) % len(wl.Q) + 2048)
+        oc.evaluate_batch(wl.Q[sl], threads)
+        done += len(wl.Q[sl])
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "jacobians/s", "cores": threads, "kind": "port",
+            "sample": f"{done} grad_mixed32 configurations (evaluate: value + Jacobian row), {dt:.1f} s on {threads} threads"}
 
 
 if __name__ == "__main__":

@@ -127,10 +127,9 @@ typedef struct pmp_validator_desc {
 typedef struct pmp_schedule {
   int32_t block;            /* threads per workgroup (multiple of 64), 0 = default  */
   int32_t waves_per_cu;     /* persistent waves per compute unit, 0 = default       */
-  int32_t lanes_per_item;   /* lanes that share one edge: 16, 32 or 64; 0 = default */
   int32_t eval_whole_chunks;  /* evaluate / trajectory batches of at least this many
                                  64-configuration chunks walk each configuration as one
-                                 unit, smaller ones split it per sphere run and self
+                                 unit, smaller ones split it per sphere block and self
                                  pair; 0 = default (512)                               */
 } pmp_schedule;
 

@@ -127,7 +127,7 @@ struct pmp_context {
   hipStream_t capture_stream = nullptr;  // used for graph capture when no stream is set
   hipEvent_t edge_ev0 = nullptr, edge_ev1 = nullptr;  // around the last direct edge kernel
   float band = 1e-5f;
-  pmp_schedule sched{0, 0, 0, 0};
+  pmp_schedule sched{0, 0, 0};
   int n_cu = 256;
   // robot
   bool has_robot = false;
@@ -593,10 +593,9 @@ pmp_status pmp_set_band(pmp_context* ctx, double band) {
 pmp_status pmp_set_schedule(pmp_context* ctx, const pmp_schedule* s) {
   return guarded([&] {
     PMP_REQUIRE(ctx != nullptr && s != nullptr, "null argument");
-    PMP_REQUIRE(s->block == 0 || (s->block % 64 == 0 && s->block <= 1024), "block must be a multiple of 64, <= 1024");
+    PMP_REQUIRE(s->block == 0 || (s->block % 64 == 0 && s->block <= 256), "block must be a multiple of 64, <= 256");
     PMP_REQUIRE(s->waves_per_cu >= 0 && s->waves_per_cu <= 32, "waves_per_cu must be in [0, 32]");
     PMP_REQUIRE(s->eval_whole_chunks >= 0, "eval_whole_chunks must be >= 0");
-    PMP_REQUIRE(s->lanes_per_item == 0 || s->lanes_per_item == 64, "lanes_per_item must be 64");
     ctx->sched = *s;
   });
 }

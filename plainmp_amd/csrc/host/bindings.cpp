@@ -460,8 +460,8 @@ PYBIND11_MODULE(_native, m) {
       .def("set_band",
            [](SphereCollisionCst& c, double band) { check(pmp_set_band(c.device_context(nullptr), band)); },
            "Width (m) of the band around zero clearance whose FP32 decisions are re-made in FP64 (default 1e-5).")
-      .def("_dev_set_schedule", [](SphereCollisionCst& c, int block, int waves_per_cu, int lanes, int whole_chunks) {
-        pmp_schedule s{block, waves_per_cu, lanes, whole_chunks};
+      .def("_dev_set_schedule", [](SphereCollisionCst& c, int block, int waves_per_cu, int whole_chunks) {
+        pmp_schedule s{block, waves_per_cu, whole_chunks};
         check(pmp_set_schedule(c.device_context(nullptr), &s));
       });
 

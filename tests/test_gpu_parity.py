@@ -348,7 +348,7 @@ def test_evaluate_unit_modes(whole_chunks):
     default for these batch sizes) agree with the oracle."""
     wl = synth.workload("traj_dual_arm", scale=0.004)
     oc, pc = oracle_cst(wl), product_cst(wl)
-    pc._dev_set_schedule(0, 0, 0, whole_chunks)
+    pc._dev_set_schedule(0, 0, whole_chunks)
     X, off = wl.traj_X, wl.traj_offsets
     vals, jac, cost, grad = pc.traj_cost_grad_batch(X, [int(v) for v in off])
     vref, jref = oc.evaluate_batch(X, 8)
@@ -362,3 +362,36 @@ def test_evaluate_unit_modes(whole_chunks):
         assert abs(cost[i] - cr) <= 1e-5 * max(1.0, abs(cr))
     v2, _ = pc.evaluate_batch(X[:300])
     assert np.abs(v2 - vref[:300]).max() <= 5e-6
+
+
+def test_autotuned_schedules_keep_results():
+    """Every candidate schedule the autotuner times gives the same edge
+    results, and the chosen one is applied to the context."""
+    import torch
+
+    from plainmp_amd import autotune
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    wl = synth.workload("edges_mixed16", scale=0.125)  # 512 edges
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(wl.box_width), 0.0, 8)
+    mv = BoxMotionValidator(pc, wl.box_width)
+    E = len(wl.S1)
+    s1 = torch.from_numpy(np.ascontiguousarray(wl.S1.T)).cuda()
+    s2 = torch.from_numpy(np.ascontiguousarray(wl.S2.T)).cuda()
+    valid = torch.empty(E, dtype=torch.uint8, device="cuda")
+    nchk = torch.empty(E, dtype=torch.int32, device="cuda")
+    sh = torch.cuda.current_stream().cuda_stream
+    for b in autotune.EDGE_BLOCKS:
+        for w in (4, 16, 32):
+            autotune.Schedule(b, w).apply(pc)
+            mv._dev_check_motions(s1.data_ptr(), s2.data_ptr(), E, E, valid.data_ptr(), nchk.data_ptr(), sh)
+            torch.cuda.synchronize()
+            assert np.array_equal(valid.cpu().numpy().astype(bool), ref.astype(bool)), (b, w)
+            assert np.array_equal(nchk.cpu().numpy(), n_ref), (b, w)
+    sched, rep = autotune.tune_edges(mv, pc, s1.data_ptr(), s2.data_ptr(), E, valid.data_ptr(), nchk.data_ptr(), sh,
+                                     repeats=2, cache=None)
+    _report("autotune", **rep)
+    assert sched.block in autotune.EDGE_BLOCKS and sched.waves_per_cu in autotune.EDGE_WAVES_PER_CU
+    got, n_got = mv.check_motions(wl.S1, wl.S2)
+    assert np.array_equal(got, ref.astype(bool)) and np.array_equal(n_got, n_ref)
