@@ -141,7 +141,7 @@ struct pmp_context {
   std::vector<int> h_block_node;
   std::vector<NodeReach> h_reach;
   // scene
-  DevBuf prims32, prims64, vprims32, vprims64, tiles32, tiles64, tileB, pts64;
+  DevBuf prims32, prims64, vprims32, vprims64, supers, tiles32, tiles64, tileB, pts64;
   SceneArgs<float> s32{};
   SceneArgs<double> s64{};
   int n_clouds = 0;
@@ -491,6 +491,26 @@ void tile_cloud(const pmp_prim_desc& d, std::vector<CloudTile>& t32, std::vector
   }
 }
 
+// Super-tiles over the tiles [t0, t32.size()) of one cloud.
+void super_tiles(const std::vector<CloudTile>& t32, int t0, std::vector<CloudSuper>& sup) {
+  const int nt = (int)t32.size();
+  for (int a = t0; a < nt; a += kCloudSuper) {
+    CloudSuper S{};
+    for (int k = 0; k < 3; ++k) {
+      S.lb[k] = INFINITY;
+      S.ub[k] = -INFINITY;
+    }
+    S.first = a;
+    S.count = std::min(kCloudSuper, nt - a);
+    for (int t = a; t < a + S.count; ++t)
+      for (int k = 0; k < 3; ++k) {
+        S.lb[k] = std::min(S.lb[k], t32[t].lb[k]);
+        S.ub[k] = std::max(S.ub[k], t32[t].ub[k]);
+      }
+    sup.push_back(S);
+  }
+}
+
 void require_robot(pmp_context* ctx) {
   PMP_REQUIRE(ctx != nullptr, "null context");
   PMP_REQUIRE(ctx->has_robot, "no robot tables uploaded (call pmp_upload_robot first)");
@@ -551,7 +571,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
     for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->units, &ctx->unit_all, &ctx->ws_vjp, &ctx->ws_eval,
-                      &ctx->vprims32, &ctx->vprims64, &ctx->cull, &ctx->ecull, &ctx->ecull_range, &ctx->vorig, &ctx->cull_range, &ctx->rad_out,
+                      &ctx->vprims32, &ctx->vprims64, &ctx->supers, &ctx->cull, &ctx->ecull, &ctx->ecull_range, &ctx->vorig, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
                       &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2})
@@ -906,16 +926,19 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     std::vector<float> tb;
     std::vector<double> pts;
     std::vector<CloudRange> clouds;
+    std::vector<CloudSuper> sup;
     for (int i = 0; i < sc->n_prims; ++i) {
       if (sc->prims[i].type != PMP_PRIM_CLOUD) continue;
       const int begin = (int)t32.size(), oc = (int)clouds.size();
       tile_cloud(sc->prims[i], t32, t64, tb, pts);
       const int nt = (int)t32.size() - begin;
+      const int sbegin = (int)sup.size();
+      super_tiles(t32, begin, sup);
       p32[i].cloud_begin = p64[i].cloud_begin = begin;
       p32[i].cloud_count = p64[i].cloud_count = nt;
       p32[i].flags |= oc << 4;
       p64[i].flags |= oc << 4;
-      clouds.push_back({begin, nt, (float)sc->prims[i].size[0]});
+      clouds.push_back({begin, nt, (float)sc->prims[i].size[0], sbegin, (int)sup.size() - sbegin});
     }
     // kind-sorted copy for the validity pass (its answer does not depend on
     // the primitive order)
@@ -944,6 +967,7 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     for (int k = 0; k <= kNumKinds; ++k) a.vrange[k] = d.vrange[k] = vrange[k];
     d.prims = ctx->prims64.upload(p64, s0);
     a.tiles = d.tiles = ctx->tiles32.upload(t32, s0);
+    a.supers = d.supers = ctx->supers.upload(sup, s0);
     a.tiles64 = d.tiles64 = ctx->tiles64.upload(t64, s0);
     a.tileB = d.tileB = ctx->tileB.upload(tb, s0);
     a.pts64 = d.pts64 = ctx->pts64.upload(pts, s0);
@@ -1010,8 +1034,7 @@ pmp_status pmp_sphere_clearance_batch(pmp_context* ctx, const double* q, int64_t
     PMP_HIP(launch_clearance(ctx->k32, ctx->s32, q, ldq, B, accurate ? 0 : 1, clearance, cen, st(ctx)));
     for (const CloudRange& c : ctx->clouds) {  // nearest point of every cloud, no cap
       float* d2 = (float*)ctx->ws_cloud.get(sizeof(float) * Q);
-      PMP_HIP(launch_cloud_min(ctx->s32.tiles, ctx->s32.tileB, c.tile_begin, c.n_tiles, cen, nullptr, B, Q, d2,
-                               st(ctx)));
+      PMP_HIP(launch_cloud_min(ctx->s32.tiles, ctx->s32.tileB, ctx->s32.supers, c, cen, nullptr, B, Q, d2, st(ctx)));
       PMP_HIP(launch_cloud_clearance(clearance, d2, (const float*)ctx->rad_out.ptr, c.radius, B, S, st(ctx)));
     }
   });

@@ -19,14 +19,16 @@ namespace pmp {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// min |c - p|^2 over the tiles [tile_begin, tile_begin + n_tiles) for queries
-// q < Q whose cap2[q] >= 0; centres in cen[(q / B * 3 + k) * B + q % B].
-// Writes out_d2[q] = min squared distance (FLT_MAX when no point lies within
-// the cap).  `skip` (optional, per configuration q % B): queries of
-// configurations already decided (0) are not evaluated.
-__global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, const float* tileB, int tile_begin,
-                                                        int n_tiles, const float* cen, const float* cap2, int64_t B,
-                                                        int64_t Q, const uint8_t* skip, float* out_d2) {
+// min |c - p|^2 over the cloud's tiles for queries q < Q whose cap2[q] >= 0;
+// centres in cen[(q / B * 3 + k) * B + q % B].  Writes out_d2[q] = min squared
+// distance (FLT_MAX when no point lies within the cap).  `skip` (optional,
+// per configuration q % B): queries of configurations already decided (0)
+// are not evaluated.  Super-tiles are tested first, then their tiles; a
+// (super-)tile is skipped when no query's cap reaches its AABB.
+__global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, const float* tileB,
+                                                        const CloudSuper* supers, int super_begin, int n_super,
+                                                        const float* cen, const float* cap2, int64_t B, int64_t Q,
+                                                        const uint8_t* skip, float* out_d2) {
   const int lane = threadIdx.x & 63;
   const int64_t q0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 16;
   if (q0 >= Q) return;  // wave-uniform
@@ -57,25 +59,34 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   }
   float best[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
-  for (int t = 0; t < n_tiles; ++t) {
-    const CloudTile T = cload(tiles, tile_begin + t);
-    const float dx = fmaxf(fmaxf(T.lb[0] - cx, cx - T.ub[0]), 0.f);
-    const float dy = fmaxf(fmaxf(T.lb[1] - cy, cy - T.ub[1]), 0.f);
-    const float dz = fmaxf(fmaxf(T.lb[2] - cz, cz - T.ub[2]), 0.f);
-    const bool need = dx * dx + dy * dy + dz * dz <= capi;
-    if (!__any(need)) continue;
-    const float a = k == 0 ? cx - T.o[0] : (k == 1 ? cy - T.o[1] : (k == 2 ? cz - T.o[2] : 1.f));
-    const float* Bt = tileB + (int64_t)(tile_begin + t) * 256;
-    floatx4 d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[lane], zero, 0, 0, 0);
-    floatx4 d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[64 + lane], zero, 0, 0, 0);
-    floatx4 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[128 + lane], zero, 0, 0, 0);
-    floatx4 d3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[192 + lane], zero, 0, 0, 0);
+  if (__any(capi >= 0.f)) {
+    for (int si = 0; si < n_super; ++si) {
+      const CloudSuper S = cload(supers, super_begin + si);
+      const float sx = fmaxf(fmaxf(S.lb[0] - cx, cx - S.ub[0]), 0.f);
+      const float sy = fmaxf(fmaxf(S.lb[1] - cy, cy - S.ub[1]), 0.f);
+      const float sz = fmaxf(fmaxf(S.lb[2] - cz, cz - S.ub[2]), 0.f);
+      if (!__any(sx * sx + sy * sy + sz * sz <= capi)) continue;
+      for (int t = S.first; t < S.first + S.count; ++t) {
+        const CloudTile T = cload(tiles, t);
+        const float dx = fmaxf(fmaxf(T.lb[0] - cx, cx - T.ub[0]), 0.f);
+        const float dy = fmaxf(fmaxf(T.lb[1] - cy, cy - T.ub[1]), 0.f);
+        const float dz = fmaxf(fmaxf(T.lb[2] - cz, cz - T.ub[2]), 0.f);
+        const bool need = dx * dx + dy * dy + dz * dz <= capi;
+        if (!__any(need)) continue;
+        const float a = k == 0 ? cx - T.o[0] : (k == 1 ? cy - T.o[1] : (k == 2 ? cz - T.o[2] : 1.f));
+        const float* Bt = tileB + (int64_t)t * 256;
+        floatx4 d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[lane], zero, 0, 0, 0);
+        floatx4 d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[64 + lane], zero, 0, 0, 0);
+        floatx4 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[128 + lane], zero, 0, 0, 0);
+        floatx4 d3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[192 + lane], zero, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const float ox = rx[r] - T.o[0], oy = ry[r] - T.o[1], oz = rz[r] - T.o[2];
-      const float off = ox * ox + oy * oy + oz * oz;
-      const float m = fminf(fminf(d0[r], d1[r]), fminf(d2[r], d3[r]));
-      best[r] = fminf(best[r], m + off);
+        for (int r = 0; r < 4; ++r) {
+          const float ox = rx[r] - T.o[0], oy = ry[r] - T.o[1], oz = rz[r] - T.o[2];
+          const float off = ox * ox + oy * oy + oz * oz;
+          const float m = fminf(fminf(d0[r], d1[r]), fminf(d2[r], d3[r]));
+          best[r] = fminf(best[r], m + off);
+        }
+      }
     }
   }
   // minimum over the 16 lanes that share the rows (columns j = lane & 15)
@@ -146,24 +157,24 @@ hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float
 
 // The FP32 validity pass has filled cen / cap2 / rr; d2 is written after rr
 // (the workspace holds Q floats there).
-hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
-                              const float* cen, const float* cap2, const float* rr, int64_t B, int64_t Q, float band,
-                              uint8_t* valid, int32_t* flags, hipStream_t st) {
+hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
+                              const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
+                              int64_t Q, float band, uint8_t* valid, int32_t* flags, hipStream_t st) {
   auto blocks = [](int64_t n, int t) { return (unsigned)((n + t - 1) / t); };
   float* d2 = const_cast<float*>(rr) + Q;
   hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, flags, 0, B);
-  hipLaunchKernelGGL(cloud_min_kernel, dim3(blocks((Q + 15) / 16 * 64, 256)), dim3(256), 0, st, tiles, tileB,
-                     tile_begin, n_tiles, cen, cap2, B, Q, valid, d2);
+  hipLaunchKernelGGL(cloud_min_kernel, dim3(blocks((Q + 15) / 16 * 64, 256)), dim3(256), 0, st, tiles, tileB, supers,
+                     cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, d2);
   hipLaunchKernelGGL(cloud_decide_kernel, dim3(blocks(Q, 256)), dim3(256), 0, st, d2, rr, B, Q, band, flags);
   hipLaunchKernelGGL(cloud_merge_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, valid, flags, B);
   return hipGetLastError();
 }
 
-hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
-                            const float* cen, const float* cap2, int64_t B, int64_t Q, float* out_d2,
-                            hipStream_t st) {
+hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
+                            const CloudRange& cr, const float* cen, const float* cap2, int64_t B, int64_t Q,
+                            float* out_d2, hipStream_t st) {
   hipLaunchKernelGGL(cloud_min_kernel, dim3((unsigned)(((Q + 15) / 16 * 64 + 255) / 256)), dim3(256), 0, st, tiles,
-                     tileB, tile_begin, n_tiles, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2);
+                     tileB, supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2);
   return hipGetLastError();
 }
 

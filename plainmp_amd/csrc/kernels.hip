@@ -1115,8 +1115,8 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
   // sphere-vs-cloud queries on the matrix cores, one pass per cloud
   const int64_t Q = (int64_t)k32.n_ext_spheres * B;
   for (int oc = 0; oc < n_clouds && Q > 0; ++oc) {
-    hipError_t e = launch_cloud_valid(s32.tiles, s32.tileB, clouds[oc].tile_begin, clouds[oc].n_tiles, cw.cen,
-                                      cw.cap2[oc], cw.rr[oc], B, Q, band, valid, flags, st);
+    hipError_t e = launch_cloud_valid(s32.tiles, s32.tileB, s32.supers, clouds[oc], cw.cen, cw.cap2[oc], cw.rr[oc],
+                                      B, Q, band, valid, flags, st);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(valid_recheck_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, k64, s64, q, ldq, B, valid,

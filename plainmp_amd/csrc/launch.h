@@ -30,6 +30,7 @@ hipError_t launch_poses(const KinArgs<double>& ka, const double* q, int64_t ldq,
 struct CloudRange {
   int32_t tile_begin, n_tiles;
   float radius;
+  int32_t super_begin, n_super;
 };
 
 hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
@@ -39,12 +40,12 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                         uint8_t* valid, int32_t* n_recheck, int block, const CloudWork& cw,
                         const CloudRange* clouds, int n_clouds, int32_t* flags, hipStream_t st);
 // cloud.hip
-hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
-                              const float* cen, const float* cap2, const float* rr, int64_t B, int64_t Q, float band,
-                              uint8_t* valid, int32_t* flags, hipStream_t st);
-hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, int tile_begin, int n_tiles,
-                            const float* cen, const float* cap2, int64_t B, int64_t Q, float* out_d2,
-                            hipStream_t st);
+hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
+                              const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
+                              int64_t Q, float band, uint8_t* valid, int32_t* flags, hipStream_t st);
+hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
+                            const CloudRange& cr, const float* cen, const float* cap2, int64_t B, int64_t Q,
+                            float* out_d2, hipStream_t st);
 hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float* radius_out, float rc, int64_t B,
                                   int64_t S, hipStream_t st);
 // ev_before / ev_after (optional) are recorded around the FP32 edge kernel.

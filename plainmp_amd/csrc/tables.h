@@ -106,6 +106,13 @@ struct alignas(16) CloudTile {
 struct alignas(16) CloudTile64 {
   double lb[3], ub[3];
 };
+// kCloudSuper consecutive (Morton-ordered) tiles under one FP32 AABB: a wave
+// tests a super-tile once before its tiles.
+constexpr int kCloudSuper = 16;
+struct alignas(16) CloudSuper {
+  float lb[3], ub[3];
+  int32_t first, count;  // tile range [first, first + count)
+};
 
 // Where the FP32 validity pass leaves sphere-vs-cloud queries.
 struct CloudWork {
@@ -146,6 +153,7 @@ struct SceneArgs {
   int32_t n_prims;
   int32_t n_cloud;              // number of cloud primitives
   const CloudTile* tiles;       // all clouds' tiles
+  const CloudSuper* supers;     // all clouds' super-tiles
   const CloudTile64* tiles64;
   const float* tileB;           // [tile][4 MFMA][4 k][16 j] B operands in tile-local coordinates
   const double* pts64;          // [tile][64][4] FP64 world points (padding points far away)
