@@ -77,6 +77,16 @@ struct DevBuf {
   }
 };
 
+// Device buffers of one edge launch setup: the direct calls of a context
+// share one set; every captured graph owns its own, so that later calls
+// cannot change what a replay reads.
+struct EdgeBufs {
+  DevBuf width, inv_width, pend, counter;
+  void release() {
+    for (DevBuf* b : {&width, &inv_width, &pend, &counter}) b->release();
+  }
+};
+
 // Farthest-first visiting order of the motion validators
 // (ompl/generate_sequence.py:11-63): element i+1 maximises the minimum
 // distance to the elements already chosen, ties to the larger value.
@@ -154,13 +164,15 @@ struct pmp_context {
   DevBuf traj_off, traj_w2;
   std::vector<int64_t> h_traj_off;
   std::vector<double> h_traj_w2;
-  DevBuf seq, counter, width, inv_width, tmp_prims, tmp_cloud, ws_cen, ws_cloud, ws_flags;
+  EdgeBufs eb;  // edge buffers of direct calls
+  DevBuf seq, tmp_prims, tmp_cloud, ws_cen, ws_cloud, ws_flags;
 };
 
 struct pmp_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
+  EdgeBufs eb;
 };
 
 namespace {
@@ -554,7 +566,6 @@ pmp_status pmp_create(int32_t device, pmp_context** out) {
       PMP_HIP(hipGetDeviceProperties(&prop, device));
       ctx->n_cu = prop.multiProcessorCount;
       ctx->seq.upload(sequence_table(), nullptr);
-      ctx->counter.get(64);
       PMP_HIP(hipDeviceSynchronize());
     } catch (...) {
       delete ctx;
@@ -573,9 +584,10 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->units, &ctx->unit_all, &ctx->ws_vjp, &ctx->ws_eval,
                       &ctx->vprims32, &ctx->vprims64, &ctx->supers, &ctx->cull, &ctx->ecull, &ctx->ecull_range, &ctx->vorig, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
-                      &ctx->counter, &ctx->width, &ctx->inv_width, &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
+                      &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
                       &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2})
       b->release();
+    ctx->eb.release();
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     if (ctx->edge_ev0) (void)hipEventDestroy(ctx->edge_ev0);
     if (ctx->edge_ev1) (void)hipEventDestroy(ctx->edge_ev1);
@@ -1150,7 +1162,7 @@ pmp_status pmp_pair_clearance_vjp(pmp_context* ctx, const double* q, int64_t ldq
 namespace {
 
 void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
-                   const pmp_validator_desc* v, uint8_t* valid, int32_t* n_checked, EdgeArgs& ea) {
+                   const pmp_validator_desc* v, uint8_t* valid, int32_t* n_checked, EdgeBufs& eb, EdgeArgs& ea) {
   require_robot(ctx);
   PMP_REQUIRE(E >= 0, "negative edge count");
   PMP_REQUIRE(E == 0 || (s1 && s2 && valid), "null argument");
@@ -1176,13 +1188,16 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
   ea.E = E;
   ea.vtype = v->type;
   ea.resolution = v->resolution;
-  ea.width = ctx->width.upload(w, st(ctx));
-  ea.inv_width = ctx->inv_width.upload(iw, st(ctx));
+  ea.width = eb.width.upload(w, st(ctx));
+  ea.inv_width = eb.inv_width.upload(iw, st(ctx));
   ea.seq = (const uint8_t*)ctx->seq.ptr;
-  ea.counter = (int32_t*)ctx->counter.ptr;
+  ea.counter = (int32_t*)eb.counter.get(64);
+  int32_t* pend = (int32_t*)eb.pend.get(sizeof(int32_t) * std::max<int64_t>(E, 1) * (kPendStates + 2));
+  ea.pend = pend;
+  ea.pend_info = pend + std::max<int64_t>(E, 1) * kPendStates;
   ea.valid = valid;
   ea.n_checked = n_checked;
-  ea.n_recheck = ((int32_t*)ctx->counter.ptr) + 1;
+  ea.n_recheck = ea.counter + 1;
 }
 
 // Persistent grid: waves_per_cu waves on every CU (default 16, four per
@@ -1201,7 +1216,7 @@ pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const doubl
                                  const pmp_validator_desc* v, uint8_t* valid, int32_t* n_checked) {
   return guarded([&] {
     EdgeArgs ea{};
-    prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ea);
+    prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ctx->eb, ea);
     if (E == 0) return;
     const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
     if (!ctx->edge_ev0) {
@@ -1217,19 +1232,19 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
                                  const pmp_validator_desc* v, uint8_t* valid, int32_t* n_checked, pmp_graph** out) {
   return guarded([&] {
     PMP_REQUIRE(out != nullptr, "null output pointer");
-    EdgeArgs ea{};
-    prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ea);
     PMP_REQUIRE(E > 0, "graph of an empty edge batch");
-    PMP_HIP(hipStreamSynchronize(st(ctx)));  // width tables are in place before capture
-    hipStream_t cs = ctx->stream;
-    if (!cs) {
-      if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
-      cs = ctx->capture_stream;
-    }
-    const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
     auto* g = new pmp_graph();
-    g->stream = cs;
     try {
+      EdgeArgs ea{};
+      prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, g->eb, ea);
+      PMP_HIP(hipStreamSynchronize(st(ctx)));  // width tables are in place before capture
+      hipStream_t cs = ctx->stream;
+      if (!cs) {
+        if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        cs = ctx->capture_stream;
+      }
+      const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
+      g->stream = cs;
       PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       hipError_t le = launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block, E),
                                    block, cs);
@@ -1240,6 +1255,7 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
     } catch (...) {
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
       if (g->graph) (void)hipGraphDestroy(g->graph);
+      g->eb.release();
       delete g;
       throw;
     }
@@ -1260,6 +1276,7 @@ pmp_status pmp_graph_destroy(pmp_graph* g) {
     (void)hipStreamSynchronize(g->stream);
     if (g->exec) PMP_HIP(hipGraphExecDestroy(g->exec));
     if (g->graph) PMP_HIP(hipGraphDestroy(g->graph));
+    g->eb.release();
     delete g;
   });
 }
@@ -1267,9 +1284,10 @@ pmp_status pmp_graph_destroy(pmp_graph* g) {
 pmp_status pmp_edge_recheck_count(pmp_context* ctx, int32_t* n) {
   return guarded([&] {
     PMP_REQUIRE(ctx != nullptr && n != nullptr, "null argument");
-    PMP_REQUIRE(ctx->counter.ptr != nullptr, "no edge batch has run on this context");
+    PMP_REQUIRE(ctx->eb.counter.ptr != nullptr, "no edge batch has run on this context");
     set_device(ctx);
-    PMP_HIP(hipMemcpyAsync(n, (const int32_t*)ctx->counter.ptr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st(ctx)));
+    PMP_HIP(hipMemcpyAsync(n, (const int32_t*)ctx->eb.counter.ptr + 1, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           st(ctx)));
     PMP_HIP(hipStreamSynchronize(st(ctx)));
   });
 }

@@ -449,8 +449,9 @@ __device__ __forceinline__ int wave_min(int v) {
 // the states the reference's checkMotion evaluates, i.e. the visiting-order
 // index of the first invalid state plus one, or every state of a valid edge
 // (0 for an edge whose step ratio is infinite, -1 for an edge that would need
-// more than kMaxEdgeStates states).  A count that depends on an in-band state
-// before the first certain collision is left to the FP64 pass as well.
+// more than kMaxEdgeStates states).  A pending edge queues its in-band states
+// that can matter (all of them, or those before the first certain collision)
+// and that collision's index; the FP64 pass decides from them.
 __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, EdgeArgs ea,
                                                     float band) {
   const int lane = threadIdx.x & 63;
@@ -464,6 +465,8 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
     const double ratio = edge_step_ratio(ea, k32.n_cols, e);
     int state = 1;
     int64_t checked = 0;
+    int n_pend = 0;        // queued in-band states, -1 once more than kPendStates
+    int kdef = INT_MAX;    // first certain collision
     if (ratio != INFINITY) {
       if (!(ratio > 0.0) || 1.0 / ratio > kMaxEdgeStates) {  // NaN states, or more states than an edge may hold
         state = 0;
@@ -471,7 +474,7 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
       } else {
         const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
         const int64_t n_wp = n_test - 1;  // s2 plus the interior waypoints
-        bool unc = false;                 // an in-band state seen so far
+        bool unc = false;                 // an in-band state that matters seen so far
         for (int64_t base = 0; base < n_wp; base += 64) {
           const int64_t k = base + lane;
           const bool has = k < n_wp;
@@ -479,14 +482,24 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
           v.idle = !has;
           run_fk(k32, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), taylor, v);
           const bool def = has && v.definite;
-          if (__any(def)) {
-            const int kmin = wave_min(def ? (int)k : INT_MAX);
-            const bool before = has && v.uncertain && (int)k < kmin;
-            state = (unc || __any(before)) ? 2 : 0;
-            checked = kmin + 1;
+          if (__any(def)) kdef = wave_min(def ? (int)k : INT_MAX);
+          const bool in_band = has && v.uncertain && !v.definite && (int)k < kdef;
+          const uint64_t m = __ballot(in_band);
+          if (m) {
+            unc = true;
+            const int cnt = __popcll(m);
+            if (n_pend >= 0 && n_pend + cnt <= kPendStates) {
+              if (in_band) ea.pend[(int64_t)e * kPendStates + n_pend + __popcll(m & ((1ull << lane) - 1))] = (int)k;
+              n_pend += cnt;
+            } else {
+              n_pend = -1;
+            }
+          }
+          if (kdef != INT_MAX) {
+            state = unc ? 2 : 0;
+            checked = kdef + 1;
             break;
           }
-          unc |= __any(has && v.uncertain);
           checked = base + 64 < n_wp ? base + 64 : n_wp;
         }
         if (state == 1 && unc) state = 2;
@@ -495,28 +508,52 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
     if (lane == 0) {
       ea.valid[e] = (uint8_t)state;
       if (ea.n_checked) ea.n_checked[e] = (int32_t)checked;
+      if (state == 2) {
+        ea.pend_info[2 * (int64_t)e] = n_pend;
+        ea.pend_info[2 * (int64_t)e + 1] = kdef;
+      }
     }
   }
 }
 
-// FP64 walk of the edges left pending by edges_kernel: one workgroup of
-// kRecheckWaves waves per edge.  A round evaluates up to kRecheckWaves chunks
-// of 64 states at once, each chunk's primitive candidates shared among
-// kRecheckWaves / chunks waves; the first colliding state in visiting order is
-// the minimum colliding index over the round, and a round that finds one
-// ends the walk.
+// FP64 decision of the edges left pending by edges_kernel, one workgroup of
+// kRecheckWaves waves per edge.  With its in-band states queued, one wave
+// re-decides just those (a lane each); the first colliding state is then the
+// smaller of the FP64 hits and the FP32 pass's first certain collision.  An
+// edge that queued too many walks its states again: a round evaluates up to
+// kRecheckWaves chunks of 64 states at once, each chunk's primitive
+// candidates shared among kRecheckWaves / chunks waves, and a round that
+// finds a collision ends the walk.
 constexpr int kRecheckWaves = 4;
 __global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinArgs<double> k64,
                                                                            SceneArgs<double> s64, EdgeArgs ea) {
   const int64_t e = blockIdx.x;
   if (e >= ea.E || ea.valid[e] != 2) return;  // uniform over the workgroup
-  __shared__ int first;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) first = INT_MAX;
-  __syncthreads();
   const double ratio = edge_step_ratio(ea, k64.n_cols, e);
   const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
   const int64_t n_wp = n_test - 1;
+  const int n_pend = ea.pend_info[2 * e];
+  if (n_pend >= 0) {  // uniform over the workgroup: no barrier on this path
+    if (w != 0) return;
+    const bool has = lane < n_pend;
+    const int64_t k = has ? ea.pend[e * kPendStates + lane] : 0;
+    ValidVis64 v{k64, s64};
+    v.collided = !has;
+    run_fk(k64, edge_waypoint(ea, e, n_test, ratio, k), k64.base_type == PMP_BASE_FIXED, v);
+    const bool hit = has && v.collided;
+    const int kdef = ea.pend_info[2 * e + 1];
+    const int first = min(kdef, __any(hit) ? wave_min(hit ? (int)k : INT_MAX) : INT_MAX);
+    if (lane == 0) {
+      ea.valid[e] = first == INT_MAX ? 1 : 0;
+      if (ea.n_checked) ea.n_checked[e] = (int32_t)(first == INT_MAX ? n_wp : (int64_t)first + 1);
+      if (ea.n_recheck) atomicAdd(ea.n_recheck, 1);
+    }
+    return;
+  }
+  __shared__ int first;
+  if (threadIdx.x == 0) first = INT_MAX;
+  __syncthreads();
   const int64_t n_chunks = (n_wp + 63) / 64;
   const int per_round = n_chunks < kRecheckWaves ? (int)n_chunks : kRecheckWaves;
   const int n_split = kRecheckWaves / per_round;

@@ -8,6 +8,8 @@
 
 namespace pmp {
 
+constexpr int kPendStates = 16;  // in-band states an edge may queue for the FP64 pass
+
 struct EdgeArgs {
   const double* s1;
   const double* s2;
@@ -18,6 +20,8 @@ struct EdgeArgs {
   const double* inv_width;  // device [n_cols], 1 / width as the reference precomputes
   const uint8_t* seq;       // device [128 * 128] farthest-first sequence table
   int32_t* counter;         // device work counter, zeroed before each launch
+  int32_t* pend;            // [E][kPendStates] in-band visiting indices of pending edges
+  int32_t* pend_info;       // [E][2] number queued (-1: more than kPendStates), first certain collision
   uint8_t* valid;
   int32_t* n_checked;
   int32_t* n_recheck;
