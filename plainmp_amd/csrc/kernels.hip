@@ -687,7 +687,20 @@ struct EvalUnitVis {
                                             const DevSphereMeta (&m)[kBlockSpheres], V3<float> g,
                                             const CullRange& cr, bool all) {
     const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
-    auto pair_tests = [&](int p, int orig) {
+    const int lo = all ? sc.vrange[K] : cr.at[K], hi = all ? sc.vrange[K + 1] : cr.at[K + 1];
+    for (int i = lo; i < hi; ++i) {
+      int p, orig;
+      if (all) {
+        p = i;
+        orig = cload(sc.vorig, i);
+      } else {
+        const CullEntry ce = cload(sc.ecull, i);
+        const bool out = g.x < ce.lb[0] || g.x > ce.ub[0] || g.y < ce.lb[1] || g.y > ce.ub[1] ||
+                         g.z < ce.lb[2] || g.z > ce.ub[2];
+        if (!wave_any(!out)) continue;
+        p = ce.prim;
+        orig = ce.orig;
+      }
       const DevPrim<float> P = cload(vp, p);
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
@@ -702,21 +715,6 @@ struct EvalUnitVis {
             bidx = bk.sph[k] * kMaxPrims + orig;
           }
         }
-      }
-    };
-    if (all) {
-      for (int p = sc.vrange[K]; p < sc.vrange[K + 1]; ++p) pair_tests(p, cload(sc.vorig, p));
-      return;
-    }
-    for (int i = cr.at[K]; i < cr.at[K + 1]; i += kCullUnroll) {
-      CullEntry ce[kCullUnroll];
-#pragma unroll
-      for (int u = 0; u < kCullUnroll; ++u) ce[u] = cload(sc.ecull, i + u);
-#pragma unroll
-      for (int u = 0; u < kCullUnroll; ++u) {
-        const bool out = g.x < ce[u].lb[0] || g.x > ce[u].ub[0] || g.y < ce[u].lb[1] || g.y > ce[u].ub[1] ||
-                         g.z < ce[u].lb[2] || g.z > ce[u].ub[2];
-        if (wave_any(!out)) pair_tests(ce[u].prim, ce[u].orig);
       }
     }
   }
