@@ -326,7 +326,6 @@ void build_cull(pmp_context* ctx, double margin, std::vector<CullEntry>& ent, st
         ce.orig = ctx->h_vorig[p];
         ent.push_back(ce);
       }
-      while (((int)ent.size() - rng[b].at[k]) % kCullUnroll != 0) ent.push_back(ent.back());
     }
     rng[b].at[kNumKinds] = (int)ent.size();
   }

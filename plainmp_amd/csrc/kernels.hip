@@ -162,24 +162,19 @@ struct ValidVis32 {
   __device__ __forceinline__ void kind_loop(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
                                             V3<float> g, const CullRange& cr) {
     const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
-    for (int i = cr.at[K]; i < cr.at[K + 1]; i += kCullUnroll) {
-      CullEntry ce[kCullUnroll];  // issued together: one scalar-load latency per kCullUnroll candidates
+    for (int i = cr.at[K]; i < cr.at[K + 1]; ++i) {
+      const CullEntry ce = cload(sc.cull, i);
+      const bool out = g.x < ce.lb[0] || g.x > ce.ub[0] || g.y < ce.lb[1] || g.y > ce.ub[1] || g.z < ce.lb[2] ||
+                       g.z > ce.ub[2];
+      if (!wave_any(alive() && !out)) continue;
+      const DevPrim<float> P = cload(vp, ce.prim);
 #pragma unroll
-      for (int u = 0; u < kCullUnroll; ++u) ce[u] = cload(sc.cull, i + u);
-#pragma unroll
-      for (int u = 0; u < kCullUnroll; ++u) {
-        const bool out = g.x < ce[u].lb[0] || g.x > ce[u].ub[0] || g.y < ce[u].lb[1] || g.y > ce[u].ub[1] ||
-                         g.z < ce[u].lb[2] || g.z > ce[u].ub[2];
-        if (!wave_any(alive() && !out)) continue;
-        const DevPrim<float> P = cload(vp, ce[u].prim);
-#pragma unroll
-        for (int k = 0; k < kBlockSpheres; ++k) {
-          if (k < n) {
-            bool inb = false;
-            const bool col = collide_kind<K>(P, c[k], bk.s[k][3], band, inb);
-            definite |= col && !inb;
-            uncertain |= inb;
-          }
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        if (k < n) {
+          bool inb = false;
+          const bool col = collide_kind<K>(P, c[k], bk.s[k][3], band, inb);
+          definite |= col && !inb;
+          uncertain |= inb;
         }
       }
     }

@@ -68,11 +68,7 @@ struct alignas(16) CullEntry {
   float ub[3];
   int32_t orig;  // index in the scene's own primitive order
 };
-// Per block: its candidates of kind k are cull[at[k], at[k + 1]); every
-// non-empty run is padded to a multiple of kCullUnroll with copies of its
-// last entry (a repeated candidate changes neither an any-collision nor a
-// minimum), so kernels read the entries kCullUnroll at a time.
-constexpr int kCullUnroll = 4;
+// Per block: its candidates of kind k are cull[at[k], at[k + 1]).
 struct alignas(16) CullRange {
   int32_t at[8];
 };
