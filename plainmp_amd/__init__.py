@@ -25,7 +25,7 @@ try:
     from . import _native  # noqa: E402
 except ImportError as e:  # pragma: no cover
     raise ImportError(
-        "plainmp_amd native extension is not built; run `python -m plainmp_amd.build` "
+        "plainmp_amd native extension is not built; run `python plainmp_amd/build.py` "
         f"(or __graft_entry__.build()) first: {e}"
     ) from e
 

@@ -5,7 +5,8 @@
 * ``_native*.so``: the host-side mirror of plainmp's classes with its
   Python bindings, linked against ``libplainmp_amd.so``.
 
-``python -m plainmp_amd.build`` rebuilds both.
+``python plainmp_amd/build.py`` rebuilds both (run as a file: importing the
+package needs the extension).
 """
 
 from __future__ import annotations
