@@ -1191,13 +1191,18 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
   ea.width = eb.width.upload(w, st(ctx));
   ea.inv_width = eb.inv_width.upload(iw, st(ctx));
   ea.seq = (const uint8_t*)ctx->seq.ptr;
-  ea.counter = (int32_t*)eb.counter.get(64);
-  int32_t* pend = (int32_t*)eb.pend.get(sizeof(int32_t) * std::max<int64_t>(E, 1) * (kPendStates + 2));
+  // The kernels leave the counters at zero after every launch; a new buffer
+  // starts there.
+  const bool fresh = eb.counter.ptr == nullptr;
+  ea.counter = (int32_t*)eb.counter.get(sizeof(int32_t) * kNumEdgeCounters);
+  if (fresh) PMP_HIP(launch_fill_i32(ea.counter, 0, kNumEdgeCounters, st(ctx)));
+  const int64_t Ea = std::max<int64_t>(E, 1);
+  int32_t* pend = (int32_t*)eb.pend.get(sizeof(int32_t) * Ea * (kPendStates + 3));
   ea.pend = pend;
-  ea.pend_info = pend + std::max<int64_t>(E, 1) * kPendStates;
+  ea.pend_info = pend + Ea * kPendStates;
+  ea.plist = pend + Ea * (kPendStates + 2);
   ea.valid = valid;
   ea.n_checked = n_checked;
-  ea.n_recheck = ea.counter + 1;
 }
 
 // Persistent grid: waves_per_cu waves on every CU (default 16, four per
@@ -1286,7 +1291,7 @@ pmp_status pmp_edge_recheck_count(pmp_context* ctx, int32_t* n) {
     PMP_REQUIRE(ctx != nullptr && n != nullptr, "null argument");
     PMP_REQUIRE(ctx->eb.counter.ptr != nullptr, "no edge batch has run on this context");
     set_device(ctx);
-    PMP_HIP(hipMemcpyAsync(n, (const int32_t*)ctx->eb.counter.ptr + 1, sizeof(int32_t), hipMemcpyDeviceToHost,
+    PMP_HIP(hipMemcpyAsync(n, (const int32_t*)ctx->eb.counter.ptr + kCntRecheck, sizeof(int32_t), hipMemcpyDeviceToHost,
                            st(ctx)));
     PMP_HIP(hipStreamSynchronize(st(ctx)));
   });

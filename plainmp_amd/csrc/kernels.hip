@@ -439,6 +439,68 @@ __device__ __forceinline__ EdgeQ edge_waypoint(const EdgeArgs& ea, int64_t e, in
   return EdgeQ{ea.s1, ea.s2, ea.lds, e, t, k == 0};
 }
 
+// An edge's endpoints held across its wave: lane i < D holds column i of s1
+// (a) and of s2 (b).  They are fetched once with vector loads, and each
+// per-joint read is a lane read into a scalar register, which keeps the
+// streamed edge data out of the scalar cache the cull and primitive tables
+// live in.
+struct EdgeEnds {
+  double a, b;
+};
+__device__ __forceinline__ EdgeEnds load_edge_ends(const EdgeArgs& ea, int D, int64_t e, int lane) {
+  EdgeEnds r{0.0, 0.0};
+  if (lane < D) {
+    r.a = ea.s1[(int64_t)lane * ea.lds + e];
+    r.b = ea.s2[(int64_t)lane * ea.lds + e];
+  }
+  return r;
+}
+__device__ __forceinline__ double lane_f64(double v, int i) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, i);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), i);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+struct EdgeQv {
+  EdgeEnds ends;
+  double t;
+  bool at_end;
+  __device__ __forceinline__ double get(int col) const {
+    const double a = lane_f64(ends.a, col), b = lane_f64(ends.b, col);
+    return at_end ? b : a + (b - a) * t;
+  }
+};
+// edge_step_ratio and edge_waypoint over the wave-held endpoints: the same
+// arithmetic on the same values.
+__device__ __forceinline__ double edge_step_ratio(const EdgeArgs& ea, int D, const EdgeEnds& en) {
+  if (ea.vtype == PMP_VALIDATOR_BOX) {
+    double max_diff = -INFINITY, diff_longest = 0.0;
+    int longest = 0;
+    for (int i = 0; i < D; ++i) {
+      const double diff = lane_f64(en.b, i) - lane_f64(en.a, i);
+      const double asd = fabs(diff) * cload(ea.inv_width, i);
+      if (asd > max_diff) {
+        max_diff = asd;
+        longest = i;
+        diff_longest = diff;
+      }
+    }
+    return fabs(diff_longest) < 1e-6 ? INFINITY : cload(ea.width, longest) / fabs(diff_longest);
+  }
+  double d2 = 0.0;
+  for (int i = 0; i < D; ++i) {
+    const double diff = lane_f64(en.a, i) - lane_f64(en.b, i);
+    d2 += diff * diff;
+  }
+  return ea.resolution / sqrt(d2);
+}
+__device__ __forceinline__ EdgeQv edge_waypoint(const EdgeArgs& ea, const EdgeEnds& en, int64_t n_test, double ratio,
+                                                int64_t k) {
+  double t = 0.0;
+  if (k > 0) t = n_test <= 128 ? (double)ea.seq[(n_test - 1) * 128 + (k + 1)] * ratio : (double)k * ratio;
+  return EdgeQv{en, t, k == 0};
+}
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) v = min(v, __shfl_xor(v, s, 64));
@@ -462,7 +524,8 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
     // the index and every branch on it are wave-uniform by construction.
     const int e = __builtin_amdgcn_readfirstlane(atomicAdd(ea.counter, lane == 0 ? 1 : 0));
     if (e >= ea.E) break;
-    const double ratio = edge_step_ratio(ea, k32.n_cols, e);
+    const EdgeEnds ends = load_edge_ends(ea, k32.n_cols, e, lane);
+    const double ratio = edge_step_ratio(ea, k32.n_cols, ends);
     int state = 1;
     int64_t checked = 0;
     int n_pend = 0;        // queued in-band states, -1 once more than kPendStates
@@ -480,7 +543,7 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
           const bool has = k < n_wp;
           ValidVis32 v{k32, s32, band};
           v.idle = !has;
-          run_fk(k32, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), taylor, v);
+          run_fk(k32, edge_waypoint(ea, ends, n_test, ratio, has ? k : 0), taylor, v);
           const bool def = has && v.definite;
           if (__any(def)) kdef = wave_min(def ? (int)k : INT_MAX);
           const bool in_band = has && v.uncertain && !v.definite && (int)k < kdef;
@@ -511,83 +574,96 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
       if (state == 2) {
         ea.pend_info[2 * (int64_t)e] = n_pend;
         ea.pend_info[2 * (int64_t)e + 1] = kdef;
+        ea.plist[atomicAdd(ea.counter + kCntPending, 1)] = e;
       }
     }
   }
 }
 
-// FP64 decision of the edges left pending by edges_kernel, one workgroup of
-// kRecheckWaves waves per edge.  With its in-band states queued, one wave
+// FP64 decision of the edges left pending by edges_kernel, read from its
+// pending list, one workgroup of kRecheckWaves waves per edge and the list
+// shared out over the grid.  With its in-band states queued, the first wave
 // re-decides just those (a lane each); the first colliding state is then the
-// smaller of the FP64 hits and the FP32 pass's first certain collision.  An
-// edge that queued too many walks its states again: a round evaluates up to
-// kRecheckWaves chunks of 64 states at once, each chunk's primitive
-// candidates shared among kRecheckWaves / chunks waves, and a round that
-// finds a collision ends the walk.
+// smaller of the FP64 hits and the FP32 pass's first certain collision.  An edge that queued too many
+// walks its states again: a round evaluates up to kRecheckWaves chunks of 64
+// states at once, each chunk's candidates shared among kRecheckWaves / chunks
+// waves, and a round that finds a collision ends the walk.  Every branch
+// around a barrier depends only on the list entry, so it is uniform over the
+// workgroup.
 constexpr int kRecheckWaves = 4;
+constexpr int kRecheckGrid = 256;
 __global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinArgs<double> k64,
                                                                            SceneArgs<double> s64, EdgeArgs ea) {
-  const int64_t e = blockIdx.x;
-  if (e >= ea.E || ea.valid[e] != 2) return;  // uniform over the workgroup
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const double ratio = edge_step_ratio(ea, k64.n_cols, e);
-  const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
-  const int64_t n_wp = n_test - 1;
-  const int n_pend = ea.pend_info[2 * e];
-  if (n_pend >= 0) {  // uniform over the workgroup: no barrier on this path
-    if (w != 0) return;
-    const bool has = lane < n_pend;
-    const int64_t k = has ? ea.pend[e * kPendStates + lane] : 0;
-    ValidVis64 v{k64, s64};
-    v.collided = !has;
-    run_fk(k64, edge_waypoint(ea, e, n_test, ratio, k), k64.base_type == PMP_BASE_FIXED, v);
-    const bool hit = has && v.collided;
-    const int kdef = ea.pend_info[2 * e + 1];
-    const int first = min(kdef, __any(hit) ? wave_min(hit ? (int)k : INT_MAX) : INT_MAX);
-    if (lane == 0) {
-      ea.valid[e] = first == INT_MAX ? 1 : 0;
-      if (ea.n_checked) ea.n_checked[e] = (int32_t)(first == INT_MAX ? n_wp : (int64_t)first + 1);
-      if (ea.n_recheck) atomicAdd(ea.n_recheck, 1);
-    }
-    return;
-  }
+  const int n_list = ea.counter[kCntPending];
   __shared__ int first;
-  if (threadIdx.x == 0) first = INT_MAX;
-  __syncthreads();
-  const int64_t n_chunks = (n_wp + 63) / 64;
-  const int per_round = n_chunks < kRecheckWaves ? (int)n_chunks : kRecheckWaves;
-  const int n_split = kRecheckWaves / per_round;
-  for (int64_t c0 = 0; c0 < n_chunks; c0 += per_round) {
-    const int64_t chunk = c0 + w / n_split;
-    if (w / n_split < per_round && chunk < n_chunks) {
-      const int64_t k = chunk * 64 + lane;
-      const bool has = k < n_wp;
-      ValidVis64 v{k64, s64};
-      v.split = w % n_split;
-      v.n_split = n_split;
-      v.collided = !has;
-      run_fk(k64, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), k64.base_type == PMP_BASE_FIXED, v);
-      const bool hit = has && v.collided;
-      if (__any(hit)) {
-        const int kmin = wave_min(hit ? (int)k : INT_MAX);
-        if (lane == 0) atomicMin(&first, kmin);
+  for (int i = blockIdx.x; i < n_list; i += gridDim.x) {
+    const int64_t e = ea.plist[i];
+    const double ratio = edge_step_ratio(ea, k64.n_cols, e);
+    const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
+    const int64_t n_wp = n_test - 1;
+    const int n_pend = ea.pend_info[2 * e];
+    if (threadIdx.x == 0) first = n_pend >= 0 ? ea.pend_info[2 * e + 1] : INT_MAX;
+    __syncthreads();
+    if (n_pend >= 0) {
+      // One wave: sharing out so few states' candidates among the waves
+      // measured slower than one wave walking them all.
+      if (w == 0) {
+        const bool has = lane < n_pend;
+        const int64_t k = has ? ea.pend[e * kPendStates + lane] : 0;
+        ValidVis64 v{k64, s64};
+        v.collided = !has;
+        run_fk(k64, edge_waypoint(ea, e, n_test, ratio, k), k64.base_type == PMP_BASE_FIXED, v);
+        const bool hit = has && v.collided;
+        if (__any(hit)) {
+          const int kmin = wave_min(hit ? (int)k : INT_MAX);
+          if (lane == 0) atomicMin(&first, kmin);
+        }
+      }
+      __syncthreads();
+    } else {
+      const int64_t n_chunks = (n_wp + 63) / 64;
+      const int per_round = n_chunks < kRecheckWaves ? (int)n_chunks : kRecheckWaves;
+      const int n_split = kRecheckWaves / per_round;
+      for (int64_t c0 = 0; c0 < n_chunks; c0 += per_round) {
+        const int64_t chunk = c0 + w / n_split;
+        if (w / n_split < per_round && chunk < n_chunks) {
+          const int64_t k = chunk * 64 + lane;
+          const bool has = k < n_wp;
+          ValidVis64 v{k64, s64};
+          v.split = w % n_split;
+          v.n_split = n_split;
+          v.collided = !has;
+          run_fk(k64, edge_waypoint(ea, e, n_test, ratio, has ? k : 0), k64.base_type == PMP_BASE_FIXED, v);
+          const bool hit = has && v.collided;
+          if (__any(hit)) {
+            const int kmin = wave_min(hit ? (int)k : INT_MAX);
+            if (lane == 0) atomicMin(&first, kmin);
+          }
+        }
+        __syncthreads();
+        const int f = first;
+        __syncthreads();
+        if (f != INT_MAX) break;  // uniform: every wave read the same value between the barriers
       }
     }
-    __syncthreads();
-    const int f = first;
-    __syncthreads();
-    if (f != INT_MAX) break;  // uniform: every wave read the same value between the barriers
+    if (threadIdx.x == 0) {
+      ea.valid[e] = first == INT_MAX ? 1 : 0;
+      if (ea.n_checked) ea.n_checked[e] = (int32_t)(first == INT_MAX ? n_wp : (int64_t)first + 1);
+    }
+    __syncthreads();  // thread 0 has read `first` before the next entry resets it
   }
+  // The last workgroup to finish resets the counters for the next launch.
+  __syncthreads();
   if (threadIdx.x == 0) {
-    ea.valid[e] = first == INT_MAX ? 1 : 0;
-    if (ea.n_checked) ea.n_checked[e] = (int32_t)(first == INT_MAX ? n_wp : (int64_t)first + 1);
-    if (ea.n_recheck) atomicAdd(ea.n_recheck, 1);
+    __threadfence();
+    if (atomicAdd(ea.counter + kCntDone, 1) == (int)gridDim.x - 1) {
+      ea.counter[kCntRecheck] = n_list;
+      ea.counter[kCntPending] = 0;
+      ea.counter[kCntWork] = 0;
+      ea.counter[kCntDone] = 0;
+    }
   }
-}
-
-__global__ void zero_kernel(int32_t* p, int n) {
-  int i = threadIdx.x;
-  if (i < n) p[i] = 0;
 }
 
 // ------------------------------------------------- Jacobian-carrying visitors
@@ -1163,7 +1239,6 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after) {
-  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ea.counter, 2);
   if (ev_before) {
     hipError_t e = hipEventRecord(ev_before, st);
     if (e != hipSuccess) return e;
@@ -1173,7 +1248,8 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
     hipError_t e = hipEventRecord(ev_after, st);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(edges_recheck_kernel, dim3((unsigned)ea.E), dim3(64 * kRecheckWaves), 0, st, k64, s64, ea);
+  const unsigned rgrid = (unsigned)(ea.E < kRecheckGrid ? ea.E : kRecheckGrid);
+  hipLaunchKernelGGL(edges_recheck_kernel, dim3(rgrid), dim3(64 * kRecheckWaves), 0, st, k64, s64, ea);
   return hipGetLastError();
 }
 
@@ -1194,6 +1270,10 @@ __global__ void fill_f32_kernel(float* p, float v, int64_t n) {
 __global__ void fill_counts_kernel(int32_t* p, int32_t v, int64_t n) {
   const int64_t i = gid();
   if (i < n) p[i] = v;
+}
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(fill_counts_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
 }
 hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                              int64_t B, const EvalArgs& e, hipStream_t st) {

@@ -9,6 +9,12 @@
 namespace pmp {
 
 constexpr int kPendStates = 16;  // in-band states an edge may queue for the FP64 pass
+// Slots of EdgeArgs::counter.  The work counter and the pending-list length
+// are raised by edges_kernel; the last workgroup of edges_recheck_kernel
+// publishes the list length as the recheck count and sets the other three
+// back to zero, so a launch leaves them as it found them (zero when the
+// buffer is allocated) and no clearing node is needed in a captured graph.
+enum EdgeCounter { kCntWork = 0, kCntRecheck = 1, kCntPending = 2, kCntDone = 3, kNumEdgeCounters = 4 };
 
 struct EdgeArgs {
   const double* s1;
@@ -19,14 +25,16 @@ struct EdgeArgs {
   const double* width;      // device [n_cols]
   const double* inv_width;  // device [n_cols], 1 / width as the reference precomputes
   const uint8_t* seq;       // device [128 * 128] farthest-first sequence table
-  int32_t* counter;         // device work counter, zeroed before each launch
+  int32_t* counter;         // device [kNumEdgeCounters], see EdgeCounter
   int32_t* pend;            // [E][kPendStates] in-band visiting indices of pending edges
   int32_t* pend_info;       // [E][2] number queued (-1: more than kPendStates), first certain collision
+  int32_t* plist;           // [E] the pending edges, in the order edges_kernel finished them
   uint8_t* valid;
   int32_t* n_checked;
-  int32_t* n_recheck;
 };
 
+// p[0, n) = v by a kernel (safe under stream capture, unlike a memset node).
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int taylor, float* out,
                           hipStream_t st);
 hipError_t launch_poses(const KinArgs<double>& ka, const double* q, int64_t ldq, int64_t B, int taylor, double* out,

@@ -70,6 +70,58 @@ def test_edges_mixed16_subset():
         assert mv.checkMotion(wl.S1[e], wl.S2[e]) == bool(ref[e])
 
 
+@pytest.mark.parametrize("band", [None, 0.05])
+def test_edges_graph_replays(band):
+    """An edge batch captured as a HIP graph and replayed several times, with
+    direct launches of a second batch on the same context and stream in
+    between: every replay and every direct call equals the oracle.  The
+    kernels reset their own work counters and pending list at the end of a
+    launch, so nothing carries over from one launch to the next; the 5 cm
+    band sends many edges through the FP64 pass."""
+    import torch
+
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    wl = synth.workload("edges_mixed16", scale=1 / 16)  # 256 edges
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    if band is not None:
+        pc.set_band(band)
+    ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(wl.box_width), 0.0, 8)
+    E, E2 = len(ref), 96
+    mv = BoxMotionValidator(pc, wl.box_width)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def soa(x):
+        return torch.from_numpy(np.ascontiguousarray(x.T)).to("cuda")
+
+    with torch.cuda.stream(stream):
+        s1, s2 = soa(wl.S1), soa(wl.S2)
+        t1, t2 = soa(wl.S1[-E2:]), soa(wl.S2[-E2:])
+        valid = torch.full((E,), 7, dtype=torch.uint8, device="cuda")
+        nchk = torch.full((E,), -5, dtype=torch.int32, device="cuda")
+        valid2 = torch.empty(E2, dtype=torch.uint8, device="cuda")
+        nchk2 = torch.empty(E2, dtype=torch.int32, device="cuda")
+        graph = mv._dev_capture_graph(s1.data_ptr(), s2.data_ptr(), E, E, valid.data_ptr(), nchk.data_ptr(), sh)
+    stream.synchronize()
+    for rep in range(4):
+        with torch.cuda.stream(stream):
+            valid.fill_(7)
+            nchk.fill_(-5)
+            graph.launch()
+        stream.synchronize()
+        got, n_got = valid.cpu().numpy(), nchk.cpu().numpy()
+        _report("edges_graph", band=band, replay=rep, valid_frac=got.astype(bool).mean())
+        assert np.array_equal(got, ref.astype(np.uint8))
+        assert np.array_equal(n_got, n_ref)
+        with torch.cuda.stream(stream):
+            mv._dev_check_motions(t1.data_ptr(), t2.data_ptr(), E2, E2, valid2.data_ptr(), nchk2.data_ptr(), sh)
+        stream.synchronize()
+        assert np.array_equal(valid2.cpu().numpy(), ref[-E2:].astype(np.uint8))
+        assert np.array_equal(nchk2.cpu().numpy(), n_ref[-E2:])
+    del graph
+
+
 def test_edges_euclidean_subset():
     from plainmp_amd.ompl import EuclideanMotionValidator
 
