@@ -1197,10 +1197,11 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
   ea.counter = (int32_t*)eb.counter.get(sizeof(int32_t) * kNumEdgeCounters);
   if (fresh) PMP_HIP(launch_fill_i32(ea.counter, 0, kNumEdgeCounters, st(ctx)));
   const int64_t Ea = std::max<int64_t>(E, 1);
-  int32_t* pend = (int32_t*)eb.pend.get(sizeof(int32_t) * Ea * (kPendStates + 3));
+  int32_t* pend = (int32_t*)eb.pend.get(sizeof(int32_t) * Ea * (3 * kPendStates + 3));
   ea.pend = pend;
   ea.pend_info = pend + Ea * kPendStates;
   ea.plist = pend + Ea * (kPendStates + 2);
+  ea.pend_pair = pend + Ea * (kPendStates + 3);
   ea.valid = valid;
   ea.n_checked = n_checked;
 }

@@ -144,6 +144,13 @@ struct ValidVis32 {
   bool idle = false;
   bool definite = false;
   bool uncertain = false;
+  // Candidates (block, primitive) found in the band; 2 also stands for an
+  // uncertainty not tied to one candidate (a self pair, a rotation outside
+  // the cull table's bounds).  With exactly one, the FP64 pass can re-decide
+  // the state on that candidate alone: every other pair was decided outside
+  // the band.
+  int n_inb = 0;
+  int inb_block = -1, inb_prim = -1;
   __device__ bool alive() const { return !definite && !idle; }
   template <class QS>
   __device__ void begin(const Pose<float>&, const QS&) {}
@@ -160,7 +167,7 @@ struct ValidVis32 {
   // neither collide nor fall in the band.
   template <int K>
   __device__ __forceinline__ void kind_loop(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
-                                            V3<float> g, const CullRange& cr) {
+                                            V3<float> g, const CullRange& cr, int bi) {
     const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
     for (int i = cr.at[K]; i < cr.at[K + 1]; ++i) {
       const CullEntry ce = cload(sc.cull, i);
@@ -168,14 +175,21 @@ struct ValidVis32 {
                        g.z > ce.ub[2];
       if (!wave_any(alive() && !out)) continue;
       const DevPrim<float> P = cload(vp, ce.prim);
+      bool ib = false;
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         if (k < n) {
           bool inb = false;
           const bool col = collide_kind<K>(P, c[k], bk.s[k][3], band, inb);
           definite |= col && !inb;
-          uncertain |= inb;
+          ib |= inb;
         }
+      }
+      if (ib) {
+        uncertain = true;
+        ++n_inb;
+        inb_block = bi;
+        inb_prim = ce.prim;
       }
     }
   }
@@ -186,7 +200,10 @@ struct ValidVis32 {
     // The cull table assumes |q|^2 of this rotation within [n2_lo, n2_hi]
     // (so that |R v| <= grow |v|); a lane outside is left to the FP64 pass.
     const float n2 = P.q.x * P.q.x + P.q.y * P.q.y + P.q.z * P.q.z + P.q.w * P.q.w;
-    if (!(n2 >= nd.n2_lo && n2 <= nd.n2_hi)) uncertain = true;
+    if (!(n2 >= nd.n2_lo && n2 <= nd.n2_hi)) {
+      uncertain = true;
+      n_inb = 2;
+    }
     const int n_solid = sc.vrange[kKindCloud];
     for (int bi = nd.block_begin; bi < nd.block_end; ++bi) {
       if (!wave_any(alive())) return;
@@ -235,12 +252,12 @@ struct ValidVis32 {
       }
       if (!wave_any(alive())) continue;
       const CullRange cr = cload(sc.cull_range, bi);
-      kind_loop<kKindBoxAA>(c, bk, n, g, cr);
-      kind_loop<kKindBoxRot>(c, bk, n, g, cr);
-      kind_loop<kKindCylZ>(c, bk, n, g, cr);
-      kind_loop<kKindCylRot>(c, bk, n, g, cr);
-      kind_loop<kKindSphere>(c, bk, n, g, cr);
-      kind_loop<kKindGround>(c, bk, n, g, cr);
+      kind_loop<kKindBoxAA>(c, bk, n, g, cr, bi);
+      kind_loop<kKindBoxRot>(c, bk, n, g, cr, bi);
+      kind_loop<kKindCylZ>(c, bk, n, g, cr, bi);
+      kind_loop<kKindCylRot>(c, bk, n, g, cr, bi);
+      kind_loop<kKindSphere>(c, bk, n, g, cr, bi);
+      kind_loop<kKindGround>(c, bk, n, g, cr, bi);
     }
   }
 
@@ -248,10 +265,12 @@ struct ValidVis32 {
     if (!alive()) return;
     const V3<float> d = sub(ci, cj);
     const float m = dot(d, d) - rs * rs;
-    if (fabsf(m) <= band * (2.0f * rs + band))
+    if (fabsf(m) <= band * (2.0f * rs + band)) {
       uncertain = true;
-    else if (m < 0.0f)
+      n_inb = 2;
+    } else if (m < 0.0f) {
       definite = true;
+    }
   }
 };
 
@@ -368,6 +387,41 @@ struct ValidVis64 {
     const V3<double> d = sub(ci, cj);
     if (dot(d, d) < rs * rs) collided = true;
   }
+};
+
+// FP64 re-decision of a state on the one candidate the FP32 pass found in
+// the band: the spheres of sphere block `block` against kind-sorted
+// primitive `prim`, both per lane, tested when the walk reaches the node
+// that carries the block.  Every other pair of the state was decided by the
+// FP32 pass outside the band, so this pair decides the state.
+struct PairVis64 {
+  static constexpr bool kSelf = false;
+  static constexpr bool kBlocks = true;
+  const KinArgs<double>& ka;
+  const SceneArgs<double>& sc;
+  int block = 0, prim = 0;
+  bool idle = false;
+  bool done = false;
+  bool collided = false;
+  __device__ bool alive() const { return !idle && !done; }
+  template <class QS>
+  __device__ void begin(const Pose<double>&, const QS&) {}
+  __device__ void node(int, const DevNode<double>&, const Pose<double>&) {}
+  __device__ void sphere(int, V3<double>, double) {}
+  __device__ void blocks(const DevNode<double>& nd, const Pose<double>& P) {
+    if (idle || done || block < nd.block_begin || block >= nd.block_end) return;
+    done = true;
+    const M3<double> R = qmat(P.q);
+    const DevSphereBlock& bk = ka.blocks[block];  // per-lane index: vector loads
+    const DevPrim<double>& pr = reinterpret_cast<const DevPrim<double>*>(sc.vprims)[prim];
+    const int n = bk.n;
+    for (int k = 0; k < n; ++k) {
+      const DevSphere<double> sp = ka.spheres[bk.sph[k]];
+      const V3<double> c = add(mv(R, V3<double>{sp.x, sp.y, sp.z}), P.t);
+      if (!outside_aabb(pr, c, sp.r) && !is_outside_ref<double>(pr, c, sp.r)) collided = true;
+    }
+  }
+  __device__ void self(const DevSelfPair&, int, int, V3<double>, V3<double>, double) {}
 };
 
 // valid[b]: 0 collision, 1 free, 2 pending FP64 re-decision.
@@ -552,7 +606,12 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
             unc = true;
             const int cnt = __popcll(m);
             if (n_pend >= 0 && n_pend + cnt <= kPendStates) {
-              if (in_band) ea.pend[(int64_t)e * kPendStates + n_pend + __popcll(m & ((1ull << lane) - 1))] = (int)k;
+              if (in_band) {
+                const int64_t slot = (int64_t)e * kPendStates + n_pend + __popcll(m & ((1ull << lane) - 1));
+                ea.pend[slot] = (int)k;
+                ea.pend_pair[2 * slot] = v.n_inb == 1 ? v.inb_block : -1;
+                ea.pend_pair[2 * slot + 1] = v.inb_prim;
+              }
               n_pend += cnt;
             } else {
               n_pend = -1;
@@ -610,11 +669,21 @@ __global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinAr
       // measured slower than one wave walking them all.
       if (w == 0) {
         const bool has = lane < n_pend;
-        const int64_t k = has ? ea.pend[e * kPendStates + lane] : 0;
-        ValidVis64 v{k64, s64};
-        v.collided = !has;
-        run_fk(k64, edge_waypoint(ea, e, n_test, ratio, k), k64.base_type == PMP_BASE_FIXED, v);
-        const bool hit = has && v.collided;
+        const int64_t slot = e * kPendStates + lane;
+        const int64_t k = has ? ea.pend[slot] : 0;
+        const int pb = has ? ea.pend_pair[2 * slot] : 0, pp = has ? ea.pend_pair[2 * slot + 1] : 0;
+        bool hit;
+        if (__any(pb < 0)) {  // a state whose uncertainty is not one candidate: full FP64 walk
+          ValidVis64 v{k64, s64};
+          v.collided = !has;
+          run_fk(k64, edge_waypoint(ea, e, n_test, ratio, k), k64.base_type == PMP_BASE_FIXED, v);
+          hit = has && v.collided;
+        } else {
+          PairVis64 v{k64, s64, pb, pp};
+          v.idle = !has;
+          run_fk(k64, edge_waypoint(ea, e, n_test, ratio, k), k64.base_type == PMP_BASE_FIXED, v);
+          hit = has && v.collided;
+        }
         if (__any(hit)) {
           const int kmin = wave_min(hit ? (int)k : INT_MAX);
           if (lane == 0) atomicMin(&first, kmin);

@@ -29,6 +29,8 @@ struct EdgeArgs {
   int32_t* pend;            // [E][kPendStates] in-band visiting indices of pending edges
   int32_t* pend_info;       // [E][2] number queued (-1: more than kPendStates), first certain collision
   int32_t* plist;           // [E] the pending edges, in the order edges_kernel finished them
+  int32_t* pend_pair;       // [E][kPendStates][2] the queued state's one in-band candidate (block, primitive),
+                            //   block -1 when the uncertainty is not one candidate
   uint8_t* valid;
   int32_t* n_checked;
 };

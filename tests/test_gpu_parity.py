@@ -70,14 +70,16 @@ def test_edges_mixed16_subset():
         assert mv.checkMotion(wl.S1[e], wl.S2[e]) == bool(ref[e])
 
 
-@pytest.mark.parametrize("band", [None, 0.05])
+@pytest.mark.parametrize("band", [None, 0.002, 0.05])
 def test_edges_graph_replays(band):
     """An edge batch captured as a HIP graph and replayed several times, with
     direct launches of a second batch on the same context and stream in
     between: every replay and every direct call equals the oracle.  The
     kernels reset their own work counters and pending list at the end of a
-    launch, so nothing carries over from one launch to the next; the 5 cm
-    band sends many edges through the FP64 pass."""
+    launch, so nothing carries over from one launch to the next.  A 2 mm band
+    queues states whose uncertainty is one candidate (re-decided on that
+    candidate alone, with and without a collision among them); the 5 cm band
+    sends many edges through the full FP64 walk."""
     import torch
 
     from plainmp_amd.ompl import BoxMotionValidator
