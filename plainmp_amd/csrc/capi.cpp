@@ -985,6 +985,8 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     a.pts64 = d.pts64 = ctx->pts64.upload(pts, s0);
     a.n_prims = d.n_prims = sc->n_prims;
     a.n_cloud = d.n_cloud = n_clouds;
+    for (int i = 0; i <= kMaxClouds; ++i)
+      a.super_at[i] = d.super_at[i] = i < n_clouds ? clouds[i].super_begin : (int32_t)sup.size();
     ctx->s32 = a;
     ctx->s64 = d;
     ctx->n_clouds = n_clouds;
@@ -1169,7 +1171,6 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
   PMP_REQUIRE(lds >= E, "leading dimension smaller than the edge count");
   PMP_REQUIRE(E < (int64_t)1 << 31, "too many edges");
   PMP_REQUIRE(v != nullptr, "null validator");
-  require_no_cloud(ctx, "pmp_check_edges_batch");
   const int D = ctx->n_cols;
   std::vector<double> w(std::max(D, 1), 1.0), iw(std::max(D, 1), 1.0);
   if (v->type == PMP_VALIDATOR_BOX) {

@@ -28,6 +28,19 @@ __device__ __forceinline__ DevPrim<T> prim_at(const SceneArgs<T>& sc, int p) {
 
 __device__ __forceinline__ int64_t gid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 
+// Squared distance from c to the box [lb, ub] (0 inside).
+__device__ __forceinline__ float box_dist2(const float (&lb)[3], const float (&ub)[3], V3<float> c) {
+  const float dx = fmaxf(fmaxf(lb[0] - c.x, c.x - ub[0]), 0.f);
+  const float dy = fmaxf(fmaxf(lb[1] - c.y, c.y - ub[1]), 0.f);
+  const float dz = fmaxf(fmaxf(lb[2] - c.z, c.z - ub[2]), 0.f);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Four consecutive floats of a table read as one wave-uniform load.
+struct alignas(16) Quad4 {
+  float v[4];
+};
+
 // ------------------------------------------------------------------ centres
 template <class T>
 struct CentersVis {
@@ -133,7 +146,11 @@ __global__ void __launch_bounds__(256) clearance_kernel(KinArgs<float> ka, Scene
 // FP32 pass: a collision decided outside the band ends the lane's work
 // ("definite"); a decision inside the band marks the configuration for the
 // FP64 re-decision ("uncertain").
-struct ValidVis32 {
+// kInlineCloud: clouds are tested inside the walk (the edge pass, whose
+// states have no batch-wide cloud pass of their own); otherwise the walk
+// writes cloud queries for the matrix-core pass when cw is set.
+template <bool kInlineCloud>
+struct ValidVis32T {
   static constexpr bool kSelf = true;
   static constexpr bool kBlocks = true;
   const KinArgs<float>& ka;
@@ -258,6 +275,81 @@ struct ValidVis32 {
       kind_loop<kKindCylRot>(c, bk, n, g, cr, bi);
       kind_loop<kKindSphere>(c, bk, n, g, cr, bi);
       kind_loop<kKindGround>(c, bk, n, g, cr, bi);
+      if constexpr (kInlineCloud) cloud_inline(c, bk, n, g, nd.n2_hi);
+    }
+  }
+
+  // Every cloud against the spheres of a block, for the edge pass.  Super-
+  // tiles, then tiles, are skipped when no live lane's block bounding sphere
+  // (centre g, radius G, stretched by the rotation's norm bound) comes
+  // within rc + band of their box; the points of a kept tile are compared
+  // in tile-local coordinates, |c - p|^2 = |c'|^2 + (|p'|^2 - 2 c'.p'), from
+  // the same operands as the matrix-core pass.  The decision is that of the
+  // cloud pass (CloudSDF::is_outside, primitive_sdf.hpp:597-603, after the
+  // AABB pre-test): collide iff min |c - p|^2 <= (r + rc)^2, with the band
+  // around it left to the FP64 walk.
+  __device__ __forceinline__ void cloud_inline(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
+                                               V3<float> g, float n2_hi) {
+    const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
+    for (int p = sc.vrange[kKindCloud]; p < sc.vrange[kKindCloud + 1]; ++p) {
+      if (!wave_any(alive())) return;
+      const DevPrim<float> Pc = cload(vp, p);
+      const float rc = Pc.p0;
+      const int oc = (Pc.flags >> 4) & 15;
+      const float reach = bk.G * n2_hi + rc + band + 1e-5f;
+      const float reach2 = reach * reach;
+      float best[kBlockSpheres];
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) best[k] = FLT_MAX;
+      const int s_end = sc.super_at[oc + 1];
+      for (int si = sc.super_at[oc]; si < s_end; ++si) {
+        const CloudSuper S = cload(sc.supers, si);
+        if (!wave_any(alive() && box_dist2(S.lb, S.ub, g) <= reach2)) continue;
+        for (int t = S.first; t < S.first + S.count; ++t) {
+          const CloudTile T = cload(sc.tiles, t);
+          if (!wave_any(alive() && box_dist2(T.lb, T.ub, g) <= reach2)) continue;
+          V3<float> cl[kBlockSpheres];
+          float tb[kBlockSpheres];
+#pragma unroll
+          for (int k = 0; k < kBlockSpheres; ++k) {
+            cl[k] = V3<float>{c[k].x - T.o[0], c[k].y - T.o[1], c[k].z - T.o[2]};
+            tb[k] = FLT_MAX;
+          }
+          const Quad4* Bt = reinterpret_cast<const Quad4*>(sc.tileB + (int64_t)t * 4 * kCloudTile);
+          for (int m = 0; m < 4; ++m) {
+            for (int j4 = 0; j4 < 4; ++j4) {
+              // points 16 m + 4 j4 .. + 3: components x, y, z, |p'|^2
+              const Quad4 X = cload(Bt, m * 16 + 0 * 4 + j4), Y = cload(Bt, m * 16 + 1 * 4 + j4);
+              const Quad4 Z = cload(Bt, m * 16 + 2 * 4 + j4), W = cload(Bt, m * 16 + 3 * 4 + j4);
+#pragma unroll
+              for (int jj = 0; jj < 4; ++jj) {
+#pragma unroll
+                for (int k = 0; k < kBlockSpheres; ++k) {
+                  if (k < n)
+                    tb[k] = fminf(tb[k], fmaf(X.v[jj], cl[k].x, fmaf(Y.v[jj], cl[k].y, fmaf(Z.v[jj], cl[k].z, W.v[jj]))));
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < kBlockSpheres; ++k) best[k] = fminf(best[k], tb[k] + dot(cl[k], cl[k]));
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        if (k < n) {
+          const float r = bk.s[k][3];
+          if (!alive() || outside_aabb(Pc, c[k], r)) continue;
+          const float R = r + rc;
+          const float m = best[k] - R * R;
+          if (fabsf(m) <= band * (2.0f * R + band)) {
+            uncertain = true;
+            n_inb = 2;
+          } else if (m < 0.0f) {
+            definite = true;
+          }
+        }
+      }
     }
   }
 
@@ -273,6 +365,8 @@ struct ValidVis32 {
     }
   }
 };
+
+using ValidVis32 = ValidVis32T<false>;
 
 // FP64 re-decision with the reference's own tests: per sphere the AABB
 // pre-test and is_outside (check_ext_collision), per self pair the strict
@@ -568,6 +662,7 @@ __device__ __forceinline__ int wave_min(int v) {
 // more than kMaxEdgeStates states).  A pending edge queues its in-band states
 // that can matter (all of them, or those before the first certain collision)
 // and that collision's index; the FP64 pass decides from them.
+template <bool kCloud>
 __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, EdgeArgs ea,
                                                     float band) {
   const int lane = threadIdx.x & 63;
@@ -595,7 +690,7 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
         for (int64_t base = 0; base < n_wp; base += 64) {
           const int64_t k = base + lane;
           const bool has = k < n_wp;
-          ValidVis32 v{k32, s32, band};
+          ValidVis32T<kCloud> v{k32, s32, band};
           v.idle = !has;
           run_fk(k32, edge_waypoint(ea, ends, n_test, ratio, has ? k : 0), taylor, v);
           const bool def = has && v.definite;
@@ -1312,7 +1407,11 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
     hipError_t e = hipEventRecord(ev_before, st);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(edges_kernel, dim3(grid), dim3(block), 0, st, k32, s32, ea, band);
+  // scenes with a point cloud take the variant that tests clouds in the walk
+  if (s32.n_cloud > 0)
+    hipLaunchKernelGGL(edges_kernel<true>, dim3(grid), dim3(block), 0, st, k32, s32, ea, band);
+  else
+    hipLaunchKernelGGL(edges_kernel<false>, dim3(grid), dim3(block), 0, st, k32, s32, ea, band);
   if (ev_after) {
     hipError_t e = hipEventRecord(ev_after, st);
     if (e != hipSuccess) return e;

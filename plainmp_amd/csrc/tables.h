@@ -154,6 +154,7 @@ struct SceneArgs {
   int32_t n_cloud;              // number of cloud primitives
   const CloudTile* tiles;       // all clouds' tiles
   const CloudSuper* supers;     // all clouds' super-tiles
+  int32_t super_at[kMaxClouds + 1];  // cloud i's super-tiles: supers[super_at[i], super_at[i + 1])
   const CloudTile64* tiles64;
   const float* tileB;           // [tile][4 MFMA][4 k][16 j] B operands in tile-local coordinates
   const double* pts64;          // [tile][64][4] FP64 world points (padding points far away)

@@ -286,6 +286,32 @@ def test_cloud_valid_and_clearance(n_configs):
     assert err <= 5e-6
 
 
+@pytest.mark.parametrize("band", [None, 0.02])
+def test_edges_cloud(band):
+    """Motion checks against the 65,536-point shelf cloud (the reference's
+    planning-with-a-cloud use): the edge pass tests the cloud inside its walk
+    and sends in-band states to the FP64 walk; validity and the states
+    checked in the reference's visiting order equal the oracle's KD-tree
+    answers.  The 2 cm band forces FP64 re-decisions of cloud states."""
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    E = 256
+    wl = synth.workload("cloud_shelves", scale=2 * E / 16384)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    if band is not None:
+        pc.set_band(band)
+    S1, S2 = wl.Q[:E], wl.Q[E : 2 * E]
+    w = [0.05] * S1.shape[1]
+    ref, n_ref = oc.check_motions(S1, S2, True, w, 0.0, 8)
+    mv = BoxMotionValidator(pc, w)
+    got, n_got = mv.check_motions(S1, S2)
+    _report("edges_cloud", band=band, E=E, valid_frac=ref.mean(), rechecked=mv.last_recheck_count())
+    if band is not None:
+        assert mv.last_recheck_count() > 0
+    assert np.array_equal(got, ref.astype(bool))
+    assert np.array_equal(n_got, n_ref)
+
+
 @pytest.mark.parametrize("height", [0.0, 0.3])
 def test_valid_every_kind_with_ground(height):
     """Boxes (axis-aligned and rotated), cylinders (z-aligned and rotated),
