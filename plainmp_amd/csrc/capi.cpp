@@ -1083,7 +1083,6 @@ pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, in
   return guarded([&] {
     PMP_BATCH_CHECKS(q, ldq, B);
     PMP_REQUIRE(B == 0 || (vals && jac), "null output");
-    require_no_cloud(ctx, "pmp_evaluate_batch");
     if (B == 0) return;
     set_device(ctx);
     EvalArgs e = eval_workspace(ctx, B);
@@ -1104,7 +1103,6 @@ pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t l
     for (int32_t i = 0; i < n_traj; ++i) PMP_REQUIRE(offsets[i + 1] >= offsets[i], "offsets must not decrease");
     PMP_REQUIRE(N == 0 || (vals && jac && grad), "null output");
     PMP_REQUIRE(n_traj == 0 || cost, "null output");
-    require_no_cloud(ctx, "pmp_traj_cost_grad_batch");
     if (n_traj == 0) return;
     set_device(ctx);
     std::vector<int64_t> off(offsets, offsets + n_traj + 1);

@@ -41,6 +41,81 @@ struct alignas(16) Quad4 {
   float v[4];
 };
 
+// Per sphere k < n of a block, min |c_k - p|^2 over the points of cloud oc
+// that lie in tiles within sqrt(reach2) of the block's bounding centre g
+// (FLT_MAX when none).  Super-tiles, then tiles, are skipped for the whole
+// wave when no `live` lane's g reaches their box; a kept tile's points are
+// compared in tile-local coordinates, |c - p|^2 = |c'|^2 + (|p'|^2 - 2 c'.p'),
+// from the matrix-core pass's operands (-2 p', |p'|^2).
+__device__ __forceinline__ void cloud_block_min(const SceneArgs<float>& sc, int oc, const V3<float> (&c)[kBlockSpheres],
+                                                int n, V3<float> g, float reach2, bool live,
+                                                float (&best)[kBlockSpheres]) {
+#pragma unroll
+  for (int k = 0; k < kBlockSpheres; ++k) best[k] = FLT_MAX;
+  const int s_end = sc.super_at[oc + 1];
+  for (int si = sc.super_at[oc]; si < s_end; ++si) {
+    const CloudSuper S = cload(sc.supers, si);
+    if (!wave_any(live && box_dist2(S.lb, S.ub, g) <= reach2)) continue;
+    for (int t = S.first; t < S.first + S.count; ++t) {
+      const CloudTile T = cload(sc.tiles, t);
+      if (!wave_any(live && box_dist2(T.lb, T.ub, g) <= reach2)) continue;
+      V3<float> cl[kBlockSpheres];
+      float tb[kBlockSpheres];
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        cl[k] = V3<float>{c[k].x - T.o[0], c[k].y - T.o[1], c[k].z - T.o[2]};
+        tb[k] = FLT_MAX;
+      }
+      const Quad4* Bt = reinterpret_cast<const Quad4*>(sc.tileB + (int64_t)t * 4 * kCloudTile);
+      for (int m = 0; m < 4; ++m) {
+        for (int j4 = 0; j4 < 4; ++j4) {
+          // points 16 m + 4 j4 .. + 3: components x, y, z, |p'|^2
+          const Quad4 X = cload(Bt, m * 16 + 0 * 4 + j4), Y = cload(Bt, m * 16 + 1 * 4 + j4);
+          const Quad4 Z = cload(Bt, m * 16 + 2 * 4 + j4), W = cload(Bt, m * 16 + 3 * 4 + j4);
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+#pragma unroll
+            for (int k = 0; k < kBlockSpheres; ++k) {
+              if (k < n)
+                tb[k] = fminf(tb[k], fmaf(X.v[jj], cl[k].x, fmaf(Y.v[jj], cl[k].y, fmaf(Z.v[jj], cl[k].z, W.v[jj]))));
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) best[k] = fminf(best[k], tb[k] + dot(cl[k], cl[k]));
+    }
+  }
+}
+
+// The nearest point of cloud oc to c, per lane (an exhaustive search over
+// the tiles, skipping those whose box is farther than the best so far).
+__device__ __forceinline__ V3<float> cloud_nearest(const SceneArgs<float>& sc, int oc, V3<float> c) {
+  float best = FLT_MAX;
+  V3<float> bp = c;
+  for (int si = sc.super_at[oc]; si < sc.super_at[oc + 1]; ++si) {
+    const CloudSuper S = sc.supers[si];
+    if (box_dist2(S.lb, S.ub, c) > best) continue;
+    for (int t = S.first; t < S.first + S.count; ++t) {
+      const CloudTile T = sc.tiles[t];
+      if (box_dist2(T.lb, T.ub, c) > best) continue;
+      const V3<float> cl{c.x - T.o[0], c.y - T.o[1], c.z - T.o[2]};
+      const float off = dot(cl, cl);
+      const float* Bt = sc.tileB + (int64_t)t * 4 * kCloudTile;
+      for (int j = 0; j < kCloudTile; ++j) {
+        const float* col = Bt + (j / 16) * 64 + (j % 16);
+        const float bx = col[0], by = col[16], bz = col[32], bw = col[48];
+        const float d2 = fmaf(bx, cl.x, fmaf(by, cl.y, fmaf(bz, cl.z, bw))) + off;
+        if (d2 < best) {
+          best = d2;
+          bp = V3<float>{T.o[0] - 0.5f * bx, T.o[1] - 0.5f * by, T.o[2] - 0.5f * bz};
+        }
+      }
+    }
+  }
+  return bp;
+}
+
 // ------------------------------------------------------------------ centres
 template <class T>
 struct CentersVis {
@@ -295,46 +370,9 @@ struct ValidVis32T {
       if (!wave_any(alive())) return;
       const DevPrim<float> Pc = cload(vp, p);
       const float rc = Pc.p0;
-      const int oc = (Pc.flags >> 4) & 15;
       const float reach = bk.G * n2_hi + rc + band + 1e-5f;
-      const float reach2 = reach * reach;
       float best[kBlockSpheres];
-#pragma unroll
-      for (int k = 0; k < kBlockSpheres; ++k) best[k] = FLT_MAX;
-      const int s_end = sc.super_at[oc + 1];
-      for (int si = sc.super_at[oc]; si < s_end; ++si) {
-        const CloudSuper S = cload(sc.supers, si);
-        if (!wave_any(alive() && box_dist2(S.lb, S.ub, g) <= reach2)) continue;
-        for (int t = S.first; t < S.first + S.count; ++t) {
-          const CloudTile T = cload(sc.tiles, t);
-          if (!wave_any(alive() && box_dist2(T.lb, T.ub, g) <= reach2)) continue;
-          V3<float> cl[kBlockSpheres];
-          float tb[kBlockSpheres];
-#pragma unroll
-          for (int k = 0; k < kBlockSpheres; ++k) {
-            cl[k] = V3<float>{c[k].x - T.o[0], c[k].y - T.o[1], c[k].z - T.o[2]};
-            tb[k] = FLT_MAX;
-          }
-          const Quad4* Bt = reinterpret_cast<const Quad4*>(sc.tileB + (int64_t)t * 4 * kCloudTile);
-          for (int m = 0; m < 4; ++m) {
-            for (int j4 = 0; j4 < 4; ++j4) {
-              // points 16 m + 4 j4 .. + 3: components x, y, z, |p'|^2
-              const Quad4 X = cload(Bt, m * 16 + 0 * 4 + j4), Y = cload(Bt, m * 16 + 1 * 4 + j4);
-              const Quad4 Z = cload(Bt, m * 16 + 2 * 4 + j4), W = cload(Bt, m * 16 + 3 * 4 + j4);
-#pragma unroll
-              for (int jj = 0; jj < 4; ++jj) {
-#pragma unroll
-                for (int k = 0; k < kBlockSpheres; ++k) {
-                  if (k < n)
-                    tb[k] = fminf(tb[k], fmaf(X.v[jj], cl[k].x, fmaf(Y.v[jj], cl[k].y, fmaf(Z.v[jj], cl[k].z, W.v[jj]))));
-                }
-              }
-            }
-          }
-#pragma unroll
-          for (int k = 0; k < kBlockSpheres; ++k) best[k] = fminf(best[k], tb[k] + dot(cl[k], cl[k]));
-        }
-      }
+      cloud_block_min(sc, (Pc.flags >> 4) & 15, c, n, g, reach * reach, alive(), best);
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         if (k < n) {
@@ -983,6 +1021,41 @@ struct EvalUnitVis {
       kind_loop<kKindCylRot>(c, bk, n, m, g, cr, all);
       kind_loop<kKindSphere>(c, bk, n, m, g, cr, all);
       kind_loop<kKindGround>(c, bk, n, m, g, cr, all);
+      if (sc.n_cloud > 0) cloud_loop(c, bk, n, m, g, nd.n2_hi, all);
+    }
+  }
+
+  // Clouds: sqrt(min |c - p|^2) - rc - r (CloudSDF::evaluate,
+  // primitive_sdf.hpp:585-590, minus the sphere radius) after the reference's
+  // AABB prefilter grown by the cutoff.  Only points within r + rc + cutoff of
+  // a sphere can give a value below the cutoff, so the minimum is taken over
+  // the tiles within that reach of the block (all tiles when the rotation is
+  // outside the norm bounds).
+  __device__ __forceinline__ void cloud_loop(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
+                                             const DevSphereMeta (&m)[kBlockSpheres], V3<float> g, float n2_hi,
+                                             bool all) {
+    const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
+    for (int p = sc.vrange[kKindCloud]; p < sc.vrange[kKindCloud + 1]; ++p) {
+      const DevPrim<float> Pc = cload(vp, p);
+      const int orig = cload(sc.vorig, p);
+      const float rc = Pc.p0;
+      const float reach = bk.G * n2_hi + rc + kCutoff + 1e-5f;
+      float d2[kBlockSpheres];
+      cloud_block_min(sc, (Pc.flags >> 4) & 15, c, n, g, all ? FLT_MAX : reach * reach, true, d2);
+#pragma unroll
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        if (k < n) {
+          const float r = bk.s[k][3];
+          if (d2[k] == FLT_MAX || outside_aabb(Pc, c[k], r + kCutoff)) continue;
+          const float v = sqrtf(d2[k]) - rc - r;
+          const int key = (m[k].group * sc.n_prims + orig) * ka.n_spheres + m[k].out;
+          if (v < best || (bidx >= 0 && v == best && key < bkey)) {
+            best = v;
+            bkey = key;
+            bidx = bk.sph[k] * kMaxPrims + orig;
+          }
+        }
+      }
     }
   }
 
@@ -1139,7 +1212,15 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
     V3<float> g{0.f, 0.f, 0.f};
     uint32_t mask = 0;
     if (bidx >= 0) {
-      g = sdf_grad(prim_at(sc, bidx % kMaxPrims), rv.c0);
+      const DevPrim<float> P = prim_at(sc, bidx % kMaxPrims);
+      if (P.type == PMP_PRIM_CLOUD) {
+        // the gradient of sqrt(min |c - p|^2) at the nearest point, which the
+        // reference's forward difference (eps 1e-6) approximates
+        const V3<float> d = sub(rv.c0, cloud_nearest(sc, (P.flags >> 4) & 15, rv.c0));
+        g = scale(1.0f / sqrtf(dot(d, d)), d);
+      } else {
+        g = sdf_grad(P, rv.c0);
+      }
       mask = cload(ka.nodes, cload(ka.meta, rv.s0).node).col_mask;
     }
 #pragma unroll

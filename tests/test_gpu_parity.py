@@ -403,6 +403,37 @@ def test_trajectory_rows_and_smoothness_dual_arm():
     _report("traj", N=len(X), T=len(off) - 1, val_err=np.abs(vals - vref).max())
 
 
+def test_evaluate_and_trajectory_cloud():
+    """evaluate rows and trajectory rows against the 65,536-point shelf cloud:
+    values (sqrt of the nearest squared distance - r_cloud - r, below the
+    0.1 m cutoff) and rows (gradient at the nearest point; the reference's
+    forward difference approximates it) against the oracle's KD-tree
+    evaluate, and the trajectory launch against the plain one."""
+    wl = synth.workload("cloud_shelves", scale=256 / 16384)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    Q = wl.Q
+    vref, jref = oc.evaluate_batch(Q, 8)
+    vgot, jgot = pc.evaluate_batch(Q)
+    assert vgot.shape == vref.shape and jgot.shape == jref.shape
+    verr = np.abs(vgot - vref).max()
+    pd = oc.pair_clearance_batch(Q, True, 8)
+    clear = (_second_gap(pd) > 1e-5) & (vref[:, 0] < 0.1)
+    _report("evaluate_cloud", B=len(Q), below_cutoff=int((vref[:, 0] < 0.1).sum()), clear=int(clear.sum()), val_err=verr)
+    assert clear.sum() >= 10
+    assert verr <= 5e-6
+    scale = np.maximum(np.abs(jref[clear]), 1.0)
+    assert (np.abs(jgot - jref)[clear] <= 1e-4 * scale).all()
+    off = [0, 40, 100, 101, 256]
+    w = np.linspace(0.5, 2.0, Q.shape[1])
+    vals, jac, cost, grad = pc.traj_cost_grad_batch(Q, off, list(w))
+    assert np.abs(vals - vref).max() <= 5e-6
+    assert (np.abs(jac - jref)[clear] <= 1e-4 * scale).all()
+    for i in range(len(off) - 1):
+        a, b = off[i], off[i + 1]
+        cref, gref = _oracle.smooth_cost(Q[a:b].ravel(), b - a, Q.shape[1], list(w))
+        assert abs(cost[i] - cref) <= 1e-5 * max(1.0, abs(cref))
+
+
 def test_trajectory_edge_cases():
     """Empty, one- and two-waypoint trajectories have zero smoothness cost and
     gradient; bad offsets are rejected as ValueError."""
