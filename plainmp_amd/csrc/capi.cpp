@@ -539,12 +539,6 @@ EvalArgs eval_workspace(pmp_context* ctx, int64_t B) {
   return e;
 }
 
-void require_no_cloud(pmp_context* ctx, const char* what) {
-  if (ctx->n_clouds > 0)
-    throw Error(PMP_ERR_INVALID, std::string(what) + ": point-cloud primitives are supported by is_valid and "
-                                                    "sphere_clearance only");
-}
-
 }  // namespace
 
 extern "C" {
@@ -1137,7 +1131,6 @@ pmp_status pmp_pair_clearance_batch(pmp_context* ctx, const double* q, int64_t l
   return guarded([&] {
     PMP_BATCH_CHECKS(q, ldq, B);
     PMP_REQUIRE(B == 0 || dist, "null output");
-    require_no_cloud(ctx, "pmp_pair_clearance_batch");
     if (B == 0 || ctx->n_spheres == 0 || ctx->s32.n_prims == 0) return;
     set_device(ctx);
     PMP_HIP(launch_pairs(ctx->k32, ctx->s32, q, ldq, B, dist, grad, st(ctx)));
@@ -1149,7 +1142,6 @@ pmp_status pmp_pair_clearance_vjp(pmp_context* ctx, const double* q, int64_t ldq
   return guarded([&] {
     PMP_BATCH_CHECKS(q, ldq, B);
     PMP_REQUIRE(B == 0 || (grad_dist && grad_q), "null argument");
-    require_no_cloud(ctx, "pmp_pair_clearance_vjp");
     if (B == 0) return;
     set_device(ctx);
     float* part = (float*)ctx->ws_vjp.get(sizeof(float) * std::max<int64_t>(1, (int64_t)ctx->k32.n_items * ctx->n_cols * B));

@@ -89,9 +89,10 @@ __device__ __forceinline__ void cloud_block_min(const SceneArgs<float>& sc, int 
 }
 
 // The nearest point of cloud oc to c, per lane (an exhaustive search over
-// the tiles, skipping those whose box is farther than the best so far).
-__device__ __forceinline__ V3<float> cloud_nearest(const SceneArgs<float>& sc, int oc, V3<float> c) {
-  float best = FLT_MAX;
+// the tiles, skipping those whose box is farther than the best so far), and
+// its squared distance in best.
+__device__ __forceinline__ V3<float> cloud_nearest(const SceneArgs<float>& sc, int oc, V3<float> c, float& best) {
+  best = FLT_MAX;
   V3<float> bp = c;
   for (int si = sc.super_at[oc]; si < sc.super_at[oc + 1]; ++si) {
     const CloudSuper S = sc.supers[si];
@@ -1216,7 +1217,8 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
       if (P.type == PMP_PRIM_CLOUD) {
         // the gradient of sqrt(min |c - p|^2) at the nearest point, which the
         // reference's forward difference (eps 1e-6) approximates
-        const V3<float> d = sub(rv.c0, cloud_nearest(sc, (P.flags >> 4) & 15, rv.c0));
+        float d2;
+        const V3<float> d = sub(rv.c0, cloud_nearest(sc, (P.flags >> 4) & 15, rv.c0, d2));
         g = scale(1.0f / sqrtf(dot(d, d)), d);
       } else {
         g = sdf_grad(P, rv.c0);
@@ -1251,7 +1253,9 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
 // own lane: the wave's item is uniform (grid y), FK runs only down to the
 // item's node, and the item's spheres meet every primitive.  Lanes b of a
 // wave are consecutive, so every store and gradient load is coalesced.
-template <int MAXC>
+// kCloud: the scene holds a point cloud (a separate kernel, so that the
+// cloud search does not weigh on the register allocation of the others).
+template <int MAXC, bool kCloud>
 struct PairVis {
   static constexpr bool kSelf = false;
   const KinArgs<float>& ka;
@@ -1281,12 +1285,22 @@ struct PairVis {
     for (int j = 0; j < MAXC; ++j) J[j] = fr.col(j, mask, ka.rev_mask, c);
     for (int p = 0; p < NP; ++p) {
       const DevPrim<float> P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) continue;
-      const float d = sdf_eval(P, c) - r;
+      float d;
+      V3<float> g{0.f, 0.f, 0.f};
+      if (kCloud && P.type == PMP_PRIM_CLOUD) {
+        // CloudSDF::evaluate (primitive_sdf.hpp:585-590): sqrt of the nearest
+        // squared distance minus r_cloud; its gradient at the nearest point
+        float d2;
+        const V3<float> dv = sub(c, cloud_nearest(sc, (P.flags >> 4) & 15, c, d2));
+        d = sqrtf(d2) - P.p0 - r;
+        g = scale(1.0f / sqrtf(dot(dv, dv)), dv);
+      } else {
+        d = sdf_eval(P, c) - r;
+        if (grad) g = sdf_grad(P, c);
+      }
       const int64_t row = (int64_t)m.out * NP + p;
       if (active) dist[row * B + b] = d;
       if (grad) {
-        const V3<float> g = sdf_grad(P, c);
 #pragma unroll
         for (int j = 0; j < MAXC; ++j)
           if (j < D && active) grad[(row * D + j) * B + b] = dot(g, J[j]);
@@ -1296,13 +1310,13 @@ struct PairVis {
   __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
 };
 
-template <int MAXC>
+template <int MAXC, bool kCloud>
 __global__ void __launch_bounds__(256) pair_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
                                                    int64_t ldq, int64_t B, float* dist, float* grad) {
   const int64_t b = gid();
   const bool active = b < B;
   BatchQ qs{q, ldq, active ? b : B - 1};
-  PairVis<MAXC> v{ka, sc, {}, dist, grad, B, b, active, cload(ka.items, blockIdx.y)};
+  PairVis<MAXC, kCloud> v{ka, sc, {}, dist, grad, B, b, active, cload(ka.items, blockIdx.y)};
   run_fk(ka, qs, false, v);
 }
 
@@ -1338,9 +1352,15 @@ struct VjpVis {
     V3<float> g{0.f, 0.f, 0.f};
     for (int p = 0; p < NP; ++p) {
       const DevPrim<float> P = prim_at(sc, p);
-      if (P.type == PMP_PRIM_CLOUD) continue;
       const float w = gd[((int64_t)m.out * NP + p) * B + b];
-      const V3<float> gp = sdf_grad(P, c);
+      V3<float> gp;
+      if (P.type == PMP_PRIM_CLOUD) {
+        float d2;
+        const V3<float> dv = sub(c, cloud_nearest(sc, (P.flags >> 4) & 15, c, d2));
+        gp = scale(1.0f / sqrtf(dot(dv, dv)), dv);
+      } else {
+        gp = sdf_grad(P, c);
+      }
       g = add(g, scale(w, gp));
     }
 #pragma unroll
@@ -1512,6 +1532,16 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
       hipLaunchKernelGGL(KERNEL<32>, __VA_ARGS__);                                            \
   } while (0)
 
+#define PMP_DISPATCH_MAXC_CLOUD(ncols, KERNEL, CLOUD, ...)                                   \
+  do {                                                                                        \
+    if ((ncols) <= 8)                                                                         \
+      hipLaunchKernelGGL((KERNEL<8, CLOUD>), __VA_ARGS__);                                    \
+    else if ((ncols) <= 16)                                                                   \
+      hipLaunchKernelGGL((KERNEL<16, CLOUD>), __VA_ARGS__);                                   \
+    else                                                                                      \
+      hipLaunchKernelGGL((KERNEL<32, CLOUD>), __VA_ARGS__);                                   \
+  } while (0)
+
 __global__ void fill_f32_kernel(float* p, float v, int64_t n) {
   const int64_t i = gid();
   if (i < n) p[i] = v;
@@ -1558,8 +1588,12 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                         int64_t B, float* dist, float* grad, hipStream_t st) {
   if (ka.n_items == 0) return hipSuccess;
-  PMP_DISPATCH_MAXC(ka.n_cols, pair_kernel, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256), 0, st, ka, sc,
-                    q, ldq, B, dist, grad);
+  if (sc.n_cloud > 0)
+    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, true, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256), 0,
+                            st, ka, sc, q, ldq, B, dist, grad);
+  else
+    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, false, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256),
+                            0, st, ka, sc, q, ldq, B, dist, grad);
   return hipGetLastError();
 }
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,

@@ -11,12 +11,13 @@ from support import oracle_cst, product_cst
 pytestmark = pytest.mark.gpu
 
 
-def test_pair_clearance_vjp_matches_jacobian_tensor():
+@pytest.mark.parametrize("name", ["grad_mixed32", "cloud_shelves"])
+def test_pair_clearance_vjp_matches_jacobian_tensor(name):
     import torch
 
     from plainmp_amd.autograd import pair_clearance
 
-    wl = synth.workload("grad_mixed32", scale=1 / 64)  # 128 configs
+    wl = synth.workload(name, scale=128 / (16384 if name == "cloud_shelves" else 8192))  # 128 configs
     pc = product_cst(wl)
     q = torch.tensor(wl.Q, device="cuda", dtype=torch.float64, requires_grad=True)
     d = pair_clearance(pc, q)

@@ -158,8 +158,13 @@ def test_evaluate_rows_grad_mixed32():
     assert (np.abs(jgot - jref)[clear] <= 1e-4 * scale).all()
 
 
-def test_pair_clearance_grad_vs_central_fd():
-    wl = synth.workload("grad_mixed32", scale=1 / 128)  # 64 configs
+@pytest.mark.parametrize("name", ["grad_mixed32", "cloud_shelves"])
+def test_pair_clearance_grad_vs_central_fd(name):
+    """Every sphere-primitive distance against the oracle and its gradient
+    against FP64 central differences (32 mixed primitives; the 65,536-point
+    shelf cloud, whose distance is sqrt of the nearest squared distance minus
+    r_cloud and whose gradient is taken at the nearest point)."""
+    wl = synth.workload(name, scale=64 / (16384 if name == "cloud_shelves" else 8192))  # 64 configs
     oc, pc = oracle_cst(wl), product_cst(wl)
     dref = oc.pair_clearance_batch(wl.Q, True, 8)
     dist, grad = pc.pair_clearance_batch(wl.Q, True)
