@@ -290,6 +290,23 @@ def _extra(torch, stream, dev):
         "note": "dense = every (sphere, point) pair as the K=4 product (8 flop); tile culling skips most "
                 "tiles, so this equivalent rate exceeds the 157.3 TFLOP/s FP32 MFMA peak",
     }
+    # the same arm and cloud, 4,096 motions between consecutive configurations
+    # (box validator 0.05 rad): the cloud variant of the edge kernel
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    E = 4096
+    S1, S2 = wl.Q[:E], wl.Q[E:2 * E]
+    mv = BoxMotionValidator(cst, [0.05] * cst.q_dim())
+    with torch.cuda.stream(stream):
+        s1, s2 = _soa(torch, S1), _soa(torch, S2)
+        ev = torch.empty(E, dtype=torch.uint8, device=dev)
+        en = torch.empty(E, dtype=torch.int32, device=dev)
+    ms = _time_stream(torch, stream, lambda: mv._dev_check_motions(s1.data_ptr(), s2.data_ptr(), E, E, ev.data_ptr(),
+                                                                   en.data_ptr(), sh), steps=10)
+    checked = int(en.sum().item())
+    res["cloud_shelves_edges"] = {"edges": E, "ms": ms, "edges_per_s": E / (ms * 1e-3),
+                                  "waypoints_checked": checked, "configs_per_s": checked / (ms * 1e-3),
+                                  "valid_frac": float(ev.float().mean().item())}
     return res
 
 
