@@ -307,6 +307,27 @@ def _extra(torch, stream, dev):
     res["cloud_shelves_edges"] = {"edges": E, "ms": ms, "edges_per_s": E / (ms * 1e-3),
                                   "waypoints_checked": checked, "configs_per_s": checked / (ms * 1e-3),
                                   "valid_frac": float(ev.float().mean().item())}
+    # path utilities on the edges_mixed16 scene (host arrays in and out, so
+    # these rates include the copies): every segment of a 4,096-state random
+    # walk in one launch, and batched shortcutting of it (1,024 candidates a
+    # round, 4 rounds)
+    wl = synth.workload("edges_mixed16")
+    cst = product_cst(wl)
+    mv = BoxMotionValidator(cst, wl.box_width)
+    rng = np.random.default_rng(11)
+    P = wl.S1[0] + np.cumsum(rng.normal(scale=0.02, size=(4096, cst.q_dim())), axis=0)
+    mv.check_path(P)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        seg = mv.check_path(P)
+    t_path = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    S = mv.shortcut_path(P, 1024, 4, 0)
+    t_short = time.perf_counter() - t0
+    res["path_check_and_shortcut"] = {"states": len(P), "check_path_ms": t_path * 1e3,
+                                      "segments_per_s": (len(P) - 1) / t_path,
+                                      "valid_segment_frac": float(seg.mean()), "shortcut_ms": t_short * 1e3,
+                                      "shortcut_states_out": len(S), "candidates": 4 * 1024}
     return res
 
 

@@ -515,6 +515,35 @@ PYBIND11_MODULE(_native, m) {
              }
              return std::make_pair(out, nc);
            })
+      .def("check_path",
+           [](MotionValidator& v, const DArr& P) {
+             int64_t n;
+             const double* p = rows(P, v.constraint()->q_dim(), n);
+             std::vector<uint8_t> ok;
+             {
+               py::gil_scoped_release rel;
+               ok = v.check_path(p, n);
+             }
+             py::array_t<bool> out((py::ssize_t)ok.size());
+             for (size_t i = 0; i < ok.size(); ++i) out.mutable_data()[i] = ok[i] != 0;
+             return out;
+           })
+      .def(
+          "shortcut_path",
+          [](MotionValidator& v, const DArr& P, int candidates, int rounds, uint64_t seed) {
+            const int64_t D = (int64_t)v.constraint()->q_dim();
+            int64_t n, m = 0;
+            const double* p = rows(P, D, n);
+            std::vector<double> out;
+            {
+              py::gil_scoped_release rel;
+              out = v.shortcut_path(p, n, candidates, rounds, seed, &m);
+            }
+            py::array_t<double> a({(py::ssize_t)m, (py::ssize_t)D});
+            std::copy(out.begin(), out.end(), a.mutable_data());
+            return a;
+          },
+          py::arg("path"), py::arg("candidates") = 256, py::arg("rounds") = 8, py::arg("seed") = 0)
       .def("_dev_check_motions",
            [](MotionValidator& v, uintptr_t s1, uintptr_t s2, int64_t lds, int64_t E, uintptr_t valid,
               uintptr_t nchk, uintptr_t stream) {

@@ -5,6 +5,8 @@
 // tree flattened at set_sdf time with primitives shared by pointer.
 #pragma once
 
+#include <random>
+
 #include "host.hpp"
 
 namespace pmp_host {
@@ -518,6 +520,76 @@ class MotionValidator {
     }
     return out;
   }
+  // Validity of every segment of a path (row-major (n, q_dim) states), all
+  // n - 1 segments in one batched launch: the check OMPL's path utilities
+  // make one checkMotion at a time (ompl_thin_wrap.hpp:130-170).
+  std::vector<uint8_t> check_path(const double* P, int64_t n) {
+    if (n < 2) return {};
+    return check_motions(P, P + cst_->q_dim(), n - 1, nullptr);
+  }
+
+  // Batched shortcutting, the planner's path simplification
+  // (PathSimplifier::shortcutPath / partialShortcutPath behind simplify(),
+  // ompl_thin_wrap.hpp:130-170, 235-248) with every candidate of a round
+  // checked in one launch.  A round draws `candidates` state pairs (i, j),
+  // j >= i + 2, uniformly from the current path (seeded, so the result is
+  // reproducible), checks the direct motions i -> j, and splices in the valid
+  // ones that do not overlap, the widest index span first (ties: smaller i).
+  // Rounds stop after `rounds` or when a round splices nothing.  The result
+  // keeps both end states, is a subsequence of the input, every segment it
+  // adds is a valid motion, and it is no longer than the input (each splice
+  // replaces a chain by its straight segment).
+  std::vector<double> shortcut_path(const double* P, int64_t n, int candidates, int rounds, uint64_t seed,
+                                    int64_t* n_out) {
+    const int64_t D = (int64_t)cst_->q_dim();
+    if (candidates < 0 || rounds < 0) throw std::invalid_argument("candidates and rounds must be non-negative");
+    std::vector<double> cur(P, P + n * D);
+    std::mt19937_64 rng(seed);
+    for (int r = 0; r < rounds; ++r) {
+      const int64_t m = (int64_t)cur.size() / D;
+      if (m < 3) break;
+      std::vector<std::pair<int64_t, int64_t>> cand;
+      cand.reserve(candidates);
+      for (int c = 0; c < candidates; ++c) {
+        const int64_t i = std::uniform_int_distribution<int64_t>(0, m - 3)(rng);
+        const int64_t j = std::uniform_int_distribution<int64_t>(i + 2, m - 1)(rng);
+        cand.push_back({i, j});
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      const int64_t E = (int64_t)cand.size();
+      std::vector<double> S1(E * D), S2(E * D);
+      for (int64_t e = 0; e < E; ++e) {
+        std::copy(&cur[cand[e].first * D], &cur[cand[e].first * D] + D, &S1[e * D]);
+        std::copy(&cur[cand[e].second * D], &cur[cand[e].second * D] + D, &S2[e * D]);
+      }
+      const std::vector<uint8_t> ok = check_motions(S1.data(), S2.data(), E, nullptr);
+      std::vector<std::pair<int64_t, int64_t>> valid;
+      for (int64_t e = 0; e < E; ++e)
+        if (ok[e]) valid.push_back(cand[e]);
+      std::stable_sort(valid.begin(), valid.end(), [](const auto& a, const auto& b) {
+        return a.second - a.first > b.second - b.first;
+      });
+      std::vector<std::pair<int64_t, int64_t>> taken;
+      for (const auto& v : valid) {
+        bool clash = false;
+        for (const auto& t : taken) clash = clash || (v.first < t.second && t.first < v.second);
+        if (!clash) taken.push_back(v);
+      }
+      if (taken.empty()) break;
+      std::vector<uint8_t> drop(m, 0);
+      for (const auto& t : taken)
+        for (int64_t k = t.first + 1; k < t.second; ++k) drop[k] = 1;
+      std::vector<double> next;
+      next.reserve(cur.size());
+      for (int64_t k = 0; k < m; ++k)
+        if (!drop[k]) next.insert(next.end(), &cur[k * D], &cur[k * D] + D);
+      cur.swap(next);
+    }
+    *n_out = (int64_t)cur.size() / D;
+    return cur;
+  }
+
   pmp_validator_desc desc() const {
     pmp_validator_desc v{};
     v.type = vc_.type == ValidatorConfig::Type::BOX ? PMP_VALIDATOR_BOX : PMP_VALIDATOR_EUCLIDEAN;

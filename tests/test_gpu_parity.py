@@ -124,6 +124,51 @@ def test_edges_graph_replays(band):
     del graph
 
 
+def test_check_path_and_shortcut():
+    """Batched path utilities over the edge pass, on a detour: a valid path
+    from a to b through a via state, where the direct motion a -> b collides.
+    check_path's per-segment validity equals the oracle's checkMotion on
+    consecutive states; the batched shortcut keeps the end states, returns a
+    subsequence of the input no longer than it that still needs an
+    intermediate state, every segment it adds is a valid motion per the
+    oracle, and it is reproducible from its seed."""
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    wl = synth.workload("edges_mixed16", scale=1 / 16)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    w = list(wl.box_width)
+    ev, _ = oc.check_motions(wl.S1, wl.S2, True, w, 0.0, 8)
+    e = int(np.flatnonzero(~ev.astype(bool))[0])
+    a, b = wl.S1[e], wl.S2[e]
+    V = np.concatenate([wl.S1, wl.S2])
+    V = V[oc.is_valid_batch(V, 8).astype(bool)]
+    r1, _ = oc.check_motions(np.repeat(a[None], len(V), 0), V, True, w, 0.0, 8)
+    r2, _ = oc.check_motions(V, np.repeat(b[None], len(V), 0), True, w, 0.0, 8)
+    v = V[np.flatnonzero(r1.astype(bool) & r2.astype(bool))[0]]
+    t = np.linspace(0, 1, 21)[:, None]
+    P = np.concatenate([a + t * (v - a), (v + t * (b - v))[1:]])
+    P[1:-1] += np.random.default_rng(7).normal(scale=0.01, size=P[1:-1].shape)
+    mv = BoxMotionValidator(pc, w)
+    seg = mv.check_path(P)
+    ref, _ = oc.check_motions(P[:-1], P[1:], True, w, 0.0, 8)
+    assert np.array_equal(seg, ref.astype(bool))
+    S = mv.shortcut_path(P, 256, 8, 3)
+    idx = [int(np.flatnonzero((P == s).all(axis=1))[0]) for s in S]
+    assert idx[0] == 0 and idx[-1] == len(P) - 1 and all(x < y for x, y in zip(idx, idx[1:]))
+    new = [(x, y) for x, y in zip(idx, idx[1:]) if y > x + 1]
+    _report("shortcut", states_in=len(P), states_out=len(S), spliced=len(new), valid_segments_in=int(seg.sum()))
+    assert len(S) >= 3 and len(new) > 0
+    r, _ = oc.check_motions(P[[x for x, _ in new]], P[[y for _, y in new]], True, w, 0.0, 8)
+    assert r.all()
+
+    def length(X):
+        return np.linalg.norm(np.diff(X, axis=0), axis=1).sum()
+
+    assert length(S) <= length(P) + 1e-12
+    assert np.array_equal(S, mv.shortcut_path(P, 256, 8, 3))
+    assert len(mv.check_path(P[:1])) == 0
+
+
 def test_edges_euclidean_subset():
     from plainmp_amd.ompl import EuclideanMotionValidator
 
