@@ -273,6 +273,23 @@ def _extra(torch, stream, dev):
         x.data_ptr(), N, N, offl, [], vals.data_ptr(), jac.data_ptr(), cost.data_ptr(), grad.data_ptr(), sh))
     res["traj_dual_arm_cost_grad"] = {"trajectories": T, "waypoints": N, "ms": ms,
                                       "waypoints_per_s": N / (ms * 1e-3), "trajectories_per_s": T / (ms * 1e-3)}
+    # the same trajectories as the SQP solver's sparse inequality sequence:
+    # evaluate rows of every waypoint + motion-step box rows, CSC per trajectory
+    msbox = [0.1] * D
+    r_off, z_off, p_off = cst.traj_sqp_sizes(offl, True)
+    with torch.cuda.stream(stream):
+        cv = torch.empty(max(r_off[-1], 1), dtype=torch.float64, device=dev)
+        cp = torch.empty(max(p_off[-1], 1), dtype=torch.int32, device=dev)
+        ri = torch.empty(max(z_off[-1], 1), dtype=torch.int32, device=dev)
+        va = torch.empty(max(z_off[-1], 1), dtype=torch.float64, device=dev)
+    ms = _time_stream(torch, stream, lambda: cst._dev_traj_sqp_ineq(
+        x.data_ptr(), N, N, offl, msbox, cv.data_ptr(), cp.data_ptr(), ri.data_ptr(), va.data_ptr(), sh))
+    out_bytes = r_off[-1] * 8 + p_off[-1] * 4 + z_off[-1] * 12
+    res["traj_dual_arm_sqp_csc"] = {"trajectories": T, "waypoints": N, "rows": r_off[-1], "nnz": z_off[-1],
+                                    "ms": ms, "nnz_per_s": z_off[-1] / (ms * 1e-3),
+                                    "output_gb_per_s": out_bytes / (ms * 1e-3) / 1e9,
+                                    "note": "includes the evaluate rows of every waypoint (same launch as "
+                                            "traj_dual_arm_cost_grad without the smoothness terms)"}
     # cloud_shelves: 16,384 configurations of an 8-DOF arm with 90 spheres against a
     # 65,536-point cloud (sphere-vs-cloud distances on the matrix cores)
     wl = synth.workload("cloud_shelves")

@@ -203,6 +203,28 @@ pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t l
                                     int32_t n_traj, const double* weights, float* vals, float* jac, float* cost,
                                     float* grad);
 
+/* The SQP solver's inequality sequence in its sparse form: SequentialCst::evaluate
+ * (constraints/sequential_constraint.cpp:63-127) of the collision constraint added
+ * at every waypoint plus, when msbox is given, the motion-step box constraint
+ * (the sequence src/plainmp/nlp_solver/sqp_based_solver.py:48-71 builds), for
+ * n_traj trajectories at once (X, ldx, N, offsets as pmp_traj_cost_grad_batch).
+ * Trajectory i with T_i waypoints has rows_i = cst_dim * T_i + 2 q_dim (T_i - 1)
+ * (no box rows without msbox; 0 rows when T_i = 0) and q_dim * T_i columns:
+ * the collision rows of every waypoint (t-major), then q_t - q_{t+1} + w and
+ * q_{t+1} - q_t + w per segment.  Its values go to c[row_off[i] ...], its
+ * Jacobian as CSC to colptr[colptr_off[i] ...] (q_dim T_i + 1 int32 column
+ * pointers counted from the trajectory's first entry), rowind / values
+ * [nnz_off[i] ...] (int32 row indices ascending within a column, FP64 values:
+ * collision entries J + 1e-12 as the reference writes them, box entries +-1).
+ * pmp_traj_sqp_sizes fills the HOST arrays row_off, nnz_off, colptr_off
+ * [n_traj + 1] (the last entry is the total); X and the outputs are device
+ * pointers, offsets and msbox ([q_dim] or NULL) host arrays. */
+pmp_status pmp_traj_sqp_sizes(pmp_context* ctx, const int64_t* offsets, int32_t n_traj, int32_t with_msbox,
+                              int64_t* row_off, int64_t* nnz_off, int64_t* colptr_off);
+pmp_status pmp_traj_sqp_ineq(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
+                             int32_t n_traj, const double* msbox, double* c, int32_t* colptr, int32_t* rowind,
+                             double* values);
+
 /* Signed distance of every (sphere, primitive) pair with exact FK and its
  * derivative with respect to q: dist[(s*P + p) * B + b],
  * grad[((s*P + p) * q_dim + j) * B + b].  grad may be NULL.  No reference

@@ -23,6 +23,10 @@
 //                        generate_sequence.py:37-39) + permutation invariants
 //   smoothness cost      closed form of the second-difference matrix
 //                        (nlp_solver/sqp_based_solver.py:74-96)
+//   SequentialCst        reference known answers (tests/test_constraints.py:
+//                        273-318: row count, msbox sign counts), FP64 central
+//                        differences of the values against the CSC Jacobian,
+//                        sparsity pattern unchanged between evaluations
 // Parity unpinned: the libstdc++ unordered_set iteration order that drives
 // the sphere reorder (primitive_sphere_collision.cpp:53-96) is derived from
 // libstdc++'s hashtable rules, not observed from the reference binary.
@@ -1341,6 +1345,63 @@ inline double smooth_cost(const std::vector<double>& x, size_t T, size_t dof, co
       }
     }
   return cost;
+}
+
+// SequentialCst::evaluate (constraints/sequential_constraint.cpp:63-127) of
+// the inequality sequence plainmp's SQP solver builds (sqp_based_solver.py:
+// 48-71): the collision constraint at every waypoint (rows J_t + 1e-12 at
+// (c_head + i, t * dof + j)), then, when msbox is given, the motion-step box
+// rows q_t - q_{t+1} + w and q_{t+1} - q_t + w with their +-1 entries.  The
+// reference fills an Eigen column-major sparse matrix with coeffRef; this
+// collects the same (row, column) -> value map and compresses it to CSC
+// (colptr, rowind sorted within each column, values).
+struct SeqCsc {
+  std::vector<double> c;
+  std::vector<int32_t> colptr, rowind;
+  std::vector<double> values;
+  size_t rows = 0, cols = 0;
+};
+inline SeqCsc seq_ineq_evaluate(SphereCollisionCst& cst, const std::vector<double>& x, size_t T,
+                                const std::vector<double>* msbox) {
+  const size_t D = cst.q_dim(), C = cst.cst_dim();
+  SeqCsc out;
+  out.cols = D * T;
+  out.rows = C * T + (msbox && T > 0 ? 2 * D * (T - 1) : 0);
+  out.c.assign(out.rows, 0.0);
+  std::vector<std::map<int32_t, double>> col(out.cols);
+  size_t c_head = 0;
+  for (size_t t = 0; t < T; ++t) {
+    std::vector<double> q(x.begin() + t * D, x.begin() + (t + 1) * D);
+    auto [v, J] = cst.evaluate(q);
+    for (size_t i = 0; i < C; ++i) {
+      out.c[c_head + i] = v[i];
+      for (size_t j = 0; j < D; ++j) col[t * D + j][(int32_t)(c_head + i)] = J[i][j] + 1e-12;
+    }
+    c_head += C;
+  }
+  if (msbox) {
+    for (size_t t = 0; t + 1 < T; ++t) {
+      for (size_t i = 0; i < D; ++i) {
+        const double q1 = x[t * D + i], q2 = x[(t + 1) * D + i], w = (*msbox)[i];
+        out.c[c_head + i] = q1 - q2 + w;
+        out.c[c_head + D + i] = q2 - q1 + w;
+        col[t * D + i][(int32_t)(c_head + i)] = 1.0;
+        col[(t + 1) * D + i][(int32_t)(c_head + i)] = -1.0;
+        col[t * D + i][(int32_t)(c_head + D + i)] = -1.0;
+        col[(t + 1) * D + i][(int32_t)(c_head + D + i)] = 1.0;
+      }
+      c_head += 2 * D;
+    }
+  }
+  out.colptr.push_back(0);
+  for (const auto& m : col) {
+    for (const auto& [r, val] : m) {
+      out.rowind.push_back(r);
+      out.values.push_back(val);
+    }
+    out.colptr.push_back((int32_t)out.rowind.size());
+  }
+  return out;
 }
 
 }  // namespace oracle

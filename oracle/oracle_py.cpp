@@ -108,6 +108,20 @@ PYBIND11_MODULE(_oracle, m) {
   m.def("box_step_ratio", &box_step_ratio);
   m.def("euclid_step_ratio", &euclid_step_ratio);
   m.def("motion_travel_rates", &motion_travel_rates);
+  // SequentialCst::evaluate of the SQP inequality sequence for one trajectory
+  // x (T, dof): (c, colptr, rowind, values, (rows, cols))
+  m.def(
+      "seq_ineq_evaluate",
+      [](SphereCollisionCst& cst, const Arr& X, std::optional<std::vector<double>> msbox) {
+        if (X.ndim() != 2 || (size_t)X.shape(1) != cst.q_dim()) throw std::invalid_argument("expected (T, dof) waypoints");
+        const size_t T = X.shape(0);
+        std::vector<double> x(X.data(), X.data() + X.size());
+        SeqCsc r = seq_ineq_evaluate(cst, x, T, msbox ? &*msbox : nullptr);
+        return py::make_tuple(Arr(r.c.size(), r.c.data()), py::array_t<int32_t>(r.colptr.size(), r.colptr.data()),
+                              py::array_t<int32_t>(r.rowind.size(), r.rowind.data()),
+                              Arr(r.values.size(), r.values.data()), py::make_tuple(r.rows, r.cols));
+      },
+      py::arg("cst"), py::arg("X"), py::arg("msbox") = py::none());
   m.def("smooth_cost", [](const Arr& x, size_t T, size_t dof, const std::vector<double>& w) {
     std::vector<double> g;
     double c = smooth_cost(row(x.data(), x.size()), T, dof, w, &g);

@@ -162,6 +162,7 @@ struct pmp_context {
   std::vector<int32_t> h_vorig;
   // misc
   DevBuf traj_off, traj_w2;
+  DevBuf sqp_rows, sqp_tables;  // SQP assembly: evaluate rows of every waypoint; offsets / box widths
   std::vector<int64_t> h_traj_off;
   std::vector<double> h_traj_w2;
   EdgeBufs eb;  // edge buffers of direct calls
@@ -579,7 +580,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
                       &ctx->vprims32, &ctx->vprims64, &ctx->supers, &ctx->cull, &ctx->ecull, &ctx->ecull_range, &ctx->vorig, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
                       &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
-                      &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2})
+                      &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2, &ctx->sqp_rows, &ctx->sqp_tables})
       b->release();
     ctx->eb.release();
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
@@ -1123,6 +1124,92 @@ pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t l
     e.grad = grad;
     e.whole_chunks = ctx->sched.eval_whole_chunks;
     PMP_HIP(launch_eval_units(ctx->k32, ctx->s32, X, ldx, N, e, st(ctx)));
+  });
+}
+
+}  // extern "C"
+
+namespace {
+int cst_dim_of(const pmp_context* ctx) { return (ctx->s32.n_prims > 0 ? 1 : 0) + (ctx->k32.n_pairs > 0 ? 1 : 0); }
+
+// Row, nonzero and column-pointer offsets of every trajectory's piece of the
+// SQP inequality sequence (the last entry is the total).
+void sqp_sizes(const pmp_context* ctx, const int64_t* offsets, int32_t n_traj, bool with_msbox, int64_t* row_off,
+               int64_t* nnz_off, int64_t* colptr_off) {
+  PMP_REQUIRE(ctx != nullptr && offsets != nullptr && row_off && nnz_off && colptr_off, "null argument");
+  PMP_REQUIRE(n_traj >= 0 && offsets[0] == 0, "offsets must start at 0");
+  const int64_t D = ctx->n_cols, C = cst_dim_of(ctx);
+  row_off[0] = nnz_off[0] = colptr_off[0] = 0;
+  for (int32_t i = 0; i < n_traj; ++i) {
+    const int64_t T = offsets[i + 1] - offsets[i];
+    PMP_REQUIRE(T >= 0, "offsets must not decrease");
+    const int64_t segs = T > 0 && with_msbox ? T - 1 : 0;
+    row_off[i + 1] = row_off[i] + C * T + 2 * D * segs;
+    nnz_off[i + 1] = nnz_off[i] + D * T * C + 4 * D * segs;
+    colptr_off[i + 1] = colptr_off[i] + D * T + 1;
+    PMP_REQUIRE(nnz_off[i + 1] - nnz_off[i] < ((int64_t)1 << 31), "trajectory too long for int32 indices");
+  }
+}
+}  // namespace
+
+extern "C" {
+
+pmp_status pmp_traj_sqp_sizes(pmp_context* ctx, const int64_t* offsets, int32_t n_traj, int32_t with_msbox,
+                              int64_t* row_off, int64_t* nnz_off, int64_t* colptr_off) {
+  return guarded([&] { sqp_sizes(ctx, offsets, n_traj, with_msbox != 0, row_off, nnz_off, colptr_off); });
+}
+
+pmp_status pmp_traj_sqp_ineq(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
+                             int32_t n_traj, const double* msbox, double* c, int32_t* colptr, int32_t* rowind,
+                             double* values) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(X, ldx, N);
+    PMP_REQUIRE(n_traj >= 0 && offsets != nullptr, "missing trajectory offsets");
+    PMP_REQUIRE(offsets[0] == 0 && offsets[n_traj] == N, "offsets must start at 0 and end at N");
+    for (int32_t i = 0; i < n_traj; ++i) PMP_REQUIRE(offsets[i + 1] >= offsets[i], "offsets must not decrease");
+    PMP_REQUIRE(n_traj == 0 || colptr, "null output");
+    PMP_REQUIRE(N == 0 || (c && rowind && values), "null output");
+    if (n_traj == 0) return;
+    set_device(ctx);
+    const int D = ctx->n_cols, C = cst_dim_of(ctx);
+    std::vector<int64_t> row_off(n_traj + 1), nnz_off(n_traj + 1), cp_off(n_traj + 1);
+    sqp_sizes(ctx, offsets, n_traj, msbox != nullptr, row_off.data(), nnz_off.data(), cp_off.data());
+    // evaluate rows of every waypoint (the plain evaluate launch over N)
+    float* rows = (float*)ctx->sqp_rows.get(sizeof(float) * std::max<int64_t>(1, (int64_t)C * (D + 1) * N));
+    if (N > 0 && C > 0) {
+      EvalArgs e = eval_workspace(ctx, N);
+      e.vals = rows;
+      e.jac = rows + (int64_t)C * N;
+      e.whole_chunks = ctx->sched.eval_whole_chunks;
+      PMP_HIP(launch_eval_units(ctx->k32, ctx->s32, X, ldx, N, e, st(ctx)));
+    }
+    // offsets, row / nnz offsets and box widths in one small table
+    std::vector<int64_t> tab;
+    tab.insert(tab.end(), offsets, offsets + n_traj + 1);
+    tab.insert(tab.end(), row_off.begin(), row_off.end());
+    tab.insert(tab.end(), nnz_off.begin(), nnz_off.end());
+    const size_t box_at = tab.size();
+    tab.resize(box_at + D);
+    if (msbox) std::memcpy(&tab[box_at], msbox, sizeof(double) * D);
+    const int64_t* dt = ctx->sqp_tables.upload(tab, st(ctx));
+    SqpArgs a{};
+    a.X = X;
+    a.ldx = ldx;
+    a.N = N;
+    a.D = D;
+    a.C = C;
+    a.offsets = dt;
+    a.n_traj = n_traj;
+    a.msbox = msbox ? (const double*)(dt + box_at) : nullptr;
+    a.vals = rows;
+    a.jac = rows + (int64_t)C * N;
+    a.row_off = dt + (n_traj + 1);
+    a.nnz_off = dt + 2 * (n_traj + 1);
+    a.c = c;
+    a.colptr = colptr;
+    a.rowind = rowind;
+    a.values = values;
+    PMP_HIP(launch_sqp_ineq(a, st(ctx)));
   });
 }
 

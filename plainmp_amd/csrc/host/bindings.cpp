@@ -353,6 +353,46 @@ PYBIND11_MODULE(_native, m) {
           },
           py::arg("X"), py::arg("offsets"), py::arg("weights") = std::vector<double>{},
           "Per-waypoint evaluate rows, smoothness cost per trajectory and its gradient, one launch.")
+      .def(
+          "traj_sqp_ineq",
+          [](SphereCollisionCst& c, const DArr& X, const std::vector<int64_t>& offsets,
+             const std::vector<double>& msbox) {
+            int64_t N;
+            const double* x = rows(X, c.q_dim(), N);
+            SphereCollisionCst::SqpPieces p;
+            {
+              py::gil_scoped_release rel;
+              p = c.traj_sqp_ineq(x, N, offsets, msbox);
+            }
+            auto arr = [](const auto& v) {
+              using T = typename std::decay_t<decltype(v)>::value_type;
+              py::array_t<T> a((py::ssize_t)v.size());
+              std::copy(v.begin(), v.end(), a.mutable_data());
+              return a;
+            };
+            return py::make_tuple(arr(p.c), arr(p.colptr), arr(p.rowind), arr(p.values), arr(p.row_off),
+                                  arr(p.nnz_off), arr(p.cp_off));
+          },
+          py::arg("X"), py::arg("offsets"), py::arg("msbox") = std::vector<double>{},
+          "SQP inequality sequence of every trajectory: (c, colptr, rowind, values, row_off, nnz_off, colptr_off).")
+      .def("traj_sqp_sizes",
+           [](SphereCollisionCst& c, const std::vector<int64_t>& offsets, bool with_msbox) {
+             const int32_t n = (int32_t)offsets.size() - 1;
+             if (n < 0) throw std::invalid_argument("offsets must hold n_traj + 1 entries");
+             std::vector<int64_t> r(n + 1), z(n + 1), p(n + 1);
+             check(pmp_traj_sqp_sizes(c.device_context(nullptr), offsets.data(), n, with_msbox ? 1 : 0, r.data(),
+                                      z.data(), p.data()));
+             return py::make_tuple(r, z, p);
+           })
+      .def("_dev_traj_sqp_ineq",
+           [](SphereCollisionCst& c, uintptr_t X, int64_t ldx, int64_t N, const std::vector<int64_t>& offsets,
+              const std::vector<double>& msbox, uintptr_t cv, uintptr_t colptr, uintptr_t rowind, uintptr_t values,
+              uintptr_t stream) {
+             pmp_context* ctx = c.device_context((void*)stream);
+             check(pmp_traj_sqp_ineq(ctx, (const double*)X, ldx, N, offsets.data(), (int32_t)offsets.size() - 1,
+                                     msbox.empty() ? nullptr : msbox.data(), (double*)cv, (int32_t*)colptr,
+                                     (int32_t*)rowind, (double*)values));
+           })
       .def("_dev_traj_cost_grad",
            [](SphereCollisionCst& c, uintptr_t X, int64_t ldx, int64_t N, const std::vector<int64_t>& offsets,
               const std::vector<double>& weights, uintptr_t vals, uintptr_t jac, uintptr_t cost, uintptr_t grad,

@@ -252,6 +252,46 @@ class SphereCollisionCst {
       for (int j = 0; j < D; ++j) grad[(size_t)n * D + j] = hg[(size_t)j * N + n];
     }
   }
+  // The SQP inequality sequence of every trajectory as CSC pieces
+  // (pmp_traj_sqp_ineq); offsets of each piece in row_off / nnz_off / cp_off.
+  struct SqpPieces {
+    std::vector<double> c, values;
+    std::vector<int32_t> colptr, rowind;
+    std::vector<int64_t> row_off, nnz_off, cp_off;
+  };
+  SqpPieces traj_sqp_ineq(const double* X, int64_t N, const std::vector<int64_t>& offsets,
+                          const std::vector<double>& msbox) {
+    sync_batch_tables();
+    const int D = (int)q_dim();
+    if (offsets.empty()) throw std::invalid_argument("offsets must hold n_traj + 1 entries");
+    if (!msbox.empty() && (int)msbox.size() != D) throw std::invalid_argument("msbox must have q_dim entries");
+    const int32_t n_traj = (int32_t)offsets.size() - 1;
+    SqpPieces p;
+    p.row_off.resize(n_traj + 1);
+    p.nnz_off.resize(n_traj + 1);
+    p.cp_off.resize(n_traj + 1);
+    check(pmp_traj_sqp_sizes(batch_.ctx(), offsets.data(), n_traj, msbox.empty() ? 0 : 1, p.row_off.data(),
+                             p.nnz_off.data(), p.cp_off.data()));
+    const int64_t R = p.row_off[n_traj], Z = p.nnz_off[n_traj], P = p.cp_off[n_traj];
+    double* dx = N ? batch_.upload_soa(0, X, N, D) : nullptr;
+    double* dc = (double*)batch_.buf(1, sizeof(double) * std::max<int64_t>(R, 1));
+    int32_t* dp = (int32_t*)batch_.buf(2, sizeof(int32_t) * std::max<int64_t>(P, 1));
+    int32_t* dr = (int32_t*)batch_.buf(3, sizeof(int32_t) * std::max<int64_t>(Z, 1));
+    double* dv = (double*)batch_.buf(4, sizeof(double) * std::max<int64_t>(Z, 1));
+    check(pmp_traj_sqp_ineq(batch_.ctx(), dx, N, N, offsets.data(), n_traj, msbox.empty() ? nullptr : msbox.data(),
+                            dc, dp, dr, dv));
+    p.c.resize(R);
+    p.colptr.resize(P);
+    p.rowind.resize(Z);
+    p.values.resize(Z);
+    if (R) batch_.d2h(p.c.data(), dc, sizeof(double) * R);
+    if (P) batch_.d2h(p.colptr.data(), dp, sizeof(int32_t) * P);
+    if (Z) {
+      batch_.d2h(p.rowind.data(), dr, sizeof(int32_t) * Z);
+      batch_.d2h(p.values.data(), dv, sizeof(double) * Z);
+    }
+    return p;
+  }
   // (B, S) per-sphere minimum clearance
   std::vector<float> sphere_clearance_batch(const double* Q, int64_t B, bool accurate) {
     sync_batch_tables();

@@ -1550,6 +1550,80 @@ __global__ void fill_counts_kernel(int32_t* p, int32_t v, int64_t n) {
   const int64_t i = gid();
   if (i < n) p[i] = v;
 }
+// SequentialCst::evaluate (constraints/sequential_constraint.cpp:63-127) of
+// the SQP inequality sequence (nlp_solver/sqp_based_solver.py:48-71), one
+// thread per column (waypoint n, joint d) of its trajectory's matrix.  Rows:
+// the collision constraint's C rows of every waypoint (t-major), then the
+// motion-step box pairs q_t - q_{t+1} + w, q_{t+1} - q_t + w of every segment.
+// Column (t, d) holds, in row order, the C collision entries J + 1e-12, the
+// (-1, +1) of segment t - 1 and the (+1, -1) of segment t, so its position in
+// the compressed arrays follows from (t, d) alone.  Threads past the columns
+// close the column pointers of empty trajectories.
+__global__ void __launch_bounds__(256) sqp_ineq_kernel(SqpArgs a) {
+  const int64_t i = gid();
+  const int D = a.D, C = a.C;
+  if (i >= a.N * D) {
+    const int64_t tr = i - a.N * D;
+    if (tr < a.n_traj && a.offsets[tr + 1] == a.offsets[tr]) a.colptr[D * a.offsets[tr] + tr] = 0;
+    return;
+  }
+  const int64_t n = i / D;
+  const int d = (int)(i % D);
+  int lo = 0, hi = a.n_traj;  // offsets[lo] <= n < offsets[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.offsets[mid] <= n) lo = mid;
+    else hi = mid;
+  }
+  const int64_t b0 = a.offsets[lo], T = a.offsets[lo + 1] - b0, t = n - b0;
+  const bool box = a.msbox != nullptr;
+  const int64_t j = t * D + d;
+  // box entries of the columns of the waypoints before t, and of each column of t
+  const int64_t e_prev = box ? 2 * (t > 0 ? t - 1 : 0) + 2 * (t < T - 1 ? t : T - 1) : 0;
+  const int et = box ? 2 * (t > 0) + 2 * (t < T - 1) : 0;
+  const int64_t before = j * C + (int64_t)D * e_prev + (int64_t)d * et;
+  int32_t* cp = a.colptr + D * b0 + lo;
+  cp[j] = (int32_t)before;
+  if (j == D * T - 1) cp[D * T] = (int32_t)(before + C + et);
+  int32_t* ri = a.rowind + a.nnz_off[lo] + before;
+  double* va = a.values + a.nnz_off[lo] + before;
+  for (int c = 0; c < C; ++c) {
+    ri[c] = (int32_t)(t * C + c);
+    va[c] = (double)a.jac[((int64_t)c * D + d) * a.N + n] + 1e-12;
+  }
+  double* cv = a.c + a.row_off[lo];
+  if (d == 0)
+    for (int c = 0; c < C; ++c) cv[t * C + c] = (double)a.vals[(int64_t)c * a.N + n];
+  if (box) {
+    const int64_t m0 = C * T;
+    int k = C;
+    if (t > 0) {
+      const int64_t r = m0 + (t - 1) * 2 * D;
+      ri[k] = (int32_t)(r + d);
+      va[k] = -1.0;
+      ri[k + 1] = (int32_t)(r + D + d);
+      va[k + 1] = 1.0;
+      k += 2;
+    }
+    if (t < T - 1) {
+      const int64_t r = m0 + t * 2 * D;
+      ri[k] = (int32_t)(r + d);
+      va[k] = 1.0;
+      ri[k + 1] = (int32_t)(r + D + d);
+      va[k + 1] = -1.0;
+      const double q1 = a.X[(int64_t)d * a.ldx + n], q2 = a.X[(int64_t)d * a.ldx + n + 1], w = a.msbox[d];
+      cv[r + d] = q1 - q2 + w;
+      cv[r + D + d] = q2 - q1 + w;
+    }
+  }
+}
+
+hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st) {
+  const int64_t n = a.N * a.D + a.n_traj;
+  if (n > 0) hipLaunchKernelGGL(sqp_ineq_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(fill_counts_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
   return hipGetLastError();

@@ -35,6 +35,25 @@ struct EdgeArgs {
   int32_t* n_checked;
 };
 
+// The SQP inequality sequence as CSC per trajectory from evaluate rows
+// (vals [C][N], jac [C][D][N]); offsets, row_off, nnz_off, msbox on the device.
+struct SqpArgs {
+  const double* X;
+  int64_t ldx, N;
+  int D, C;
+  const int64_t* offsets;  // [n_traj + 1]
+  int n_traj;
+  const double* msbox;     // [D] or nullptr
+  const float* vals;
+  const float* jac;
+  const int64_t* row_off;  // [n_traj + 1]
+  const int64_t* nnz_off;  // [n_traj + 1]
+  double* c;
+  int32_t* colptr;         // trajectory i from D * offsets[i] + i
+  int32_t* rowind;
+  double* values;
+};
+hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st);
 // p[0, n) = v by a kernel (safe under stream capture, unlike a memset node).
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int taylor, float* out,

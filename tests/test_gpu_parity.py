@@ -484,6 +484,47 @@ def test_evaluate_and_trajectory_cloud():
         assert abs(cost[i] - cref) <= 1e-5 * max(1.0, abs(cref))
 
 
+@pytest.mark.parametrize("with_box", [True, False])
+def test_sqp_ineq_csc_dual_arm(with_box):
+    """The SQP inequality sequence (collision rows of every waypoint + the
+    motion-step box) of many trajectories, assembled on the GPU as CSC, against
+    the oracle's SequentialCst restatement: the sparsity structure (column
+    pointers and row indices) exactly, box rows and entries exactly, collision
+    values within 5e-6 and collision entries within 1e-4 (relative) where the
+    waypoint's argmin pair is unambiguous.  Empty and one-waypoint
+    trajectories are included."""
+    from plainmp_amd.constraint import sqp_ineq_matrices
+
+    wl = synth.workload("traj_dual_arm", scale=0.004)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    off0, X0 = list(wl.traj_offsets), wl.traj_X
+    X = np.concatenate([X0, X0[:1]])
+    off = [0] + [o for o in off0[1:]] + [off0[-1], off0[-1] + 1]  # then an empty and a one-waypoint trajectory
+    D = X.shape[1]
+    msbox = list(np.linspace(0.05, 0.2, D)) if with_box else None
+    got = sqp_ineq_matrices(pc, X, off, msbox)
+    pd = oc.pair_clearance_batch(X, True, 8)
+    clear_wp = _second_gap(pd) > 1e-5
+    C = pc.cst_dim()
+    assert len(got) == len(off) - 1
+    for i, (c, J) in enumerate(got):
+        a, b = off[i], off[i + 1]
+        T = b - a
+        cr, cpr, rir, var, (rows, cols) = _oracle.seq_ineq_evaluate(oc, X[a:b].reshape(T, D), msbox)
+        assert J.shape == (rows, cols) and len(c) == rows
+        assert np.array_equal(J.indptr, cpr) and np.array_equal(J.indices, rir)
+        nc = C * T
+        assert np.abs(c[:nc] - cr[:nc]).max(initial=0) <= 5e-6
+        assert np.array_equal(c[nc:], cr[nc:])
+        box = J.indices >= nc
+        assert np.array_equal(J.data[box], var[box])
+        col_wp = np.repeat(np.arange(T), D)  # waypoint of each column
+        entry_wp = np.repeat(col_wp, np.diff(J.indptr))
+        sel = (~box) & clear_wp[a + entry_wp]
+        assert (np.abs(J.data[sel] - var[sel]) <= 1e-4 * np.maximum(np.abs(var[sel]), 1.0)).all()
+    _report("sqp_csc", box=with_box, trajectories=len(got), waypoints=len(X), nnz=sum(J.nnz for _, J in got))
+
+
 def test_trajectory_edge_cases():
     """Empty, one- and two-waypoint trajectories have zero smoothness cost and
     gradient; bad offsets are rejected as ValueError."""

@@ -434,3 +434,70 @@ def test_smoothness_cost_matches_matrix_form():
     cost, grad = _oracle.smooth_cost(x, T, dof, w)
     assert cost == pytest.approx(x @ P @ x)
     assert np.allclose(grad, 2 * P @ x)
+
+
+def _csc_dense(colptr, rowind, values, shape):
+    M = np.zeros(shape)
+    for j in range(shape[1]):
+        for k in range(colptr[j], colptr[j + 1]):
+            M[rowind[k], j] = values[k]
+    return M
+
+
+def test_seq_ineq_known_answers_and_structure():
+    # SequentialCst with the collision constraint at every waypoint and the
+    # motion-step box (tests/test_constraints.py:273-318): row count
+    # cst_dim * T + 2 dof (T - 1); msbox rows all >= 0 for steps of half the
+    # box, and exactly half of them negative for steps of 1.1 boxes; the
+    # sparsity pattern does not change between evaluations.
+    wl = synth.workload("grad_mixed32", scale=1 / 256)
+    oc = oracle_cst(wl)
+    D, C, T = oc.q_dim(), oc.cst_dim(), 4
+    msbox = list(np.linspace(0.1, 0.8, D))
+    q1 = wl.Q[0]
+    for step, check in ((0.5, "ok"), (1.1, "ng")):
+        X = np.array([q1 + k * step * np.array(msbox) for k in range(T)])
+        c, cp, ri, va, (rows, cols) = _oracle.seq_ineq_evaluate(oc, X, msbox)
+        assert rows == C * T + 2 * D * (T - 1) and cols == D * T and len(c) == rows
+        tail = c[-2 * D * (T - 1):]
+        if check == "ok":
+            assert np.all(tail >= 0)
+        else:
+            assert np.sum(tail < 0) == D * (T - 1) and np.sum(tail > 0) == D * (T - 1)
+    X1 = wl.Q[:T]
+    X2 = wl.Q[T:2 * T]
+    a = _oracle.seq_ineq_evaluate(oc, X1, msbox)
+    b = _oracle.seq_ineq_evaluate(oc, X2, msbox)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    # without the box: only the collision rows, C nonzeros per column
+    c, cp, ri, va, (rows, cols) = _oracle.seq_ineq_evaluate(oc, X1, None)
+    assert rows == C * T and np.array_equal(np.diff(cp), np.full(D * T, C))
+
+
+def test_seq_ineq_jacobian_vs_central_fd():
+    # the CSC Jacobian against FP64 central differences of the stacked values
+    # (collision rows use the reference's forward-difference SDF gradient, so
+    # compare where the argmin pair is stable)
+    wl = synth.workload("grad_mixed32", scale=1 / 256)
+    oc = oracle_cst(wl)
+    D, T = oc.q_dim(), 3
+    msbox = [0.2] * D
+    X = wl.Q[:T]
+    c, cp, ri, va, shape = _oracle.seq_ineq_evaluate(oc, X, msbox)
+    J = _csc_dense(cp, ri, va, shape)
+    h = 1e-6
+    x = X.ravel()
+    fd = np.empty(shape)
+    fd4 = np.empty(shape)
+    for j in range(len(x)):
+        e = np.zeros(len(x))
+        e[j] = 1.0
+        cp_ = _oracle.seq_ineq_evaluate(oc, (x + h * e).reshape(T, D), msbox)[0]
+        cm_ = _oracle.seq_ineq_evaluate(oc, (x - h * e).reshape(T, D), msbox)[0]
+        cp4 = _oracle.seq_ineq_evaluate(oc, (x + 4 * h * e).reshape(T, D), msbox)[0]
+        cm4 = _oracle.seq_ineq_evaluate(oc, (x - 4 * h * e).reshape(T, D), msbox)[0]
+        fd[:, j] = (cp_ - cm_) / (2 * h)
+        fd4[:, j] = (cp4 - cm4) / (8 * h)
+    smooth = np.abs(fd - fd4) <= 1e-6
+    assert smooth.mean() > 0.9
+    assert np.all(np.abs(J - fd)[smooth] <= 1e-4 * np.maximum(np.abs(fd[smooth]), 1.0))
