@@ -249,6 +249,14 @@ PYBIND11_MODULE(_native, m) {
            py::arg("only_self_collision") = false)
       .def_readonly("parent_link_name", &SphereAttachmentSpec::parent_link_name)
       .def_property_readonly("radii", [](const SphereAttachmentSpec& s) { return s.radii; })
+      .def_property_readonly("relative_positions",
+                             [](const SphereAttachmentSpec& s) {  // (3, k), as the reference exposes it
+                               const py::ssize_t k = (py::ssize_t)s.radii.size();
+                               DArr a({(py::ssize_t)3, k});
+                               for (py::ssize_t j = 0; j < k; ++j)
+                                 for (int c = 0; c < 3; ++c) a.mutable_data()[c * k + j] = s.relative_positions[3 * j + c];
+                               return a;
+                             })
       .def_readonly("only_self_collision", &SphereAttachmentSpec::only_self_collision);
 
   py::class_<SphereCollisionCst, std::shared_ptr<SphereCollisionCst>>(mc, "SphereCollisionCst")

@@ -525,6 +525,50 @@ def test_sqp_ineq_csc_dual_arm(with_box):
     _report("sqp_csc", box=with_box, trajectories=len(got), waypoints=len(X), nnz=sum(J.nnz for _, J in got))
 
 
+def test_robot_spec_yaml_ingestion(tmp_path):
+    """A robot described by a plainmp YAML configuration (sphere groups in an
+    included file, only-self links, self pairs given as pairs and as group
+    pairs, a base-fixed box with a yaw) gives, through RobotSpec, a
+    constraint whose validity and evaluate values equal the oracle's for the
+    same spheres, pairs and fixed primitive."""
+    import yaml
+
+    from plainmp_amd import psdf
+    from plainmp_amd.robot_spec import RobotSpec, rpy_matrix
+    from plainmp_amd.synth import build_sdf
+
+    wl = synth.workload("traj_dual_arm", scale=0.004)
+    r = wl.robot
+    spheres = {"collision_spheres": {
+        ln: {"spheres": [[float(v) for v in c] + [float(rr)] for c, rr in zip(pos, rad)]}
+        for ln, pos, rad, _ in r.sphere_specs}}
+    pairs = [list(p) for p in r.self_pairs]
+    # placed where it changes answers, and where the opposite yaw would change others
+    box = {"type": "box", "extents": [0.3, 0.8, 0.4], "position": [0.35, 0.0, 0.4], "rotation": [0.0, 0.0, 0.4]}
+    conf = {"control_joint_names": list(r.joint_names),
+            "only_self_collision_links": [ln for ln, _, _, only in r.sphere_specs if only],
+            "self_collision_pairs": pairs[:2],
+            "self_collision_group_pairs": [{"group1": [a], "group2": [b]} for a, b in pairs[2:]],
+            "self_body_collision_primitives": [box],
+            "include": ["spheres.yaml"]}
+    (tmp_path / "spheres.yaml").write_text(yaml.safe_dump(spheres))
+    (tmp_path / "robot.yaml").write_text(yaml.safe_dump(conf))
+    cst = RobotSpec(tmp_path / "robot.yaml", urdf=r.urdf).create_collision_const()
+    cst.set_sdf(psdf.UnionSDF([build_sdf(p) for p in wl.prims]))
+    fixed = {"type": "box", "width": box["extents"], "position": box["position"], "rotation": rpy_matrix(0.4, 0.0, 0.0)}
+    oc = oracle_cst(wl, fixed_prims=[fixed])
+    Q = wl.traj_X[:256]
+    ref = oc.is_valid_batch(Q, 8).astype(bool)
+    got = cst.is_valid_batch(Q)
+    vref, _ = oc.evaluate_batch(Q, 8)
+    vgot, _ = cst.evaluate_batch(Q)
+    _report("robot_spec", B=len(Q), valid_frac=ref.mean(), val_err=np.abs(vgot - vref).max())
+    assert 0.0 < ref.mean() < 1.0
+    assert (oracle_cst(wl).is_valid_batch(Q, 8).astype(bool) != ref).any()  # the box matters
+    assert np.array_equal(got, ref)
+    assert np.abs(vgot - vref).max() <= 5e-6
+
+
 def test_trajectory_edge_cases():
     """Empty, one- and two-waypoint trajectories have zero smoothness cost and
     gradient; bad offsets are rejected as ValueError."""
