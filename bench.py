@@ -341,6 +341,23 @@ def _extra(torch, stream, dev):
     t0 = time.perf_counter()
     S = mv.shortcut_path(P, 1024, 4, 0)
     t_short = time.perf_counter() - t0
+    # batched multi-start position IK on the two-bar scene: 256 restarts x 60
+    # iterations (link pose, link Jacobian and evaluate launches per iteration)
+    from plainmp_amd.ik import solve_ik_batched
+
+    wl_ik = synth.workload("plain_two_bars")
+    cst_ik = product_cst(wl_ik)
+    link = wl_ik.robot.sphere_specs[-1][0]
+    target = [0.4, 0.2, 0.5]
+    solve_ik_batched(cst_ik, link, target, wl_ik.robot.lower, wl_ik.robot.upper, n_restarts=256, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ik = solve_ik_batched(cst_ik, link, target, wl_ik.robot.lower, wl_ik.robot.upper, n_restarts=256, seed=0)
+    torch.cuda.synchronize()
+    t_ik = time.perf_counter() - t0
+    res["ik_batched_restarts"] = {"restarts": 256, "iterations": 60, "ms": t_ik * 1e3, "success": bool(ik.success),
+                                  "first_success_restart": int(ik.n_trial),
+                                  "restart_iterations_per_s": 256 * 60 / t_ik}
     res["path_check_and_shortcut"] = {"states": len(P), "check_path_ms": t_path * 1e3,
                                       "segments_per_s": (len(P) - 1) / t_path,
                                       "valid_segment_frac": float(seg.mean()), "shortcut_ms": t_short * 1e3,

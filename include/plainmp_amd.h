@@ -159,6 +159,16 @@ pmp_status pmp_set_schedule(pmp_context* ctx, const pmp_schedule* schedule);
 pmp_status pmp_fk_spheres_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
                                 float* centers);
 
+/* Jacobian of a point fixed to one link with respect to q, rotation
+ * ignored, with exact FK: jac[(k * q_dim + j) * B + b] for k < 3 (FP32).
+ * link: the link's node index (get_link_ids numbering, as pmp_link_poses_batch);
+ * point: HOST [3] in the link frame, NULL for the link origin.  Replaces
+ * KinematicModel::get_jacobian / get_attached_point_jacobian with
+ * RotationType::IGNORE (kinematics/algorithm.cpp:212-433), the Jacobian of
+ * LinkPoseCst for a position target (constraints/primitive_link_pose.cpp:32-44). */
+pmp_status pmp_link_jacobian_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t link,
+                                   const double* point, float* jac);
+
 /* Link poses of all nodes, poses[(n*7 + k) * B + b] as x,y,z,qx,qy,qz,qw
  * (FP64).  Replaces set_joint_angles + get_link_pose
  * (kinematics/algorithm.cpp:88-107, kinematics/kinematics.hpp:131-136). */

@@ -1031,6 +1031,19 @@ pmp_status pmp_link_poses_batch(pmp_context* ctx, const double* q, int64_t ldq, 
   });
 }
 
+pmp_status pmp_link_jacobian_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t link,
+                                   const double* point, float* jac) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || jac, "null output");
+    PMP_REQUIRE(link >= 0 && link < ctx->n_nodes, "link index out of range");
+    if (B == 0) return;
+    set_device(ctx);
+    const float off[3] = {point ? (float)point[0] : 0.f, point ? (float)point[1] : 0.f, point ? (float)point[2] : 0.f};
+    PMP_HIP(launch_link_jacobian(ctx->k32, q, ldq, B, link, off, jac, st(ctx)));
+  });
+}
+
 pmp_status pmp_sphere_clearance_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
                                       float* clearance) {
   return guarded([&] {

@@ -499,6 +499,36 @@ PYBIND11_MODULE(_native, m) {
              check(pmp_sphere_clearance_batch(c.device_context((void*)stream), (const double*)q, ldq, B, accurate,
                                               (float*)out));
            })
+      .def(
+          "link_jacobian_batch",
+          [](SphereCollisionCst& c, const DArr& Q, const std::string& link, const std::vector<double>& point) {
+            int64_t B;
+            const double* q = rows(Q, c.q_dim(), B);
+            const int32_t id = (int32_t)c.get_kin()->get_link_ids({link})[0];
+            std::vector<float> j;
+            {
+              py::gil_scoped_release rel;
+              j = c.link_jacobian_batch(q, B, id, point);
+            }
+            py::array_t<float> a({(py::ssize_t)B, (py::ssize_t)3, (py::ssize_t)c.q_dim()});
+            std::copy(j.begin(), j.end(), a.mutable_data());
+            return a;
+          },
+          py::arg("Q"), py::arg("link_name"), py::arg("point") = std::vector<double>{},
+          "(B, 3, q_dim) Jacobian of a point fixed to a link, rotation ignored (exact FK).")
+      .def("_dev_link_poses",
+           [](SphereCollisionCst& c, uintptr_t q, int64_t ldq, int64_t B, int accurate, uintptr_t out,
+              uintptr_t stream) {
+             check(pmp_link_poses_batch(c.device_context((void*)stream), (const double*)q, ldq, B, accurate,
+                                        (double*)out));
+           })
+      .def("_dev_link_jacobian",
+           [](SphereCollisionCst& c, uintptr_t q, int64_t ldq, int64_t B, int link, const std::vector<double>& point,
+              uintptr_t jac, uintptr_t stream) {
+             if (!point.empty() && point.size() != 3) throw std::invalid_argument("point must have 3 entries");
+             check(pmp_link_jacobian_batch(c.device_context((void*)stream), (const double*)q, ldq, B, link,
+                                           point.empty() ? nullptr : point.data(), (float*)jac));
+           })
       .def("_dev_evaluate",
            [](SphereCollisionCst& c, uintptr_t q, int64_t ldq, int64_t B, uintptr_t vals, uintptr_t jac,
               uintptr_t stream) {

@@ -292,6 +292,23 @@ class SphereCollisionCst {
     }
     return p;
   }
+  // (B, 3, q_dim) Jacobian of a point fixed to a link (rotation ignored)
+  std::vector<float> link_jacobian_batch(const double* Q, int64_t B, int32_t link, const std::vector<double>& point) {
+    sync_batch_tables();
+    const int D = (int)q_dim();
+    if (!point.empty() && point.size() != 3) throw std::invalid_argument("point must have 3 entries");
+    std::vector<float> out((size_t)B * 3 * D);
+    if (B == 0) return out;
+    double* dq = batch_.upload_soa(0, Q, B, D);
+    float* dj = (float*)batch_.buf(1, sizeof(float) * B * 3 * D);
+    check(pmp_link_jacobian_batch(batch_.ctx(), dq, B, B, link, point.empty() ? nullptr : point.data(), dj));
+    std::vector<float> h((size_t)B * 3 * D);
+    batch_.d2h(h.data(), dj, h.size() * sizeof(float));
+    for (int64_t b = 0; b < B; ++b)
+      for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < D; ++j) out[((size_t)b * 3 + k) * D + j] = h[((size_t)k * D + j) * B + b];
+    return out;
+  }
   // (B, S) per-sphere minimum clearance
   std::vector<float> sphere_clearance_batch(const double* Q, int64_t B, bool accurate) {
     sync_batch_tables();

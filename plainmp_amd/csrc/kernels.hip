@@ -1320,6 +1320,56 @@ __global__ void __launch_bounds__(256) pair_kernel(KinArgs<float> ka, SceneArgs<
   run_fk(ka, qs, false, v);
 }
 
+// Jacobian of a point fixed to one link (rotation ignored): the joint frames
+// of every node up to the link, then a_j x (p - o_j) (or a_j) for the columns
+// that move it (KinematicModel::get_jacobian / get_attached_point_jacobian,
+// kinematics/algorithm.cpp:212-433).
+template <int MAXC>
+struct LinkJacVis {
+  static constexpr bool kSelf = false;
+  const KinArgs<float>& ka;
+  Frames<MAXC> fr;
+  int target;
+  V3<float> off;
+  V3<float> p{0.f, 0.f, 0.f};
+  uint32_t mask = 0;
+  bool done = false;
+  __device__ bool alive() const { return !done; }
+  template <class QS>
+  __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
+  __device__ void node(int n, const DevNode<float>& nd, const Pose<float>& P) {
+    if (nd.q_col >= 0) fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
+    if (n == target) {
+      p = add(qrot(P.q, off), P.t);
+      mask = nd.col_mask;
+      done = true;
+    }
+  }
+  __device__ void sphere(int, V3<float>, float) {}
+  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
+};
+
+template <int MAXC>
+__global__ void __launch_bounds__(256) link_jac_kernel(KinArgs<float> ka, const double* q, int64_t ldq, int64_t B,
+                                                       int target, float ox, float oy, float oz, float* jac) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  LinkJacVis<MAXC> v{ka, {}, target, V3<float>{ox, oy, oz}};
+  run_fk(ka, qs, false, v);
+  if (!active) return;
+  const int D = ka.n_cols;
+#pragma unroll
+  for (int j = 0; j < MAXC; ++j) {
+    if (j < D) {
+      const V3<float> c = v.fr.col(j, v.mask, ka.rev_mask, v.p);
+      jac[((int64_t)0 * D + j) * B + b] = c.x;
+      jac[((int64_t)1 * D + j) * B + b] = c.y;
+      jac[((int64_t)2 * D + j) * B + b] = c.z;
+    }
+  }
+}
+
 // Reverse mode: per sphere the upstream weights of its primitive distances
 // are folded into one world-space force g = sum_p w_p grad sdf_p(c); each
 // q column then receives g . (a_j x (c - o_j)) (or g . a_j).  Each
@@ -1616,6 +1666,14 @@ __global__ void __launch_bounds__(256) sqp_ineq_kernel(SqpArgs a) {
       cv[r + D + d] = q2 - q1 + w;
     }
   }
+}
+
+hipError_t launch_link_jacobian(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int link,
+                                const float (&off)[3], float* jac, hipStream_t st) {
+  if (B > 0)
+    PMP_DISPATCH_MAXC(ka.n_cols, link_jac_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, q, ldq, B, link,
+                      off[0], off[1], off[2], jac);
+  return hipGetLastError();
 }
 
 hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st) {

@@ -54,6 +54,8 @@ struct SqpArgs {
   double* values;
 };
 hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st);
+hipError_t launch_link_jacobian(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int link,
+                                const float (&off)[3], float* jac, hipStream_t st);
 // p[0, n) = v by a kernel (safe under stream capture, unlike a memset node).
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int taylor, float* out,

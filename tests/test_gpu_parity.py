@@ -641,3 +641,73 @@ def test_autotuned_schedules_keep_results():
     assert sched.block in autotune.EDGE_BLOCKS and sched.waves_per_cu in autotune.EDGE_WAVES_PER_CU
     got, n_got = mv.check_motions(wl.S1, wl.S2)
     assert np.array_equal(got, ref.astype(bool)) and np.array_equal(n_got, n_ref)
+
+
+def _oracle_link_position(wl, link_name, Q, point=(0.0, 0.0, 0.0)):
+    from oracle import urdf_parse
+
+    kin = urdf_parse.kinematic_model(wl.robot.urdf)
+    ids = kin.get_joint_ids(list(wl.robot.joint_names))
+    lid = kin.get_link_ids([link_name])[0]
+    out = np.empty((len(Q), 3))
+    for b, q in enumerate(Q):
+        kin.set_joint_positions(ids, list(q), True)
+        x, y, z, qx, qy, qz, qw = kin.get_link_pose(lid)
+        R = np.array([[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+                      [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+                      [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]])
+        out[b] = np.array([x, y, z]) + R @ np.asarray(point)
+    return out
+
+
+@pytest.mark.parametrize("point", [(0.0, 0.0, 0.0), (0.05, -0.02, 0.1)])
+def test_link_jacobian_vs_central_fd(point):
+    """Jacobian of a point fixed to the last link (LinkPoseCst's position
+    Jacobian, get_jacobian with rotation ignored) against FP64 central
+    differences of the oracle's exact FK."""
+    wl = synth.workload("plain_two_bars", scale=1 / 32)
+    pc = product_cst(wl)
+    link = wl.robot.sphere_specs[-1][0]
+    Q = wl.Q
+    J = pc.link_jacobian_batch(Q, link, list(point))
+    D = Q.shape[1]
+    h = 1e-6
+    fd = np.empty((len(Q), 3, D))
+    for j in range(D):
+        e = np.zeros(D)
+        e[j] = h
+        fd[:, :, j] = (_oracle_link_position(wl, link, Q + e, point) - _oracle_link_position(wl, link, Q - e, point)) / (2 * h)
+    err = np.abs(J - fd).max()
+    _report("link_jacobian", B=len(Q), point=point, max_err=err)
+    assert err <= 1e-5
+
+
+def test_ik_batched_restarts():
+    """Batched multi-start position IK (solve_ik's acceptance test: squared
+    position error <= 1e-6 and is_valid) for targets reached by valid
+    configurations of the two-bar scene; the answer is checked with the
+    oracle's exact FK and validity, and an unreachable target fails."""
+    from plainmp_amd.ik import solve_ik_batched
+
+    wl = synth.workload("plain_two_bars", scale=1 / 16)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    link = wl.robot.sphere_specs[-1][0]
+    ok = oc.is_valid_batch(wl.Q, 8).astype(bool)
+    lb, ub = wl.robot.lower, wl.robot.upper
+    solved = 0
+    for q_star in wl.Q[ok][:4]:
+        target = _oracle_link_position(wl, link, q_star[None])[0]
+        res = solve_ik_batched(pc, link, target, lb, ub, n_restarts=128, seed=1)
+        if not res.success:
+            continue
+        solved += 1
+        err = _oracle_link_position(wl, link, res.q[None])[0] - target
+        assert err @ err <= 1e-6
+        assert bool(oc.is_valid_batch(res.q[None], 1)[0])
+        assert np.all(res.q >= lb - 1e-12) and np.all(res.q <= ub + 1e-12)
+        res2 = solve_ik_batched(pc, link, target, lb, ub, n_restarts=128, seed=1)
+        assert res2.n_trial == res.n_trial and np.array_equal(res2.q, res.q)
+    _report("ik", solved=solved, of=4)
+    assert solved >= 3
+    far = solve_ik_batched(pc, link, [5.0, 5.0, 5.0], lb, ub, n_restarts=32, seed=0)
+    assert not far.success
