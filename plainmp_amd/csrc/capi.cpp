@@ -1284,7 +1284,7 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
   ea.seq = (const uint8_t*)ctx->seq.ptr;
   // The kernels leave the counters at zero after every launch; a new buffer
   // starts there.
-  const bool fresh = eb.counter.ptr == nullptr;
+  const bool fresh = eb.counter.bytes < sizeof(int32_t) * kNumEdgeCounters;  // new (or grown) buffer
   ea.counter = (int32_t*)eb.counter.get(sizeof(int32_t) * kNumEdgeCounters);
   if (fresh) PMP_HIP(launch_fill_i32(ea.counter, 0, kNumEdgeCounters, st(ctx)));
   const int64_t Ea = std::max<int64_t>(E, 1);

@@ -702,8 +702,8 @@ __device__ __forceinline__ int wave_min(int v) {
 // that can matter (all of them, or those before the first certain collision)
 // and that collision's index; the FP64 pass decides from them.
 template <bool kCloud>
-__global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, EdgeArgs ea,
-                                                    float band) {
+__global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, KinArgs<double> k64,
+                                                    SceneArgs<double> s64, EdgeArgs ea, float band) {
   const int lane = threadIdx.x & 63;
   const bool taylor = k32.base_type == PMP_BASE_FIXED;
   for (;;) {
@@ -726,6 +726,7 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
         const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
         const int64_t n_wp = n_test - 1;  // s2 plus the interior waypoints
         bool unc = false;                 // an in-band state that matters seen so far
+        int qk = 0, qb = -1, qp = 0;      // queue entry `lane` (lanes < kPendStates) in registers
         for (int64_t base = 0; base < n_wp; base += 64) {
           const int64_t k = base + lane;
           const bool has = k < n_wp;
@@ -740,11 +741,26 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
             unc = true;
             const int cnt = __popcll(m);
             if (n_pend >= 0 && n_pend + cnt <= kPendStates) {
+              const int pb = v.n_inb == 1 ? v.inb_block : -1;
               if (in_band) {
                 const int64_t slot = (int64_t)e * kPendStates + n_pend + __popcll(m & ((1ull << lane) - 1));
                 ea.pend[slot] = (int)k;
-                ea.pend_pair[2 * slot] = v.n_inb == 1 ? v.inb_block : -1;
+                ea.pend_pair[2 * slot] = pb;
                 ea.pend_pair[2 * slot + 1] = v.inb_prim;
+              }
+              // lane n_pend + r takes the entry of the in-band lane of rank r
+              const int r = lane - n_pend;
+              int src = 0;
+              if (r >= 0 && r < cnt) {
+                uint64_t mm = m;
+                for (int i = 0; i < r; ++i) mm &= mm - 1;
+                src = __builtin_ctzll(mm);
+              }
+              const int sk = __shfl((int)k, src, 64), sb = __shfl(pb, src, 64), sp = __shfl(v.inb_prim, src, 64);
+              if (r >= 0 && r < cnt) {
+                qk = sk;
+                qb = sb;
+                qp = sp;
               }
               n_pend += cnt;
             } else {
@@ -759,6 +775,24 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
           checked = base + 64 < n_wp ? base + 64 : n_wp;
         }
         if (state == 1 && unc) state = 2;
+        // A pending edge whose queued states each have one in-band candidate
+        // is re-decided here, in FP64 on those candidates as the FP64 pass's
+        // one-candidate path does, and never reaches that pass.
+        if (state == 2 && n_pend > 0) {
+          const bool has = lane < n_pend;
+          const int k = has ? qk : 0;
+          const int pb = has ? qb : 0, pp = has ? qp : 0;
+          if (!__any(has && pb < 0)) {
+            PairVis64 v{k64, s64, pb, pp};
+            v.idle = !has;
+            run_fk(k64, edge_waypoint(ea, ends, n_test, ratio, k), k64.base_type == PMP_BASE_FIXED, v);
+            const bool hit = has && v.collided;
+            const int first = min(kdef, __any(hit) ? wave_min(hit ? k : INT_MAX) : INT_MAX);
+            state = first == INT_MAX ? 1 : 0;
+            checked = first == INT_MAX ? n_wp : (int64_t)first + 1;
+            if (lane == 0) atomicAdd(ea.counter + kCntInline, 1);
+          }
+        }
       }
     }
     if (lane == 0) {
@@ -784,7 +818,7 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
 // around a barrier depends only on the list entry, so it is uniform over the
 // workgroup.
 constexpr int kRecheckWaves = 4;
-constexpr int kRecheckGrid = 256;
+constexpr int kRecheckGrid = 32;  // workgroups; few, so the idle launch and its end-of-launch atomic stay cheap
 __global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinArgs<double> k64,
                                                                            SceneArgs<double> s64, EdgeArgs ea) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -861,7 +895,8 @@ __global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinAr
   if (threadIdx.x == 0) {
     __threadfence();
     if (atomicAdd(ea.counter + kCntDone, 1) == (int)gridDim.x - 1) {
-      ea.counter[kCntRecheck] = n_list;
+      ea.counter[kCntRecheck] = n_list + ea.counter[kCntInline];
+      ea.counter[kCntInline] = 0;
       ea.counter[kCntPending] = 0;
       ea.counter[kCntWork] = 0;
       ea.counter[kCntDone] = 0;
@@ -1560,9 +1595,9 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
   }
   // scenes with a point cloud take the variant that tests clouds in the walk
   if (s32.n_cloud > 0)
-    hipLaunchKernelGGL(edges_kernel<true>, dim3(grid), dim3(block), 0, st, k32, s32, ea, band);
+    hipLaunchKernelGGL(edges_kernel<true>, dim3(grid), dim3(block), 0, st, k32, s32, k64, s64, ea, band);
   else
-    hipLaunchKernelGGL(edges_kernel<false>, dim3(grid), dim3(block), 0, st, k32, s32, ea, band);
+    hipLaunchKernelGGL(edges_kernel<false>, dim3(grid), dim3(block), 0, st, k32, s32, k64, s64, ea, band);
   if (ev_after) {
     hipError_t e = hipEventRecord(ev_after, st);
     if (e != hipSuccess) return e;

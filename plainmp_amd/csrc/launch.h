@@ -9,12 +9,14 @@
 namespace pmp {
 
 constexpr int kPendStates = 16;  // in-band states an edge may queue for the FP64 pass
-// Slots of EdgeArgs::counter.  The work counter and the pending-list length
-// are raised by edges_kernel; the last workgroup of edges_recheck_kernel
-// publishes the list length as the recheck count and sets the other three
-// back to zero, so a launch leaves them as it found them (zero when the
-// buffer is allocated) and no clearing node is needed in a captured graph.
-enum EdgeCounter { kCntWork = 0, kCntRecheck = 1, kCntPending = 2, kCntDone = 3, kNumEdgeCounters = 4 };
+// Slots of EdgeArgs::counter.  The work counter, the pending-list length and
+// the count of edges re-decided in FP64 inside edges_kernel are raised by
+// edges_kernel; the last workgroup of edges_recheck_kernel publishes the
+// FP64 re-decisions of the launch (listed + inline) as the recheck count and
+// sets the others back to zero, so a launch leaves them as it found them
+// (zero when the buffer is allocated) and no clearing node is needed in a
+// captured graph.
+enum EdgeCounter { kCntWork = 0, kCntRecheck = 1, kCntPending = 2, kCntDone = 3, kCntInline = 4, kNumEdgeCounters = 5 };
 
 struct EdgeArgs {
   const double* s1;
