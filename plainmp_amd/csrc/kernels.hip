@@ -20,6 +20,9 @@ constexpr int64_t kWholeUnitChunks = 512;
 // Edges are walked 64 states at a time; an edge needing more states than this
 // (a step ratio below 2^-24) is reported invalid with n_checked = -1 instead.
 constexpr double kMaxEdgeStates = 16777216.0;
+// XCDs of an MI355X; workgroups of a launch are handed to them in turn.  Only
+// the locality of the edge pass's index mapping depends on it.
+constexpr int kXcds = 8;
 
 template <class T>
 __device__ __forceinline__ DevPrim<T> prim_at(const SceneArgs<T>& sc, int p) {
@@ -706,12 +709,25 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
                                                     SceneArgs<double> s64, EdgeArgs ea, float band) {
   const int lane = threadIdx.x & 63;
   const bool taylor = k32.base_type == PMP_BASE_FIXED;
-  for (;;) {
-    // Every lane takes part in the atomic (lane 0 adds 1, the others 0) and
-    // the edge index is read from the first lane into a scalar register, so
-    // the index and every branch on it are wave-uniform by construction.
-    const int e = __builtin_amdgcn_readfirstlane(atomicAdd(ea.counter, lane == 0 ? 1 : 0));
-    if (e >= ea.E) break;
+  // A grid with a wave for every edge gives each wave its edge by index:
+  // workgroups go to the XCDs in turn, so XCD x takes a contiguous range of
+  // edges (neighbouring edges share their endpoints' cache lines in that
+  // XCD's L2) and no wave waits on the work counter.  A smaller grid pulls
+  // edges from the counter.  Every lane takes part in the atomic (lane 0
+  // adds 1, the others 0) and the index is read from the first lane into a
+  // scalar register, so it and every branch on it are wave-uniform.
+  const int W = blockDim.x >> 6;
+  const bool by_index = (int64_t)gridDim.x * W >= ea.E;
+  int e;
+  if (by_index) {
+    const int g = blockIdx.x, x = g % kXcds;
+    int base = 0;
+    for (int xx = 0; xx < x; ++xx) base += ((int)gridDim.x - xx + kXcds - 1) / kXcds;
+    e = (base + g / kXcds) * W + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  } else {
+    e = __builtin_amdgcn_readfirstlane(atomicAdd(ea.counter, lane == 0 ? 1 : 0));
+  }
+  for (; e < ea.E; e = by_index ? ea.E : __builtin_amdgcn_readfirstlane(atomicAdd(ea.counter, lane == 0 ? 1 : 0))) {
     const EdgeEnds ends = load_edge_ends(ea, k32.n_cols, e, lane);
     const double ratio = edge_step_ratio(ea, k32.n_cols, ends);
     int state = 1;
