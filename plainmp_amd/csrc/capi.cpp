@@ -1299,8 +1299,14 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
 
 // Persistent grid: waves_per_cu waves on every CU (default 16, four per
 // SIMD), never more waves than edges.
+// The schedule's waves per CU are capped at what the edge kernel variant can
+// hold at once, so a grid with a wave for every edge (whose waves take their
+// edge by index) is launched only when all of it is resident; otherwise the
+// resident grid pulls edges from the counter.
 int edge_grid(pmp_context* ctx, int block, int64_t E) {
-  const int wpc = ctx->sched.waves_per_cu > 0 ? ctx->sched.waves_per_cu : 16;
+  const int occ = edges_resident_waves_per_cu(ctx->s32.n_cloud > 0, block);
+  int wpc = ctx->sched.waves_per_cu > 0 ? ctx->sched.waves_per_cu : 16;
+  if (occ > 0) wpc = std::min(wpc, occ);
   const int64_t by_edges = (E * 64 + block - 1) / block;
   return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * wpc * 64 / block, by_edges));
 }

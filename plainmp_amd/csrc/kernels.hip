@@ -1210,28 +1210,36 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
   const int u = blockIdx.y;
   BatchQ qs{q, ldq, active ? b : B - 1};
   const int64_t UB = (int64_t)o.n_units * o.ld;
+  const bool whole = o.n_units == 1;  // one unit per chunk: this wave is its own reducer
+  float best = kCutoff, sbest = kCutoff;
+  int bkey = 0x7fffffff, bidx = -1, sidx = -1;
   {
     EvalUnitVis<MAXC> v{ka, sc, cload(o.units, u)};
     run_fk(ka, qs, false, v);
-    if (active) {
+    if (whole) {
+      best = v.best;
+      bidx = v.bidx;
+      sbest = v.sbest;
+      sidx = v.sidx;
+    } else if (active) {
       o.pval[(int64_t)u * o.ld + b] = v.best;
       o.pidx[(int64_t)u * o.ld + b] = v.bidx;
       o.pval[UB + (int64_t)u * o.ld + b] = v.sbest;
       o.pidx[UB + (int64_t)u * o.ld + b] = v.sidx;
     }
   }
-  // the last unit of this chunk reduces it
-  __threadfence();
-  int prev = 0;
-  if (lane == 0) prev = atomicAdd(&o.count[chunk], 1);
-  prev = __shfl(prev, 0, 64);
-  if (prev != o.n_units - 1) return;
-  __threadfence();
+  if (!whole) {
+    // the last unit of this chunk reduces it
+    __threadfence();
+    int prev = 0;
+    if (lane == 0) prev = atomicAdd(&o.count[chunk], 1);
+    prev = __shfl(prev, 0, 64);
+    if (prev != o.n_units - 1) return;
+    __threadfence();
+  }
   if (!active) return;
   const int D = ka.n_cols, NP = sc.n_prims;
-  float best = kCutoff, sbest = kCutoff;
-  int bkey = 0x7fffffff, bidx = -1, sidx = -1;
-  for (int k = 0; k < o.n_units; ++k) {
+  for (int k = 0; k < (whole ? 0 : o.n_units); ++k) {
     const float val = o.pval[(int64_t)k * o.ld + b];
     const int idx = o.pidx[(int64_t)k * o.ld + b];
     if (idx >= 0) {
@@ -1727,6 +1735,14 @@ hipError_t launch_link_jacobian(const KinArgs<float>& ka, const double* q, int64
   return hipGetLastError();
 }
 
+int edges_resident_waves_per_cu(bool cloud, int block) {
+  int n = 0;
+  const hipError_t err =
+      cloud ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, edges_kernel<true>, block, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, edges_kernel<false>, block, 0);
+  return err == hipSuccess ? n * (block / 64) : 0;
+}
+
 hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st) {
   const int64_t n = a.N * a.D + a.n_traj;
   if (n > 0) hipLaunchKernelGGL(sqp_ineq_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, a);
@@ -1763,7 +1779,8 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
     hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks_for(e.n_traj, 256)), dim3(256), 0, st, e.cost, 0.0f,
                        (int64_t)e.n_traj);
   if (B == 0) return hipGetLastError();
-  hipLaunchKernelGGL(fill_counts_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0, st, e.count, 0, chunks);
+  if (o.n_units > 1)
+    hipLaunchKernelGGL(fill_counts_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0, st, e.count, 0, chunks);
   PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks, (unsigned)o.n_units), dim3(64), 0, st, ka,
                     sc, q, ldq, B, o);
   return hipGetLastError();
