@@ -324,6 +324,24 @@ def _extra(torch, stream, dev):
     res["cloud_shelves_edges"] = {"edges": E, "ms": ms, "edges_per_s": E / (ms * 1e-3),
                                   "waypoints_checked": checked, "configs_per_s": checked / (ms * 1e-3),
                                   "valid_frac": float(ev.float().mean().item())}
+    # the cloud's spatial structure built on the device from device-resident
+    # points (bounds, Morton sort, tiles, super-tiles): the shelf cloud and a
+    # 1,048,576-point cloud
+    from plainmp_amd._native import kdtree as kd
+
+    builds = {}
+    for name, pts in (("shelves_65536", wl.cloud), ("normal_1048576", np.random.default_rng(11).normal(
+            size=(1 << 20, 3)))):
+        n = len(pts)
+        nt, ns, scr = kd.cloud_tile_sizes(n)
+        with torch.cuda.stream(stream):
+            dp = torch.tensor(np.ascontiguousarray(pts, dtype=np.float64), device=dev)
+            bufs = [torch.empty(int(b), dtype=torch.uint8, device=dev)
+                    for b in (48 * nt, 48 * nt, 1024 * nt, 2048 * nt, 32 * ns, scr)]
+        args = [dp.data_ptr(), n] + [b.data_ptr() for b in bufs] + [sh]
+        ms = _time_stream(torch, stream, lambda: kd._dev_cloud_build_tiles(*args), steps=10)
+        builds[name] = {"points": n, "tiles": int(nt), "ms": ms, "points_per_s": n / (ms * 1e-3)}
+    res["cloud_tile_build"] = builds
     # path utilities on the edges_mixed16 scene (host arrays in and out, so
     # these rates include the copies): every segment of a 4,096-state random
     # walk in one launch, and batched shortcutting of it (1,024 candidates a

@@ -289,6 +289,28 @@ pmp_status pmp_prims_evaluate(pmp_context* ctx, const pmp_scene_desc* prims, con
 pmp_status pmp_nearest_batch(pmp_context* ctx, const double* points, int64_t N, const double* queries, int64_t M,
                              double* sqdist, int32_t* index);
 
+/* Sizes of the tile tables of a cloud of n_points points (1 <= n_points <
+ * 2^30): tiles of 64 points, super-tiles of 16 tiles, and the scratch bytes
+ * pmp_cloud_build_tiles needs.  Host values. */
+pmp_status pmp_cloud_tile_sizes(int64_t n_points, int64_t* n_tiles, int64_t* n_supers, int64_t* scratch_bytes);
+
+/* The spatial structure of a point cloud built on the device from
+ * device-resident points (FP64, points[3 * i + k]): the points in Morton order
+ * in tiles of 64 with their bounds and the matrix-core operands the cloud
+ * distance kernels read, and super-tiles of 16 tiles (the tables
+ * pmp_upload_scene builds for each cloud primitive).  Replaces the KD-tree
+ * build of CloudSDF (collision/kdtree.cpp:40-65, collision/primitive_sdf.hpp:568-583).
+ * Device outputs, per tile: 48 B at tiles (FP32 lb[3], ub[3], centre[3],
+ * int32 count), 48 B at tiles64 (FP64 lb[3], ub[3]), 256 floats at tile_b
+ * (for point 16 m + j: tile_b[64 m + 16 k + j] = -2 p'_k (k < 3) and |p'|^2
+ * (k = 3), p' = p - centre rounded to FP32; padding 0, 0, 0, 1e30) and 256
+ * doubles at points64 (x, y, z, 0 per point; padding 1e30); per super-tile
+ * 32 B at supers (FP32 lb[3], ub[3], int32 first tile, tile count).
+ * scratch: scratch_bytes of device memory.  Asynchronous on the context's
+ * stream. */
+pmp_status pmp_cloud_build_tiles(pmp_context* ctx, const double* points, int64_t n_points, void* tiles,
+                                 void* tiles64, float* tile_b, double* points64, void* supers, void* scratch);
+
 /* Number of spheres, primitives, q columns and constraint rows of the
  * uploaded tables (host values). */
 pmp_status pmp_query_sizes(pmp_context* ctx, int32_t* n_spheres, int32_t* n_prims, int32_t* q_dim,

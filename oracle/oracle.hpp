@@ -27,6 +27,8 @@
 //                        273-318: row count, msbox sign counts), FP64 central
 //                        differences of the values against the CSC Jacobian,
 //                        sparsity pattern unchanged between evaluations
+//   cloud tile tables    invariants (a partition of the points, exact bounds,
+//                        ascending Morton codes, B operands giving |c - p|^2)
 // Parity unpinned: the libstdc++ unordered_set iteration order that drives
 // the sphere reorder (primitive_sphere_collision.cpp:53-96) is derived from
 // libstdc++'s hashtable rules, not observed from the reference binary.
@@ -1400,6 +1402,121 @@ inline SeqCsc seq_ineq_evaluate(SphereCollisionCst& cst, const std::vector<doubl
       out.values.push_back(val);
     }
     out.colptr.push_back((int32_t)out.rowind.size());
+  }
+  return out;
+}
+
+// ------------------------------------------------------ cloud tile tables
+// The tables the device build of a point cloud's spatial structure writes in
+// place of the reference's KD-tree (collision/kdtree.cpp:40-65): the points
+// sorted by (30-bit Morton code of their position in the cloud's FP64
+// bounds, index), cut into tiles of 64 with FP64 bounds, the FP32 centre,
+// FP32 bounds rounded outwards and the B operands (-2 p', |p'|^2) of
+// p' = (float)(p - centre), padding points far away; super-tiles of 16
+// tiles under the union of their FP32 bounds.  Byte layout as the device
+// records (48 B / 48 B / 256 floats / 256 doubles per tile, 32 B per super).
+// Not a reference algorithm: the structure is this library's own; pinned by
+// invariants in the tests (every point in exactly one tile, bounds equal to
+// the min / max of the tile's points, codes ascending, B operands giving
+// |c - p|^2).
+struct CloudTilesRef {
+  struct alignas(16) T32 { float lb[3], ub[3], o[3]; int32_t n; };  // 40 B of fields, 48 B per record
+  struct T64 { double lb[3], ub[3]; };
+  struct Sup { float lb[3], ub[3]; int32_t first, count; };
+  std::vector<T32> t32;
+  std::vector<T64> t64;
+  std::vector<float> tb;
+  std::vector<double> p64;
+  std::vector<Sup> sup;
+};
+
+inline uint32_t morton_spread10(uint32_t v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+inline CloudTilesRef cloud_tiles(const std::vector<double>& pts) {  // pts[3 i + k]
+  constexpr int kTile = 64, kSuper = 16;
+  const int64_t n = (int64_t)(pts.size() / 3);
+  CloudTilesRef out;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], pts[3 * i + k]);
+      hi[k] = std::max(hi[k], pts[3 * i + k]);
+    }
+  std::vector<std::pair<uint32_t, int64_t>> key(n);
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t c[3];
+    for (int k = 0; k < 3; ++k) {
+      const double span = hi[k] - lo[k];
+      const double u = span > 0 ? (pts[3 * i + k] - lo[k]) / span : 0.0;
+      c[k] = (uint32_t)std::min(1023.0, std::max(0.0, u * 1023.0));
+    }
+    key[i] = {morton_spread10(c[0]) | (morton_spread10(c[1]) << 1) | (morton_spread10(c[2]) << 2), i};
+  }
+  std::sort(key.begin(), key.end());
+  const int64_t nt = (n + kTile - 1) / kTile;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t b = t * kTile, e = std::min(n, b + kTile);
+    CloudTilesRef::T64 T64;
+    for (int k = 0; k < 3; ++k) {
+      T64.lb[k] = 1e300;
+      T64.ub[k] = -1e300;
+    }
+    for (int64_t j = b; j < e; ++j)
+      for (int k = 0; k < 3; ++k) {
+        const double v = pts[3 * key[j].second + k];
+        T64.lb[k] = std::min(T64.lb[k], v);
+        T64.ub[k] = std::max(T64.ub[k], v);
+      }
+    CloudTilesRef::T32 T{};
+    for (int k = 0; k < 3; ++k) {
+      T.o[k] = (float)(0.5 * (T64.lb[k] + T64.ub[k]));
+      T.lb[k] = std::nextafter((float)T64.lb[k], -std::numeric_limits<float>::infinity());
+      T.ub[k] = std::nextafter((float)T64.ub[k], std::numeric_limits<float>::infinity());
+    }
+    T.n = (int32_t)(e - b);
+    out.t32.push_back(T);
+    out.t64.push_back(T64);
+    std::vector<float> blk(4 * kTile, 0.f);
+    for (int jj = 0; jj < kTile; ++jj) {
+      float* col = &blk[(jj / 16) * 64 + jj % 16];
+      if (b + jj < e) {
+        const double* pw = &pts[3 * key[b + jj].second];
+        double pl[3], n2 = 0;
+        for (int k = 0; k < 3; ++k) {
+          pl[k] = (double)(float)(pw[k] - (double)T.o[k]);
+          n2 += pl[k] * pl[k];
+        }
+        for (int k = 0; k < 3; ++k) col[k * 16] = (float)(-2.0 * pl[k]);
+        col[3 * 16] = (float)n2;
+        out.p64.insert(out.p64.end(), {pw[0], pw[1], pw[2], 0.0});
+      } else {
+        col[3 * 16] = 1e30f;
+        out.p64.insert(out.p64.end(), {1e30, 1e30, 1e30, 0.0});
+      }
+    }
+    out.tb.insert(out.tb.end(), blk.begin(), blk.end());
+  }
+  for (int64_t a = 0; a < nt; a += kSuper) {
+    CloudTilesRef::Sup S{};
+    for (int k = 0; k < 3; ++k) {
+      S.lb[k] = std::numeric_limits<float>::infinity();
+      S.ub[k] = -std::numeric_limits<float>::infinity();
+    }
+    S.first = (int32_t)a;
+    S.count = (int32_t)std::min<int64_t>(kSuper, nt - a);
+    for (int64_t t = a; t < a + S.count; ++t)
+      for (int k = 0; k < 3; ++k) {
+        S.lb[k] = std::min(S.lb[k], out.t32[t].lb[k]);
+        S.ub[k] = std::max(S.ub[k], out.t32[t].ub[k]);
+      }
+    out.sup.push_back(S);
   }
   return out;
 }

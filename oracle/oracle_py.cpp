@@ -240,6 +240,20 @@ PYBIND11_MODULE(_oracle, m) {
       .def("query", [](const KDTree& t, const Arr& p) { return to_arr(t.query(v3(p))); })
       .def("sqdist", [](const KDTree& t, const Arr& p) { return t.sqdist(v3(p)); });
 
+  m.def("cloud_tiles", [](const Arr& pts) {  // (N, 3) -> dict of the tables as bytes
+    if (pts.ndim() != 2 || pts.shape(1) != 3) throw std::invalid_argument("expected (N, 3) points");
+    std::vector<double> v(pts.data(), pts.data() + pts.size());
+    const CloudTilesRef r = cloud_tiles(v);
+    auto raw = [](const void* p, size_t n) { return py::bytes((const char*)p, n); };
+    py::dict d;
+    d["tiles"] = raw(r.t32.data(), r.t32.size() * sizeof(CloudTilesRef::T32));
+    d["tiles64"] = raw(r.t64.data(), r.t64.size() * sizeof(CloudTilesRef::T64));
+    d["tile_b"] = raw(r.tb.data(), r.tb.size() * sizeof(float));
+    d["points64"] = raw(r.p64.data(), r.p64.size() * sizeof(double));
+    d["supers"] = raw(r.sup.data(), r.sup.size() * sizeof(CloudTilesRef::Sup));
+    return d;
+  });
+
   py::class_<SphereSpec>(m, "SphereSpec")
       .def(py::init([](const std::string& link, const Arr& rel, const std::vector<double>& radii, bool only_self) {
         if (rel.ndim() != 2 || rel.shape(1) != 3) throw std::invalid_argument("expected (k, 3) positions");

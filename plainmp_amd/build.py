@@ -49,7 +49,8 @@ def _run(cmd, verbose):
 
 def build_library(verbose: bool = False) -> str:
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
-    srcs = [os.path.join(CSRC, "kernels.hip"), os.path.join(CSRC, "cloud.hip"), os.path.join(CSRC, "capi.cpp")]
+    srcs = [os.path.join(CSRC, "kernels.hip"), os.path.join(CSRC, "cloud.hip"),
+            os.path.join(CSRC, "cloud_build.hip"), os.path.join(CSRC, "capi.cpp")]
     tmp = LIB + ".tmp"
     _run([hipcc, *HIP_FLAGS, "-shared", *srcs, "-o", tmp], verbose)
     os.replace(tmp, LIB)

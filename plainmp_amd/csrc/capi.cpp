@@ -141,7 +141,7 @@ struct pmp_context {
   int n_cu = 256;
   // robot
   bool has_robot = false;
-  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, units, unit_all, ws_vjp, ws_eval;
+  DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, units, unit_all, ws_vjp;
   KinArgs<float> k32{};
   KinArgs<double> k64{};
   int n_spheres = 0, n_groups = 0, n_cols = 0, n_nodes = 0;
@@ -152,6 +152,7 @@ struct pmp_context {
   std::vector<NodeReach> h_reach;
   // scene
   DevBuf prims32, prims64, vprims32, vprims64, supers, tiles32, tiles64, tileB, pts64;
+  DevBuf cloud_stage, cloud_scratch;  // points of one cloud and the sort keys of its tile build
   SceneArgs<float> s32{};
   SceneArgs<double> s64{};
   int n_clouds = 0;
@@ -422,122 +423,9 @@ void pack_prims(const pmp_scene_desc& sc, std::vector<DevPrim<float>>& p32, std:
   }
 }
 
-// Interleave the low 10 bits of v into every third bit (Morton code).
-uint32_t spread10(uint32_t v) {
-  v &= 1023u;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-// Tiles of kCloudTile points in Morton order with their bounds, centre, the
-// matrix-core B operands in tile-local coordinates and FP64 world points.
-void tile_cloud(const pmp_prim_desc& d, std::vector<CloudTile>& t32, std::vector<CloudTile64>& t64,
-                std::vector<float>& tb, std::vector<double>& p64) {
-  const int64_t n = d.n_points;
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int64_t i = 0; i < n; ++i)
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = std::min(lo[k], d.points[3 * i + k]);
-      hi[k] = std::max(hi[k], d.points[3 * i + k]);
-    }
-  std::vector<std::pair<uint32_t, int64_t>> key(n);
-  for (int64_t i = 0; i < n; ++i) {
-    uint32_t c[3];
-    for (int k = 0; k < 3; ++k) {
-      double span = hi[k] - lo[k];
-      double u = span > 0 ? (d.points[3 * i + k] - lo[k]) / span : 0.0;
-      c[k] = (uint32_t)std::min(1023.0, std::max(0.0, u * 1023.0));
-    }
-    key[i] = {spread10(c[0]) | (spread10(c[1]) << 1) | (spread10(c[2]) << 2), i};
-  }
-  std::sort(key.begin(), key.end());
-  const int64_t nt = (n + kCloudTile - 1) / kCloudTile;
-  for (int64_t t = 0; t < nt; ++t) {
-    const int64_t b = t * kCloudTile, e = std::min(n, b + kCloudTile);
-    CloudTile64 T64;
-    for (int k = 0; k < 3; ++k) {
-      T64.lb[k] = 1e300;
-      T64.ub[k] = -1e300;
-    }
-    for (int64_t j = b; j < e; ++j)
-      for (int k = 0; k < 3; ++k) {
-        const double v = d.points[3 * key[j].second + k];
-        T64.lb[k] = std::min(T64.lb[k], v);
-        T64.ub[k] = std::max(T64.ub[k], v);
-      }
-    CloudTile T{};
-    double o[3];
-    for (int k = 0; k < 3; ++k) {
-      o[k] = 0.5 * (T64.lb[k] + T64.ub[k]);
-      T.o[k] = (float)o[k];
-      T.lb[k] = std::nextafter((float)T64.lb[k], -INFINITY);  // conservative FP32 bounds
-      T.ub[k] = std::nextafter((float)T64.ub[k], INFINITY);
-    }
-    T.n = (int32_t)(e - b);
-    t32.push_back(T);
-    t64.push_back(T64);
-    // B operand [m][k][j] for point 16 m + j of the tile: (-2 p', |p'|^2) with p' = p - o
-    std::vector<float> blk(4 * kCloudTile, 0.f);
-    for (int jj = 0; jj < kCloudTile; ++jj) {
-      const int m = jj / 16, j = jj % 16;
-      float* col = &blk[m * 64];
-      if (b + jj < e) {
-        const double* pw = d.points + 3 * key[b + jj].second;
-        double pl[3], n2 = 0;
-        for (int k = 0; k < 3; ++k) {
-          pl[k] = (double)(float)(pw[k] - (double)T.o[k]);
-          n2 += pl[k] * pl[k];
-        }
-        for (int k = 0; k < 3; ++k) col[k * 16 + j] = (float)(-2.0 * pl[k]);
-        col[3 * 16 + j] = (float)n2;
-        p64.insert(p64.end(), {pw[0], pw[1], pw[2], 0.0});
-      } else {  // padding: never the nearest
-        for (int k = 0; k < 3; ++k) col[k * 16 + j] = 0.f;
-        col[3 * 16 + j] = 1e30f;
-        p64.insert(p64.end(), {1e30, 1e30, 1e30, 0.0});
-      }
-    }
-    tb.insert(tb.end(), blk.begin(), blk.end());
-  }
-}
-
-// Super-tiles over the tiles [t0, t32.size()) of one cloud.
-void super_tiles(const std::vector<CloudTile>& t32, int t0, std::vector<CloudSuper>& sup) {
-  const int nt = (int)t32.size();
-  for (int a = t0; a < nt; a += kCloudSuper) {
-    CloudSuper S{};
-    for (int k = 0; k < 3; ++k) {
-      S.lb[k] = INFINITY;
-      S.ub[k] = -INFINITY;
-    }
-    S.first = a;
-    S.count = std::min(kCloudSuper, nt - a);
-    for (int t = a; t < a + S.count; ++t)
-      for (int k = 0; k < 3; ++k) {
-        S.lb[k] = std::min(S.lb[k], t32[t].lb[k]);
-        S.ub[k] = std::max(S.ub[k], t32[t].ub[k]);
-      }
-    sup.push_back(S);
-  }
-}
-
 void require_robot(pmp_context* ctx) {
   PMP_REQUIRE(ctx != nullptr, "null context");
   PMP_REQUIRE(ctx->has_robot, "no robot tables uploaded (call pmp_upload_robot first)");
-}
-// Unit values / winners ([units][B]) and per-chunk counters of the evaluate kernels.
-EvalArgs eval_workspace(pmp_context* ctx, int64_t B) {
-  const int64_t units = std::max<int64_t>(1, ctx->k32.n_units);
-  const int64_t chunks = std::max<int64_t>(1, (B + 63) / 64);
-  char* w = (char*)ctx->ws_eval.get(units * B * 16 + chunks * 4 + 256);
-  EvalArgs e{};
-  e.pval = (float*)w;
-  e.pidx = (int32_t*)(w + units * B * 8);
-  e.count = (int32_t*)(w + units * B * 16);
-  return e;
 }
 
 }  // namespace
@@ -576,10 +464,10 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
-    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->units, &ctx->unit_all, &ctx->ws_vjp, &ctx->ws_eval,
+    for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->units, &ctx->unit_all, &ctx->ws_vjp,
                       &ctx->vprims32, &ctx->vprims64, &ctx->supers, &ctx->cull, &ctx->ecull, &ctx->ecull_range, &ctx->vorig, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
-                      &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen,
+                      &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen, &ctx->cloud_stage, &ctx->cloud_scratch,
                       &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2, &ctx->sqp_rows, &ctx->sqp_tables})
       b->release();
     ctx->eb.release();
@@ -928,24 +816,24 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     int n_clouds = 0;
     pack_prims(*sc, p32, p64, c32, c64, n_clouds);
     PMP_REQUIRE(n_clouds <= kMaxClouds, "at most 4 point clouds per scene");
-    std::vector<CloudTile> t32;
-    std::vector<CloudTile64> t64;
-    std::vector<float> tb;
-    std::vector<double> pts;
+    // cloud tile tables: sizes here, contents built on the device below
     std::vector<CloudRange> clouds;
-    std::vector<CloudSuper> sup;
+    std::vector<int> cloud_prim;
+    int64_t n_tiles = 0, n_sup = 0, scratch = 16, max_pts = 0;
     for (int i = 0; i < sc->n_prims; ++i) {
       if (sc->prims[i].type != PMP_PRIM_CLOUD) continue;
-      const int begin = (int)t32.size(), oc = (int)clouds.size();
-      tile_cloud(sc->prims[i], t32, t64, tb, pts);
-      const int nt = (int)t32.size() - begin;
-      const int sbegin = (int)sup.size();
-      super_tiles(t32, begin, sup);
+      const CloudBuildSizes z = cloud_build_sizes(sc->prims[i].n_points);
+      const int begin = (int)n_tiles, oc = (int)clouds.size();
       p32[i].cloud_begin = p64[i].cloud_begin = begin;
-      p32[i].cloud_count = p64[i].cloud_count = nt;
+      p32[i].cloud_count = p64[i].cloud_count = (int)z.n_tiles;
       p32[i].flags |= oc << 4;
       p64[i].flags |= oc << 4;
-      clouds.push_back({begin, nt, (float)sc->prims[i].size[0], sbegin, (int)sup.size() - sbegin});
+      clouds.push_back({begin, (int)z.n_tiles, (float)sc->prims[i].size[0], (int)n_sup, (int)z.n_supers});
+      cloud_prim.push_back(i);
+      n_tiles += z.n_tiles;
+      n_sup += z.n_supers;
+      scratch = std::max(scratch, z.scratch_bytes);
+      max_pts = std::max(max_pts, sc->prims[i].n_points);
     }
     // kind-sorted copy for the validity pass (its answer does not depend on
     // the primitive order)
@@ -973,15 +861,33 @@ pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
     a.vorig = d.vorig = ctx->vorig.upload(vorig, s0);
     for (int k = 0; k <= kNumKinds; ++k) a.vrange[k] = d.vrange[k] = vrange[k];
     d.prims = ctx->prims64.upload(p64, s0);
-    a.tiles = d.tiles = ctx->tiles32.upload(t32, s0);
-    a.supers = d.supers = ctx->supers.upload(sup, s0);
-    a.tiles64 = d.tiles64 = ctx->tiles64.upload(t64, s0);
-    a.tileB = d.tileB = ctx->tileB.upload(tb, s0);
-    a.pts64 = d.pts64 = ctx->pts64.upload(pts, s0);
+    CloudTile* t32 = (CloudTile*)ctx->tiles32.get(std::max<size_t>(n_tiles * sizeof(CloudTile), 16));
+    CloudTile64* t64 = (CloudTile64*)ctx->tiles64.get(std::max<size_t>(n_tiles * sizeof(CloudTile64), 16));
+    float* tb = (float*)ctx->tileB.get(std::max<size_t>(n_tiles * 4 * kCloudTile * sizeof(float), 16));
+    double* p64c = (double*)ctx->pts64.get(std::max<size_t>(n_tiles * kCloudTile * 4 * sizeof(double), 16));
+    CloudSuper* sup = (CloudSuper*)ctx->supers.get(std::max<size_t>(n_sup * sizeof(CloudSuper), 16));
+    if (!clouds.empty()) {
+      double* stage = (double*)ctx->cloud_stage.get(max_pts * 3 * sizeof(double));
+      void* scr = ctx->cloud_scratch.get(scratch);
+      for (size_t c = 0; c < clouds.size(); ++c) {
+        const pmp_prim_desc& pd = sc->prims[cloud_prim[c]];
+        const CloudRange& cr = clouds[c];
+        PMP_HIP(hipMemcpyAsync(stage, pd.points, pd.n_points * 3 * sizeof(double), hipMemcpyHostToDevice, s0));
+        PMP_HIP(launch_cloud_build(stage, pd.n_points, cr.tile_begin, t32 + cr.tile_begin, t64 + cr.tile_begin,
+                                   tb + (int64_t)cr.tile_begin * 4 * kCloudTile, p64c + (int64_t)cr.tile_begin * kCloudTile * 4,
+                                   sup + cr.super_begin, scr, s0));
+        PMP_HIP(hipStreamSynchronize(s0));  // the next cloud reuses the staging buffer
+      }
+    }
+    a.tiles = d.tiles = t32;
+    a.supers = d.supers = sup;
+    a.tiles64 = d.tiles64 = t64;
+    a.tileB = d.tileB = tb;
+    a.pts64 = d.pts64 = p64c;
     a.n_prims = d.n_prims = sc->n_prims;
     a.n_cloud = d.n_cloud = n_clouds;
     for (int i = 0; i <= kMaxClouds; ++i)
-      a.super_at[i] = d.super_at[i] = i < n_clouds ? clouds[i].super_begin : (int32_t)sup.size();
+      a.super_at[i] = d.super_at[i] = i < n_clouds ? clouds[i].super_begin : (int32_t)n_sup;
     ctx->s32 = a;
     ctx->s64 = d;
     ctx->n_clouds = n_clouds;
@@ -1093,7 +999,7 @@ pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, in
     PMP_REQUIRE(B == 0 || (vals && jac), "null output");
     if (B == 0) return;
     set_device(ctx);
-    EvalArgs e = eval_workspace(ctx, B);
+    EvalArgs e{};
     e.vals = vals;
     e.jac = jac;
     e.whole_chunks = ctx->sched.eval_whole_chunks;
@@ -1127,7 +1033,7 @@ pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t l
     }
     const int64_t* doff = (const int64_t*)ctx->traj_off.ptr;
     const double* dw = (const double*)ctx->traj_w2.ptr;
-    EvalArgs e = eval_workspace(ctx, N);
+    EvalArgs e{};
     e.vals = vals;
     e.jac = jac;
     e.offsets = doff;
@@ -1190,7 +1096,7 @@ pmp_status pmp_traj_sqp_ineq(pmp_context* ctx, const double* X, int64_t ldx, int
     // evaluate rows of every waypoint (the plain evaluate launch over N)
     float* rows = (float*)ctx->sqp_rows.get(sizeof(float) * std::max<int64_t>(1, (int64_t)C * (D + 1) * N));
     if (N > 0 && C > 0) {
-      EvalArgs e = eval_workspace(ctx, N);
+      EvalArgs e{};
       e.vals = rows;
       e.jac = rows + (int64_t)C * N;
       e.whole_chunks = ctx->sched.eval_whole_chunks;
@@ -1437,6 +1343,28 @@ pmp_status pmp_nearest_batch(pmp_context* ctx, const double* points, int64_t N, 
     if (M == 0) return;
     set_device(ctx);
     PMP_HIP(launch_nearest(points, N, queries, M, sqdist, index, st(ctx)));
+  });
+}
+
+pmp_status pmp_cloud_tile_sizes(int64_t n_points, int64_t* n_tiles, int64_t* n_supers, int64_t* scratch_bytes) {
+  return guarded([&] {
+    PMP_REQUIRE(n_points >= 1 && n_points < (int64_t)1 << 30, "point count out of range");
+    const CloudBuildSizes z = cloud_build_sizes(n_points);
+    if (n_tiles) *n_tiles = z.n_tiles;
+    if (n_supers) *n_supers = z.n_supers;
+    if (scratch_bytes) *scratch_bytes = z.scratch_bytes;
+  });
+}
+
+pmp_status pmp_cloud_build_tiles(pmp_context* ctx, const double* points, int64_t n_points, void* tiles,
+                                 void* tiles64, float* tile_b, double* points64, void* supers, void* scratch) {
+  return guarded([&] {
+    PMP_REQUIRE(ctx != nullptr, "null context");
+    PMP_REQUIRE(n_points >= 1 && n_points < (int64_t)1 << 30, "point count out of range");
+    PMP_REQUIRE(points && tiles && tiles64 && tile_b && points64 && supers && scratch, "null argument");
+    set_device(ctx);
+    PMP_HIP(launch_cloud_build(points, n_points, 0, (CloudTile*)tiles, (CloudTile64*)tiles64, tile_b, points64,
+                               (CloudSuper*)supers, scratch, st(ctx)));
   });
 }
 

@@ -228,6 +228,50 @@ PYBIND11_MODULE(_native, m) {
         }
         return std::make_pair(P, S);
       });
+  // The cloud's tile tables built on the device (pmp_cloud_build_tiles).
+  mt.def("cloud_tile_sizes", [](int64_t n) {
+    int64_t nt = 0, ns = 0, scr = 0;
+    check(pmp_cloud_tile_sizes(n, &nt, &ns, &scr));
+    return py::make_tuple(nt, ns, scr);
+  });
+  mt.def("build_cloud_tiles", [](const DArr& pts) {  // (N, 3) -> dict of the tables as bytes
+    if (pts.ndim() != 2 || pts.shape(1) != 3) throw std::invalid_argument("expected (N, 3) points");
+    const int64_t n = pts.shape(0);
+    int64_t nt = 0, ns = 0, scr = 0;
+    check(pmp_cloud_tile_sizes(n, &nt, &ns, &scr));
+    Device& dev = shared_device();
+    const size_t sz[5] = {(size_t)nt * 48, (size_t)nt * 48, (size_t)nt * 256 * sizeof(float),
+                          (size_t)nt * 256 * sizeof(double), (size_t)ns * 32};
+    void* out[5];
+    for (int i = 0; i < 5; ++i) out[i] = dev.buf(10 + i, sz[i]);
+    void* dp = dev.buf(15, (size_t)n * 3 * sizeof(double));
+    void* ds = dev.buf(16, (size_t)scr);
+    std::vector<std::string> host(5);
+    {
+      py::gil_scoped_release rel;
+      dev.h2d(dp, pts.data(), (size_t)n * 3 * sizeof(double));
+      check(pmp_cloud_build_tiles(dev.ctx(), (const double*)dp, n, out[0], out[1], (float*)out[2], (double*)out[3],
+                                  out[4], ds));
+      check(pmp_synchronize(dev.ctx()));
+      for (int i = 0; i < 5; ++i) {
+        host[i].resize(sz[i]);
+        dev.d2h(host[i].data(), out[i], sz[i]);
+      }
+    }
+    const char* names[5] = {"tiles", "tiles64", "tile_b", "points64", "supers"};
+    py::dict d;
+    for (int i = 0; i < 5; ++i) d[names[i]] = py::bytes(host[i]);
+    return d;
+  });
+  mt.def("_dev_cloud_build_tiles", [](uintptr_t pts, int64_t n, uintptr_t tiles, uintptr_t tiles64, uintptr_t tile_b,
+                                      uintptr_t points64, uintptr_t supers, uintptr_t scratch, uintptr_t stream) {
+    Device& dev = shared_device();
+    check(pmp_set_stream(dev.ctx(), (void*)stream));
+    const pmp_status s = pmp_cloud_build_tiles(dev.ctx(), (const double*)pts, n, (void*)tiles, (void*)tiles64,
+                                               (float*)tile_b, (double*)points64, (void*)supers, (void*)scratch);
+    check(pmp_set_stream(dev.ctx(), nullptr));
+    check(s);
+  });
 
   // ------------------------------------------------------------- constraint
   auto mc = m.def_submodule("constraint");

@@ -1151,9 +1151,6 @@ struct EvalOut {
   int64_t ld;
   const DevEvalUnit* units;
   int n_units;
-  float* pval;    // [2][units][ld] unit minima: external, then self
-  int32_t* pidx;  // [2][units][ld] winners (sphere * kMaxPrims + prim; sphere_i * kMaxSpheres + sphere_j), -1 none
-  int32_t* count; // [chunks] finished units, zeroed before the launch
   // trajectories (offsets == nullptr: plain evaluate)
   const int64_t* offsets;
   int n_traj;
@@ -1200,48 +1197,23 @@ __device__ void smooth_terms(const double* X, int64_t ldx, int64_t n, int D, con
   }
 }
 
-template <int MAXC>
-__global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
-                                                        int64_t ldq, int64_t B, EvalOut o) {
-  const int lane = threadIdx.x;
-  const int64_t chunk = blockIdx.x;
-  const int64_t b = chunk * 64 + lane;
-  const bool active = b < B;
-  const int u = blockIdx.y;
-  BatchQ qs{q, ldq, active ? b : B - 1};
-  const int64_t UB = (int64_t)o.n_units * o.ld;
-  const bool whole = o.n_units == 1;  // one unit per chunk: this wave is its own reducer
-  float best = kCutoff, sbest = kCutoff;
-  int bkey = 0x7fffffff, bidx = -1, sidx = -1;
-  {
-    EvalUnitVis<MAXC> v{ka, sc, cload(o.units, u)};
-    run_fk(ka, qs, false, v);
-    if (whole) {
-      best = v.best;
-      bidx = v.bidx;
-      sbest = v.sbest;
-      sidx = v.sidx;
-    } else if (active) {
-      o.pval[(int64_t)u * o.ld + b] = v.best;
-      o.pidx[(int64_t)u * o.ld + b] = v.bidx;
-      o.pval[UB + (int64_t)u * o.ld + b] = v.sbest;
-      o.pidx[UB + (int64_t)u * o.ld + b] = v.sidx;
-    }
-  }
-  if (!whole) {
-    // the last unit of this chunk reduces it
-    __threadfence();
-    int prev = 0;
-    if (lane == 0) prev = atomicAdd(&o.count[chunk], 1);
-    prev = __shfl(prev, 0, 64);
-    if (prev != o.n_units - 1) return;
-    __threadfence();
-  }
-  if (!active) return;
+// Reduction of a configuration's unit results in the reference's order
+// (external minimum by (group, primitive, sphere) key, self minimum by
+// visiting order; units hold increasing self-pair ranges, so unit order is
+// visiting order), then the Jacobian rows of the winners from one more FK
+// walk, and for trajectories the smoothness terms.  Unit k's results are at
+// xv[k * stride], xi, sv, si (any address space); (best, bidx, sbest, sidx)
+// start the reduction.
+template <int MAXC, class QS>
+__device__ __forceinline__ void eval_finish(const KinArgs<float>& ka, const SceneArgs<float>& sc, const QS& qs, const double* q,
+                            int64_t ldq, int64_t b, const EvalOut& o, float best, int bidx, float sbest, int sidx,
+                            const float* xv, const int* xi, const float* sv, const int* si, int64_t stride,
+                            int n_red) {
   const int D = ka.n_cols, NP = sc.n_prims;
-  for (int k = 0; k < (whole ? 0 : o.n_units); ++k) {
-    const float val = o.pval[(int64_t)k * o.ld + b];
-    const int idx = o.pidx[(int64_t)k * o.ld + b];
+  int bkey = 0x7fffffff;
+  for (int k = 0; k < n_red; ++k) {
+    const float val = xv[k * stride];
+    const int idx = xi[k * stride];
     if (idx >= 0) {
       const DevSphereMeta m = cload(ka.meta, idx / kMaxPrims);
       const int key = (m.group * NP + idx % kMaxPrims) * ka.n_spheres + m.out;
@@ -1251,9 +1223,8 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
         bidx = idx;
       }
     }
-    // self pairs: units hold increasing pair ranges, so unit order is visiting order
-    const float sval = o.pval[UB + (int64_t)k * o.ld + b];
-    const int sid = o.pidx[UB + (int64_t)k * o.ld + b];
+    const float sval = sv[k * stride];
+    const int sid = si[k * stride];
     if (sid >= 0 && sval < sbest) {
       sbest = sval;
       sidx = sid;
@@ -1306,6 +1277,58 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
             dot(n, sub(rv.fr.col(j, m1, ka.rev_mask, rv.c1), rv.fr.col(j, m2, ka.rev_mask, rv.c2)));
   }
   if (o.offsets) smooth_terms<MAXC>(q, ldq, b, D, o);
+}
+
+// Whole mode: one wave per chunk of 64 configurations walks every sphere
+// block and self pair of them and reduces in place.
+template <int MAXC>
+__global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, SceneArgs<float> sc, const double* q,
+                                                        int64_t ldq, int64_t B, EvalOut o) {
+  const int lane = threadIdx.x;
+  const int64_t b = (int64_t)blockIdx.x * 64 + lane;
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  EvalUnitVis<MAXC> v{ka, sc, cload(o.units, 0)};
+  run_fk(ka, qs, false, v);
+  if (!active) return;
+  eval_finish<MAXC>(ka, sc, qs, q, ldq, b, o, v.best, v.bidx, v.sbest, v.sidx, nullptr, nullptr, nullptr, nullptr, 0,
+                    0);
+}
+
+// Split mode: one workgroup per chunk of 64 configurations; its waves take
+// the units in turn, leave their results in LDS, and after a barrier the
+// first wave reduces them (no device-scope fence or atomic: a version with a
+// wave per (chunk, unit) that met through a device-scope counter took
+// longer).  LDS holds
+// 16 B x 64 x n_units.
+constexpr int kEvalGroupWaves = 8;
+constexpr int kEvalGroupMaxUnits = 64;  // 64 KiB of LDS
+template <int MAXC>
+__global__ void __launch_bounds__(64 * kEvalGroupWaves) eval_group_kernel(KinArgs<float> ka, SceneArgs<float> sc,
+                                                                          const double* q, int64_t ldq, int64_t B,
+                                                                          EvalOut o) {
+  HIP_DYNAMIC_SHARED(int32_t, eval_lds)
+  const int U = o.n_units;
+  float* xv = reinterpret_cast<float*>(eval_lds);
+  int* xi = eval_lds + U * 64;
+  float* sv = reinterpret_cast<float*>(eval_lds + 2 * U * 64);
+  int* si = eval_lds + 3 * U * 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int64_t b = (int64_t)blockIdx.x * 64 + lane;
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  for (int u = w; u < U; u += W) {
+    EvalUnitVis<MAXC> v{ka, sc, cload(o.units, u)};
+    run_fk(ka, qs, false, v);
+    xv[u * 64 + lane] = v.best;
+    xi[u * 64 + lane] = v.bidx;
+    sv[u * 64 + lane] = v.sbest;
+    si[u * 64 + lane] = v.sidx;
+  }
+  __syncthreads();
+  if (w != 0 || !active) return;
+  eval_finish<MAXC>(ka, sc, qs, q, ldq, b, o, kCutoff, -1, kCutoff, -1, xv + lane, xi + lane, sv + lane, si + lane, 64,
+                    U);
 }
 
 // The pair and adjoint kernels give each (configuration, sphere item) its
@@ -1760,16 +1783,14 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
   o.vals = e.vals;
   o.jac = e.jac;
   o.ld = B;
-  // Small batches give each configuration many units (one per sphere item and
-  // per self pair) so that the launch fills the GPU; from kWholeUnitChunks
-  // chunks on, one unit walks the whole configuration once.
+  // Small batches give each configuration many units (one per sphere block
+  // and per self pair, the units of a chunk in one workgroup) so that the
+  // launch fills the GPU; from kWholeUnitChunks chunks on (or beyond
+  // kEvalGroupMaxUnits units), one unit walks the whole configuration.
   const int64_t whole = e.whole_chunks > 0 ? e.whole_chunks : kWholeUnitChunks;
-  const bool split = chunks < whole && ka.n_units > 1;
+  const bool split = chunks < whole && ka.n_units > 1 && ka.n_units <= kEvalGroupMaxUnits;
   o.units = split ? ka.units : ka.unit_all;
   o.n_units = split ? ka.n_units : 1;
-  o.pval = e.pval;
-  o.pidx = e.pidx;
-  o.count = e.count;
   o.offsets = e.offsets;
   o.n_traj = e.n_traj;
   o.w2 = e.w2;
@@ -1779,10 +1800,14 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
     hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks_for(e.n_traj, 256)), dim3(256), 0, st, e.cost, 0.0f,
                        (int64_t)e.n_traj);
   if (B == 0) return hipGetLastError();
-  if (o.n_units > 1)
-    hipLaunchKernelGGL(fill_counts_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0, st, e.count, 0, chunks);
-  PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks, (unsigned)o.n_units), dim3(64), 0, st, ka,
-                    sc, q, ldq, B, o);
+  if (split) {
+    const int waves = o.n_units < kEvalGroupWaves ? o.n_units : kEvalGroupWaves;
+    const size_t lds = sizeof(int32_t) * 4 * 64 * (size_t)o.n_units;
+    PMP_DISPATCH_MAXC(ka.n_cols, eval_group_kernel, dim3((unsigned)chunks), dim3(64 * waves), lds, st, ka, sc, q, ldq,
+                      B, o);
+  } else {
+    PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks), dim3(64), 0, st, ka, sc, q, ldq, B, o);
+  }
   return hipGetLastError();
 }
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,

@@ -92,15 +92,10 @@ hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const EdgeArgs& ea, float band, int grid, int block,
                         hipStream_t st, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr);
-// Evaluate rows (and, with offsets, trajectory smoothness) by units; the
-// workspace holds units * B values and winners and one counter per 64
-// configurations.
+// Evaluate rows (and, with offsets, trajectory smoothness) by units.
 struct EvalArgs {
   float* vals;
   float* jac;
-  float* pval;
-  int32_t* pidx;
-  int32_t* count;
   const int64_t* offsets;  // device, n_traj + 1; nullptr for plain evaluate
   int n_traj;
   const double* w2;
@@ -119,5 +114,19 @@ hipError_t launch_prims_eval(const DevPrim<double>* prims, int n, const double* 
                              int64_t N, int mode, double radius, double* out, hipStream_t st);
 hipError_t launch_nearest(const double* pts, int64_t N, const double* qry, int64_t M, double* sqd, int32_t* idx,
                           hipStream_t st);
+
+// Device build of one cloud's tile tables (cloud_build.hip).
+constexpr int kSortChunk = 512;  // keys sorted per workgroup in LDS
+struct CloudBuildSizes {
+  int64_t n_tiles, n_supers;
+  int64_t n_keys;         // sort length: a power of two >= max(n, kSortChunk)
+  int64_t scratch_bytes;  // keys and bounds
+};
+CloudBuildSizes cloud_build_sizes(int64_t n);
+// pts: device [n][3] FP64.  Writes n_tiles records to t32 / t64, n_tiles *
+// 4 * kCloudTile floats to tb, n_tiles * kCloudTile * 4 doubles to p64 and
+// n_supers records to sup, whose tile ranges start at tile_base.
+hipError_t launch_cloud_build(const double* pts, int64_t n, int32_t tile_base, CloudTile* t32, CloudTile64* t64,
+                              float* tb, double* p64, CloudSuper* sup, void* scratch, hipStream_t st);
 
 }  // namespace pmp

@@ -106,6 +106,8 @@ struct alignas(16) CloudTile {
 struct alignas(16) CloudTile64 {
   double lb[3], ub[3];
 };
+// Record sizes are part of pmp_cloud_build_tiles' output contract.
+static_assert(sizeof(CloudTile) == 48 && sizeof(CloudTile64) == 48, "cloud tile record layout");
 // kCloudSuper consecutive (Morton-ordered) tiles under one FP32 AABB: a wave
 // tests a super-tile once before its tiles.
 constexpr int kCloudSuper = 16;
@@ -113,6 +115,7 @@ struct alignas(16) CloudSuper {
   float lb[3], ub[3];
   int32_t first, count;  // tile range [first, first + count)
 };
+static_assert(sizeof(CloudSuper) == 32, "cloud super-tile record layout");
 
 // Where the FP32 validity pass leaves sphere-vs-cloud queries.
 struct CloudWork {

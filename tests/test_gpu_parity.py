@@ -320,6 +320,45 @@ def test_kdtree_and_cloud():
     assert np.allclose(cs.evaluate_batch(ev), ref, atol=1e-12)
 
 
+def _tile_fields(d):
+    """The field bytes of the tile tables (the 8 padding bytes of each
+    48-byte FP32 tile record carry no value)."""
+    d = dict(d)
+    d["tiles"] = np.frombuffer(d["tiles"], np.uint8).reshape(-1, 48)[:, :40].tobytes()
+    return d
+
+
+@pytest.mark.parametrize("case", ["one", "ragged65", "flat_axis", "duplicates", "shelves65536", "wide1M"])
+def test_cloud_tile_build_matches_oracle(case):
+    """The device build of a cloud's tile tables (bounds, Morton sort, tiles,
+    matrix-core operands, super-tiles; pmp_cloud_build_tiles) against the
+    oracle's restatement, bit-exact in every table."""
+    from plainmp_amd._native import kdtree as kd
+
+    rng = np.random.default_rng(11)
+    if case == "one":
+        P = np.array([[0.1, -0.2, 0.3]])
+    elif case == "ragged65":
+        P = rng.uniform(-1, 1, (65, 3))
+    elif case == "flat_axis":  # zero span on z: every code has z = 0
+        P = rng.uniform(-1, 1, (1000, 3))
+        P[:, 2] = 0.25
+    elif case == "duplicates":  # equal codes, ordered by index
+        P = rng.uniform(-1, 1, (3000, 3))
+        P[100:900] = P[5]
+    elif case == "shelves65536":
+        P = synth.shelf_cloud(65536, seed=2)
+    else:
+        P = rng.normal(size=(1 << 20, 3)) * np.array([2.0, 1.0, 0.5])
+    got = _tile_fields(kd.build_cloud_tiles(P))
+    ref = _tile_fields(_oracle.cloud_tiles(P))
+    n_tiles, n_supers, _ = kd.cloud_tile_sizes(len(P))
+    _report("cloud_tiles", case=case, n=len(P), tiles=n_tiles, supers=n_supers)
+    assert len(ref["tiles"]) == 40 * n_tiles and len(ref["supers"]) == 32 * n_supers
+    for k in ref:
+        assert got[k] == ref[k], k
+
+
 @pytest.mark.parametrize("n_configs", [256])
 def test_cloud_valid_and_clearance(n_configs):
     wl = synth.workload("cloud_shelves", scale=n_configs / 16384)

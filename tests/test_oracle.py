@@ -247,6 +247,64 @@ def test_kdtree_vs_brute_force():
 
 
 # ----------------------------------------------------------- collision cst
+def _morton_codes(P):
+    """30-bit Morton codes of points in their bounds, interleaved bit by bit
+    (x in bit 3 b, y in 3 b + 1, z in 3 b + 2 for bit b of each 10-bit
+    coordinate)."""
+    lo, hi = P.min(0), P.max(0)
+    span = hi - lo
+    u = np.where(span > 0, (P - lo) / np.where(span > 0, span, 1.0), 0.0)
+    c = np.clip(u * 1023.0, 0.0, 1023.0).astype(np.uint64)
+    code = np.zeros(len(P), np.uint64)
+    for b in range(10):
+        for k in range(3):
+            code |= ((c[:, k] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + k)
+    return code
+
+
+@pytest.mark.parametrize("n", [1, 64, 65, 1000, 5000])
+def test_cloud_tiles_invariants(n):
+    """The oracle's tile tables: the tiles hold the points in (Morton code,
+    index) order, exactly once; FP64 bounds are the min / max of each tile's
+    points and the FP32 bounds enclose them; the B operands give
+    |c - p|^2 = |c - o|^2 + |p'|^2 - 2 (c - o).p'; padding is far away;
+    super-tiles cover 16 tiles under the union of their bounds."""
+    rng = np.random.default_rng(n)
+    P = rng.uniform(-1, 1, (n, 3)) * np.array([2.0, 0.5, 1.0])
+    d = _oracle.cloud_tiles(P)
+    t32 = np.frombuffer(d["tiles"], np.float32).reshape(-1, 12)
+    tn = np.frombuffer(d["tiles"], np.int32).reshape(-1, 12)[:, 9]
+    t64 = np.frombuffer(d["tiles64"], np.float64).reshape(-1, 6)
+    tb = np.frombuffer(d["tile_b"], np.float32).reshape(-1, 4, 4, 16)  # [tile][m][k][j]
+    p64 = np.frombuffer(d["points64"], np.float64).reshape(-1, 64, 4)
+    sup = np.frombuffer(d["supers"], np.float32).reshape(-1, 8)
+    supi = np.frombuffer(d["supers"], np.int32).reshape(-1, 8)
+    nt = (n + 63) // 64
+    assert len(t32) == nt and len(p64) == nt and len(tb) == nt and len(sup) == (nt + 15) // 16
+    order = np.lexsort((np.arange(n), _morton_codes(P)))
+    live = p64[:, :, :3].reshape(-1, 3)[:n]
+    assert np.array_equal(live, P[order])
+    assert np.all(p64.reshape(-1, 4)[n:, :3] == 1e30)
+    for t in range(nt):
+        pts = P[order[64 * t:64 * t + 64]]
+        assert tn[t] == len(pts)
+        assert np.array_equal(t64[t, :3], pts.min(0)) and np.array_equal(t64[t, 3:], pts.max(0))
+        assert np.all(t32[t, 0:3].astype(np.float64) < pts.min(0)) and np.all(t32[t, 3:6].astype(np.float64) > pts.max(0))
+        o = t32[t, 6:9].astype(np.float64)
+        assert np.allclose(o, 0.5 * (pts.min(0) + pts.max(0)), atol=1e-7)
+        B = tb[t].transpose(0, 2, 1).reshape(64, 4).astype(np.float64)  # point 16 m + j -> (-2p', |p'|^2)
+        c = rng.uniform(-1, 1, (5, 3))
+        d2 = ((c - o) ** 2).sum(1)[:, None] + B[None, :len(pts), 3] + (c - o) @ B[:len(pts), :3].T
+        ref = ((c[:, None, :] - pts[None]) ** 2).sum(2)
+        assert np.abs(d2 - ref).max() <= 1e-6
+        assert np.all(B[len(pts):, :3] == 0) and np.all(B[len(pts):, 3] == np.float32(1e30))
+    for s in range(len(sup)):
+        first, count = supi[s, 6], supi[s, 7]
+        assert first == 16 * s and count == min(16, nt - 16 * s)
+        assert np.array_equal(sup[s, 0:3], t32[first:first + count, 0:3].min(0))
+        assert np.array_equal(sup[s, 3:6], t32[first:first + count, 3:6].max(0))
+
+
 def test_is_valid_consistent_with_evaluate():
     # tests/test_constraints.py:47-53
     wl = synth.workload("grad_mixed32", scale=1 / 64)
