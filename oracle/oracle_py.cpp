@@ -103,6 +103,15 @@ PYBIND11_MODULE(_oracle, m) {
     Q4 q = quat_from_rpy(r, p, y);
     return std::vector<double>{q.x, q.y, q.z, q.w};
   });
+  m.def("quat_to_rpy", [](double x, double y, double z, double w) {
+    Q4 q;
+    q.x = x;
+    q.y = y;
+    q.z = z;
+    q.w = w;
+    V3 r = quat_to_rpy(q);
+    return std::vector<double>{r.x, r.y, r.z};
+  });
   m.def("farthest_sequence", &farthest_sequence);
   m.def("sequence_table", []() { return sequence_table(); });
   m.def("box_step_ratio", &box_step_ratio);

@@ -5,14 +5,25 @@ The reference's ``solve_ik`` runs SLSQP from one random seed at a time, up to
 is at most ``acceptable_error`` and which the inequality constraint reports
 valid; its expensive inner calls are the pose and collision Jacobians.  Here
 every restart runs at once: each iteration makes one link-pose launch (FP64
-FK), one link-Jacobian launch and one evaluate launch (collision values and
-rows) for all restarts, and takes a damped Gauss-Newton step per restart on
-the position error with the collision values below ``margin`` as penalty
-residuals, projected onto the joint bounds.  The answer is the first restart,
-in restart order, that passes the reference's acceptance test (position
-error squared <= ``acceptable_error``, ``is_valid``), so a result is
-reproducible from the seed.  Targets are link positions (``LinkPoseCst``
-with 3-vector poses, constraints/primitive_link_pose.cpp:32-44).
+FK), one link-Jacobian launch (four with a rotation target) and one evaluate
+launch (collision values and rows) for all restarts, and takes a damped
+Gauss-Newton step per restart on the pose residual with the collision values
+below ``margin`` as penalty residuals, projected onto the joint bounds.  The
+answer is the first restart, in restart order, that passes the reference's
+acceptance test (squared pose residual <= ``acceptable_error``,
+``is_valid``), so a result is reproducible from the seed.
+
+Targets are ``LinkPoseCst`` poses (constraints/primitive_link_pose.cpp:32-57):
+a 3-vector position, a 6-vector position + roll-pitch-yaw (the residual is
+``getRPY() - rpy``, transform.hpp:93-120) or a 7-vector position + quaternion
+coefficients x, y, z, w (residual ``quat.coeffs() - q``).  The rotation rows
+of the Jacobian are the reference's (kinematics/algorithm.cpp:21-47, 255-272):
+``rpy_derivative(rpy, w_j)`` and ``q_derivative(q^-1, w_j)`` of each revolute
+joint's world axis w_j, which this module obtains from link-Jacobian launches
+at the link origin and at its three unit axes: the point Jacobian column of a
+revolute joint is w_j x (p - o_j), so the differences D_k at p + u_k give
+w_j = 1/2 sum_k u_k x D_k for the orthonormal world axes u_k of the link
+(prismatic and unrelated joints give zero).
 """
 
 from __future__ import annotations
@@ -26,7 +37,7 @@ import numpy as np
 
 @dataclass
 class IKConfig:
-    acceptable_error: float = 1e-6  # squared position error, as solve_ik's res.fun test
+    acceptable_error: float = 1e-6  # squared pose residual, as solve_ik's res.fun test
     n_iter: int = 60
     damping: float = 1e-4
     max_step: float = 0.3  # largest joint step per iteration (rad or m)
@@ -42,16 +53,72 @@ class IKResult:
     n_trial: int
 
 
+def _rpy_of(qx, qy, qz, qw):
+    """QuatTrans::getRPY (kinematics/transform.hpp:93-120), elementwise."""
+    import torch
+
+    sqx, sqy, sqz, sqw = qx * qx, qy * qy, qz * qz, qw * qw
+    sarg = -2 * (qx * qz - qw * qy)
+    half_pi = 1.57079632679489661923
+    lock = 2 * torch.atan2(qx, qy)
+    pitch = torch.where(sarg <= -0.99999, torch.full_like(sarg, -half_pi),
+                        torch.where(sarg >= 0.99999, torch.full_like(sarg, half_pi),
+                                    torch.asin(torch.clamp(sarg, -1.0, 1.0))))
+    roll = torch.where((sarg <= -0.99999) | (sarg >= 0.99999), torch.zeros_like(sarg),
+                       torch.atan2(2 * (qy * qz + qw * qx), sqw - sqx - sqy + sqz))
+    yaw = torch.where(sarg <= -0.99999, -lock, torch.where(sarg >= 0.99999, lock,
+                                                            torch.atan2(2 * (qx * qy + qw * qz), sqw + sqx - sqy - sqz)))
+    return torch.stack([roll, pitch, yaw], dim=-1)
+
+
+def _rotation_rows(kind, quat, rpy, w):
+    """Rotation rows of the Jacobian, (R, 3 or 4, D), from the world axes
+    w (R, 3, D): rpy_derivative or q_derivative of the inverse quaternion
+    (kinematics/algorithm.cpp:21-47)."""
+    import torch
+
+    wx, wy, wz = w[:, 0], w[:, 1], w[:, 2]
+    if kind == 6:
+        a2, a3 = -rpy[:, 1:2], -rpy[:, 2:3]
+        c2, c3, s2, s3 = torch.cos(a2), torch.cos(a3), torch.sin(a2), torch.sin(a3)
+        dr = c3 / c2 * wx - s3 / c2 * wy
+        dp = s3 * wx + c3 * wy
+        dy = -c3 * s2 / c2 * wx + s3 * s2 / c2 * wy + wz
+        return torch.stack([dr, dp, dy], dim=1)
+    # q^-1 of a unit quaternion: (-x, -y, -z, w)
+    x, y, z, qw = (-quat[:, 0:1], -quat[:, 1:2], -quat[:, 2:3], quat[:, 3:4])
+    dxdt = 0.5 * (wz * y - wy * z + wx * qw)
+    dydt = 0.5 * (-wz * x + wx * z + wy * qw)
+    dzdt = 0.5 * (wy * x - wx * y + wz * qw)
+    dwdt = 0.5 * (-wx * x - wy * y - wz * z)
+    return torch.stack([dxdt, dydt, dzdt, -dwdt], dim=1)  # coeffs of Quaternion(-dwdt, dxdt, dydt, dzdt)
+
+
+def _rot_matrix(quat):
+    """World rotation matrices (R, 3, 3) of unit quaternions (x, y, z, w)."""
+    import torch
+
+    x, y, z, w = quat[:, 0], quat[:, 1], quat[:, 2], quat[:, 3]
+    return torch.stack([
+        torch.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], dim=-1),
+        torch.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], dim=-1),
+        torch.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], dim=-1)], dim=1)
+
+
 def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[float], ub: Sequence[float], *,
                      check_collision: bool = True, q_seed: Optional[np.ndarray] = None, n_restarts: int = 256,
                      seed: int = 0, config: Optional[IKConfig] = None) -> IKResult:
-    """Position IK for ``link_name`` of the constraint's robot; ``cst`` also
+    """IK for ``link_name`` of the constraint's robot to a 3-, 6- or 7-vector
+    ``target`` (position, + RPY, + quaternion x, y, z, w); ``cst`` also
     supplies the collision constraint when ``check_collision`` is set.
     ``q_seed`` (optional) is the first restart; the others are uniform in
     [lb, ub] from ``seed``."""
     import torch
 
     ts = time.time()
+    kind = len(target)
+    if kind not in (3, 6, 7):
+        raise RuntimeError("All poses must be 3 or 6 or 7 dimensional")
     cfg = config or IKConfig()
     dev = torch.device("cuda")
     stream = torch.cuda.current_stream()
@@ -69,20 +136,40 @@ def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[
         Q[0] = torch.tensor(np.asarray(q_seed, float), device=dev, dtype=torch.float64)
     poses = torch.empty((n_nodes * 7, R), device=dev, dtype=torch.float64)
     J = torch.empty((3, D, R), device=dev, dtype=torch.float32)
+    Ju = [torch.empty((3, D, R), device=dev, dtype=torch.float32) for _ in range(3 if kind > 3 else 0)]
+    units = ([1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0])
     vals = torch.empty((max(C, 1), R), device=dev, dtype=torch.float32)
     cjac = torch.empty((max(C, 1), D, R), device=dev, dtype=torch.float32)
     eye = torch.eye(D, device=dev, dtype=torch.float64)
     use_col = check_collision and C > 0
 
-    def position_error(qs):
+    def pose_error(qs):
+        """Residual (R, kind) of LinkPoseCst, the link quaternion and RPY."""
         cst._dev_link_poses(qs.data_ptr(), R, R, 1, poses.data_ptr(), sh)
-        return poses[7 * link:7 * link + 3].T - tgt  # (R, 3)
+        pose = poses[7 * link:7 * link + 7].T  # (R, 7): x, y, z, qx, qy, qz, qw
+        quat = pose[:, 3:7]
+        rpy = _rpy_of(quat[:, 0], quat[:, 1], quat[:, 2], quat[:, 3])
+        e = pose[:, :3] - tgt[:3]
+        if kind == 6:
+            e = torch.cat([e, rpy - tgt[3:]], dim=1)
+        elif kind == 7:
+            e = torch.cat([e, quat - tgt[3:]], dim=1)
+        return e, quat, rpy
 
     for _ in range(cfg.n_iter):
         qs = Q.T.contiguous()
-        e = position_error(qs)
+        e, quat, rpy = pose_error(qs)
         cst._dev_link_jacobian(qs.data_ptr(), R, R, link, [], J.data_ptr(), sh)
         A = J.permute(2, 0, 1).double()  # (R, 3, D)
+        if kind > 3:
+            U = _rot_matrix(quat)  # columns: the link's world axes
+            w = torch.zeros((R, 3, D), device=dev, dtype=torch.float64)
+            for k in range(3):
+                cst._dev_link_jacobian(qs.data_ptr(), R, R, link, units[k], Ju[k].data_ptr(), sh)
+                Dk = Ju[k].permute(2, 0, 1).double() - A  # (R, 3, D) = w_j x u_k
+                uk = U[:, :, k:k + 1].expand(-1, -1, D)
+                w += 0.5 * torch.linalg.cross(uk, Dk, dim=1)
+            A = torch.cat([A, _rotation_rows(kind, quat, rpy, w)], dim=1)
         r = e
         if use_col:
             cst._dev_evaluate(qs.data_ptr(), R, R, vals.data_ptr(), cjac.data_ptr(), sh)
@@ -98,7 +185,7 @@ def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[
         Q = torch.minimum(torch.maximum(Q + step, lo), hi)
 
     qs = Q.T.contiguous()
-    err2 = (position_error(qs) ** 2).sum(dim=1)
+    err2 = (pose_error(qs)[0] ** 2).sum(dim=1)
     ok = err2 <= cfg.acceptable_error
     if check_collision:
         valid = torch.empty(R, device=dev, dtype=torch.uint8)

@@ -750,3 +750,47 @@ def test_ik_batched_restarts():
     assert solved >= 3
     far = solve_ik_batched(pc, link, [5.0, 5.0, 5.0], lb, ub, n_restarts=32, seed=0)
     assert not far.success
+
+
+def _oracle_link_pose(wl, link_name, q):
+    from oracle import urdf_parse
+
+    kin = urdf_parse.kinematic_model(wl.robot.urdf)
+    kin.set_joint_positions(kin.get_joint_ids(list(wl.robot.joint_names)), list(q), True)
+    return np.array(kin.get_link_pose(kin.get_link_ids([link_name])[0]))  # x, y, z, qx, qy, qz, qw
+
+
+@pytest.mark.parametrize("kind", [6, 7])
+def test_ik_batched_pose_targets(kind):
+    """Batched IK to position + RPY (6) and position + quaternion (7) targets
+    (LinkPoseCst residuals: getRPY() - rpy, quat.coeffs() - q) reached by valid
+    configurations of the two-bar scene; answers checked with the oracle's
+    exact FK, getRPY and validity (squared residual <= 1e-6)."""
+    from plainmp_amd.ik import solve_ik_batched
+
+    wl = synth.workload("plain_two_bars", scale=1 / 16)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    link = wl.robot.sphere_specs[-1][0]
+    ok = oc.is_valid_batch(wl.Q, 8).astype(bool)
+    lb, ub = wl.robot.lower, wl.robot.upper
+
+    def residual(q, target):
+        p = _oracle_link_pose(wl, link, q)
+        rot = np.array(_oracle.quat_to_rpy(*p[3:])) if kind == 6 else p[3:]
+        return np.concatenate([p[:3], rot]) - target
+
+    solved = 0
+    for q_star in wl.Q[ok][:4]:
+        target = residual(q_star, np.zeros(kind))
+        res = solve_ik_batched(pc, link, target, lb, ub, n_restarts=256, seed=2)
+        if not res.success:
+            continue
+        solved += 1
+        r = residual(res.q, target)
+        assert r @ r <= 1e-6
+        assert bool(oc.is_valid_batch(res.q[None], 1)[0])
+        assert np.all(res.q >= lb - 1e-12) and np.all(res.q <= ub + 1e-12)
+    _report("ik_pose", kind=kind, solved=solved, of=4)
+    assert solved >= 3
+    with pytest.raises(RuntimeError):
+        solve_ik_batched(pc, link, [0.0] * 4, lb, ub, n_restarts=8)

@@ -115,6 +115,35 @@ def test_fk_exact_matches_numpy_homogeneous(robot_fn):
             assert np.allclose(_quat_to_matrix(*p[3:]), ref[ln][:3, :3], atol=1e-12)
 
 
+def test_quat_to_rpy_round_trip_and_gimbal_lock():
+    """getRPY (transform.hpp:93-120): inverse of setQuaternionFromRPY away from
+    |pitch| = pi/2; the two gimbal-lock branches as known answers."""
+    rng = np.random.default_rng(3)
+
+    def rot(q):
+        x, y, z, w = q
+        return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                         [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                         [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+    for _ in range(200):
+        r, p, y = rng.uniform(-3.1, 3.1), rng.uniform(-1.5, 1.5), rng.uniform(-3.1, 3.1)
+        assert np.allclose(_oracle.quat_to_rpy(*_oracle.quat_from_rpy(r, p, y)), [r, p, y], atol=1e-9)
+    # at the lock only yaw -/+ roll is defined.  The reference's branch for
+    # pitch = -pi/2 (yaw = -2 atan2(x, y)) reproduces the rotation; the one
+    # for +pi/2 (yaw = 2 atan2(x, y)) returns roll - yaw where yaw - roll
+    # would, a quirk kept as the reference computes it
+    wrap = lambda a: (a + math.pi) % (2 * math.pi) - math.pi  # noqa: E731
+    for r, y in [(0.4, -0.7), (0.0, 0.5), (-0.2, 1.0)]:
+        q = _oracle.quat_from_rpy(r, -math.pi / 2, y)
+        rr = _oracle.quat_to_rpy(*q)
+        assert rr[0] == 0.0 and rr[1] == -math.pi / 2
+        assert np.allclose(rot(_oracle.quat_from_rpy(*rr)), rot(q), atol=1e-9)
+        rr = _oracle.quat_to_rpy(*_oracle.quat_from_rpy(r, math.pi / 2, y))
+        assert rr[0] == 0.0 and rr[1] == math.pi / 2
+        assert abs(wrap(rr[2] - (r - y))) < 1e-9
+
+
 def test_fk_known_answers():
     robot = synth.panda_like(0)
     kin = urdf_parse.kinematic_model(robot.urdf)
