@@ -148,7 +148,7 @@ def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[
         cst._dev_link_poses(qs.data_ptr(), R, R, 1, poses.data_ptr(), sh)
         pose = poses[7 * link:7 * link + 7].T  # (R, 7): x, y, z, qx, qy, qz, qw
         quat = pose[:, 3:7]
-        rpy = _rpy_of(quat[:, 0], quat[:, 1], quat[:, 2], quat[:, 3])
+        rpy = _rpy_of(quat[:, 0], quat[:, 1], quat[:, 2], quat[:, 3]) if kind == 6 else None
         e = pose[:, :3] - tgt[:3]
         if kind == 6:
             e = torch.cat([e, rpy - tgt[3:]], dim=1)
