@@ -168,6 +168,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(wl)
         out["speedup_vs_cpu_configs"] = configs_per_s / out["cpu_baseline"]["value"]
+        out["speedup_vs_cpu_one_thread_configs"] = configs_per_s / out["cpu_baseline"]["one_thread"]["value"]
         if not args.no_extra:
             out["cpu_baseline_evaluate"] = _cpu_evaluate_baseline()
             out["speedup_vs_cpu_jacobians"] = (out["extra"]["grad_mixed32_evaluate"]["jacobians_per_s"]
@@ -392,28 +393,36 @@ def _plain_per_call(wl, mv, n=200):
     return {"edges": n, "s": dt, "edges_per_s": n / dt}
 
 
-def _cpu_baseline(wl, budget_s=2.0):
-    """The FP64 oracle (a C++ restatement of the reference path) on host threads."""
+def _cpu_baseline(wl, budget_s=2.0, one_thread_s=1.0):
+    """The FP64 oracle (a C++ restatement of the reference path) on host
+    threads, and on one thread (the reference's own single-threaded call)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from support import oracle_cst
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     oc = oracle_cst(wl)
     E = len(wl.S1)
-    edges = checked = 0
-    t0 = time.perf_counter()
-    k = 0
-    while time.perf_counter() - t0 < budget_s:
-        sl = slice((k * 1024) % E, (k * 1024) % E + 1024)
-        ok, n = oc.check_motions(wl.S1[sl], wl.S2[sl], True, list(wl.box_width), 0.0, threads)
-        edges += len(ok)
-        checked += int(n.sum())
-        k += 1
-    dt = time.perf_counter() - t0
+
+    def sample(threads, budget, chunk):
+        edges = checked = 0
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < budget:
+            sl = slice((k * chunk) % E, (k * chunk) % E + chunk)
+            ok, n = oc.check_motions(wl.S1[sl], wl.S2[sl], True, list(wl.box_width), 0.0, threads)
+            edges += len(ok)
+            checked += int(n.sum())
+            k += 1
+        return edges, checked, time.perf_counter() - t0
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    edges, checked, dt = sample(threads, budget_s, 1024)
+    e1, c1, dt1 = sample(1, one_thread_s, 64)
     return {"value": checked / dt, "unit": "configs/s", "cores": threads, "kind": "port",
             "edges_per_s": edges / dt,
             "sample": f"{edges} edges of edges_mixed16 ({checked} waypoints checked in reference order), "
-                      f"{dt:.1f} s wall on {threads} threads"}
+                      f"{dt:.1f} s wall on {threads} threads",
+            "one_thread": {"value": c1 / dt1, "unit": "configs/s", "edges_per_s": e1 / dt1,
+                           "sample": f"{e1} edges ({c1} waypoints) in {dt1:.1f} s on 1 thread"}}
 
 
 def _cpu_evaluate_baseline(budget_s=2.0):
