@@ -11,7 +11,8 @@
 //      code and, within a code, by index;
 //   3. a bitonic sort of the keys, padded to a power of two (at least
 //      kSortChunk) with all-ones keys: stages whose partner distance is below
-//      kSortChunk / 2 run in LDS, the others as one pass over global memory;
+//      kSortChunk run in LDS, the others over global memory, two stages a
+//      pass where both partner distances are at least kSortChunk;
 //   4. one wave per tile of kCloudTile consecutive sorted points: FP64 bounds,
 //      the FP32 centre and outward-rounded FP32 bounds, the matrix-core B
 //      operands (-2 p', |p'|^2) of the points in tile-local coordinates
@@ -185,6 +186,27 @@ __global__ void __launch_bounds__(256) merge_global_kernel(uint64_t* keys, int64
   }
 }
 
+// Stages j and j / 2 of stage k (j / 2 >= kSortChunk) in one pass: a thread
+// holds the four keys i0, i0 + j / 2, i0 + j, i0 + 3 j / 2 (i0 with both bits
+// clear), compares across j, then across j / 2.  All four lie in one block
+// of size k, so they share a direction.
+__global__ void __launch_bounds__(256) merge_global2_kernel(uint64_t* keys, int64_t npad, int64_t k, int64_t j) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npad / 4) return;
+  const int64_t h = j >> 1;
+  const int64_t i0 = 2 * j * (t / h) + (t % h);
+  const bool ascending = (i0 & k) == 0;
+  uint64_t a = keys[i0], b = keys[i0 + h], c = keys[i0 + j], d = keys[i0 + j + h];
+  cmpx(a, c, ascending);
+  cmpx(b, d, ascending);
+  cmpx(a, b, ascending);
+  cmpx(c, d, ascending);
+  keys[i0] = a;
+  keys[i0 + h] = b;
+  keys[i0 + j] = c;
+  keys[i0 + j + h] = d;
+}
+
 __device__ inline double wave_min(double v) {
   for (int m = 32; m > 0; m >>= 1) v = fmin(v, __shfl_xor(v, m));
   return v;
@@ -299,7 +321,11 @@ hipError_t launch_cloud_build(const double* pts, int64_t n, int32_t tile_base, C
   const unsigned chunks = (unsigned)(z.n_keys / kSortChunk);
   hipLaunchKernelGGL(sort_chunk_kernel, dim3(chunks), dim3(kSortChunk / 2), 0, st, keys);
   for (int64_t k = 2 * kSortChunk; k <= z.n_keys; k <<= 1) {
-    for (int64_t j = k >> 1; j >= kSortChunk; j >>= 1)
+    int64_t j = k >> 1;
+    for (; j >= 2 * kSortChunk; j >>= 2)
+      hipLaunchKernelGGL(merge_global2_kernel, dim3((unsigned)((z.n_keys / 4 + 255) / 256)), dim3(256), 0, st, keys,
+                         z.n_keys, k, j);
+    if (j >= kSortChunk)
       hipLaunchKernelGGL(merge_global_kernel, dim3((unsigned)((z.n_keys / 2 + 255) / 256)), dim3(256), 0, st, keys,
                          z.n_keys, k, j);
     hipLaunchKernelGGL(merge_chunk_kernel, dim3(chunks), dim3(kSortChunk / 2), 0, st, keys, k);
