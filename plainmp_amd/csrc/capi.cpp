@@ -82,8 +82,10 @@ struct DevBuf {
 // cannot change what a replay reads.
 struct EdgeBufs {
   DevBuf width, inv_width, pend, counter;
+  std::vector<double> h_width;  // what width / inv_width hold, so a repeated call uploads nothing
   void release() {
     for (DevBuf* b : {&width, &inv_width, &pend, &counter}) b->release();
+    h_width.clear();
   }
 };
 
@@ -1185,8 +1187,16 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
   ea.E = E;
   ea.vtype = v->type;
   ea.resolution = v->resolution;
-  ea.width = eb.width.upload(w, st(ctx));
-  ea.inv_width = eb.inv_width.upload(iw, st(ctx));
+  // The width tables change only with the validator: a call that repeats the
+  // last one's widths enqueues no copy (and, the copies being synchronous,
+  // does not wait for the stream).
+  if (eb.h_width != w || eb.width.ptr == nullptr) {
+    eb.width.upload(w, st(ctx));
+    eb.inv_width.upload(iw, st(ctx));
+    eb.h_width = w;
+  }
+  ea.width = (const double*)eb.width.ptr;
+  ea.inv_width = (const double*)eb.inv_width.ptr;
   ea.seq = (const uint8_t*)ctx->seq.ptr;
   // The kernels leave the counters at zero after every launch; a new buffer
   // starts there.
