@@ -166,6 +166,7 @@ struct pmp_context {
   // misc
   DevBuf traj_off, traj_w2;
   DevBuf sqp_rows, sqp_tables;  // SQP assembly: evaluate rows of every waypoint; offsets / box widths
+  std::vector<int64_t> h_sqp_tables;  // what sqp_tables holds
   std::vector<int64_t> h_traj_off;
   std::vector<double> h_traj_w2;
   EdgeBufs eb;  // edge buffers of direct calls
@@ -1095,6 +1096,21 @@ pmp_status pmp_traj_sqp_ineq(pmp_context* ctx, const double* X, int64_t ldx, int
     const int D = ctx->n_cols, C = cst_dim_of(ctx);
     std::vector<int64_t> row_off(n_traj + 1), nnz_off(n_traj + 1), cp_off(n_traj + 1);
     sqp_sizes(ctx, offsets, n_traj, msbox != nullptr, row_off.data(), nnz_off.data(), cp_off.data());
+    // offsets, row / nnz offsets and box widths in one small table, uploaded
+    // (a synchronous copy) only when it changes, and before the evaluate
+    // launch so that the copy never waits for it
+    std::vector<int64_t> tab;
+    tab.insert(tab.end(), offsets, offsets + n_traj + 1);
+    tab.insert(tab.end(), row_off.begin(), row_off.end());
+    tab.insert(tab.end(), nnz_off.begin(), nnz_off.end());
+    const size_t box_at = tab.size();
+    tab.resize(box_at + D);
+    if (msbox) std::memcpy(&tab[box_at], msbox, sizeof(double) * D);
+    if (tab != ctx->h_sqp_tables || ctx->sqp_tables.ptr == nullptr) {
+      ctx->sqp_tables.upload(tab, st(ctx));
+      ctx->h_sqp_tables = tab;
+    }
+    const int64_t* dt = (const int64_t*)ctx->sqp_tables.ptr;
     // evaluate rows of every waypoint (the plain evaluate launch over N)
     float* rows = (float*)ctx->sqp_rows.get(sizeof(float) * std::max<int64_t>(1, (int64_t)C * (D + 1) * N));
     if (N > 0 && C > 0) {
@@ -1104,15 +1120,6 @@ pmp_status pmp_traj_sqp_ineq(pmp_context* ctx, const double* X, int64_t ldx, int
       e.whole_chunks = ctx->sched.eval_whole_chunks;
       PMP_HIP(launch_eval_units(ctx->k32, ctx->s32, X, ldx, N, e, st(ctx)));
     }
-    // offsets, row / nnz offsets and box widths in one small table
-    std::vector<int64_t> tab;
-    tab.insert(tab.end(), offsets, offsets + n_traj + 1);
-    tab.insert(tab.end(), row_off.begin(), row_off.end());
-    tab.insert(tab.end(), nnz_off.begin(), nnz_off.end());
-    const size_t box_at = tab.size();
-    tab.resize(box_at + D);
-    if (msbox) std::memcpy(&tab[box_at], msbox, sizeof(double) * D);
-    const int64_t* dt = ctx->sqp_tables.upload(tab, st(ctx));
     SqpArgs a{};
     a.X = X;
     a.ldx = ldx;
