@@ -333,7 +333,7 @@ def test_cloud_tile_build_matches_oracle(case):
     """The device build of a cloud's tile tables (bounds, Morton sort, tiles,
     matrix-core operands, super-tiles; pmp_cloud_build_tiles) against the
     oracle's restatement, bit-exact in every table."""
-    from plainmp_amd._native import kdtree as kd
+    from plainmp_amd import kdtree as kd
 
     rng = np.random.default_rng(11)
     if case == "one":
