@@ -171,6 +171,7 @@ struct pmp_context {
   std::vector<double> h_traj_w2;
   EdgeBufs eb;  // edge buffers of direct calls
   DevBuf seq, tmp_prims, tmp_cloud, ws_cen, ws_cloud, ws_flags;
+  DevBuf ws_vunit;  // per-configuration flags of the unit validity pass (kValidUnitsMaxBatch)
 };
 
 struct pmp_graph {
@@ -470,7 +471,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     for (DevBuf* b : {&ctx->nodes32, &ctx->nodes64, &ctx->sph32, &ctx->sph64, &ctx->meta, &ctx->pairs, &ctx->blocks, &ctx->items, &ctx->units, &ctx->unit_all, &ctx->ws_vjp,
                       &ctx->vprims32, &ctx->vprims64, &ctx->supers, &ctx->cull, &ctx->ecull, &ctx->ecull_range, &ctx->vorig, &ctx->cull_range, &ctx->rad_out,
                       &ctx->prims32, &ctx->prims64, &ctx->tiles32, &ctx->tiles64, &ctx->tileB, &ctx->pts64, &ctx->seq,
-                      &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen, &ctx->cloud_stage, &ctx->cloud_scratch,
+                      &ctx->tmp_prims, &ctx->tmp_cloud, &ctx->ws_cen, &ctx->cloud_stage, &ctx->cloud_scratch, &ctx->ws_vunit,
                       &ctx->ws_cloud, &ctx->ws_flags, &ctx->traj_off, &ctx->traj_w2, &ctx->sqp_rows, &ctx->sqp_tables})
       b->release();
     ctx->eb.release();
@@ -991,8 +992,16 @@ pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, in
       }
       flags = (int32_t*)ctx->ws_flags.get(sizeof(int32_t) * B);
     }
+    // small cloud-free batches: a lane per (configuration, unit); the flags
+    // start at zero (cleared when the buffer is new) and the pass leaves them so
+    int32_t* unit_flags = nullptr;
+    if (ctx->n_clouds == 0 && B <= kValidUnitsMaxBatch && ctx->k32.n_units > 0) {
+      const bool fresh = ctx->ws_vunit.bytes < sizeof(int32_t) * (size_t)kValidUnitsMaxBatch;
+      unit_flags = (int32_t*)ctx->ws_vunit.get(sizeof(int32_t) * kValidUnitsMaxBatch);
+      if (fresh) PMP_HIP(launch_fill_i32(unit_flags, 0, kValidUnitsMaxBatch, st(ctx)));
+    }
     PMP_HIP(launch_valid(ctx->k32, ctx->k64, ctx->s32, ctx->s64, q, ldq, B, ctx->band, valid, n_recheck,
-                         ctx->sched.block, cw, ctx->clouds.data(), ctx->n_clouds, flags, st(ctx)));
+                         ctx->sched.block, cw, ctx->clouds.data(), ctx->n_clouds, flags, unit_flags, st(ctx)));
   });
 }
 

@@ -291,7 +291,11 @@ struct ValidVis32T {
   }
 
   __device__ void blocks(const DevNode<float>& nd, const Pose<float>& P) {
-    if (nd.block_begin >= nd.block_end) return;
+    blocks_range(nd, P, nd.block_begin, nd.block_end);
+  }
+  // The node's sphere blocks [lo, hi).
+  __device__ __forceinline__ void blocks_range(const DevNode<float>& nd, const Pose<float>& P, int lo, int hi) {
+    if (lo >= hi) return;
     const M3<float> R = qmat(P.q);
     // The cull table assumes |q|^2 of this rotation within [n2_lo, n2_hi]
     // (so that |R v| <= grow |v|); a lane outside is left to the FP64 pass.
@@ -301,7 +305,7 @@ struct ValidVis32T {
       n_inb = 2;
     }
     const int n_solid = sc.vrange[kKindCloud];
-    for (int bi = nd.block_begin; bi < nd.block_end; ++bi) {
+    for (int bi = lo; bi < hi; ++bi) {
       if (!wave_any(alive())) return;
       const DevSphereBlock bk = cload(ka.blocks, bi);
       const int n = bk.n;
@@ -409,6 +413,21 @@ struct ValidVis32T {
 };
 
 using ValidVis32 = ValidVis32T<false>;
+
+// The FP32 validity test of one evaluate unit (a sphere block, or a self
+// link pair) of a configuration: FK stops at the unit's last node, blocks
+// and self pairs outside the unit are skipped.  Clouds are not handled (the
+// unit pass runs only for scenes without one).
+struct ValidUnitVis : ValidVis32 {
+  static constexpr bool kRanged = true;
+  DevEvalUnit u{};
+  __device__ int last_node() const { return u.last; }
+  __device__ int pair_lo() const { return u.k_lo; }
+  __device__ int pair_hi() const { return u.k_hi; }
+  __device__ void blocks(const DevNode<float>& nd, const Pose<float>& P) {
+    blocks_range(nd, P, max(nd.block_begin, u.b_lo), min(nd.block_end, u.b_hi));
+  }
+};
 
 // FP64 re-decision with the reference's own tests: per sphere the AABB
 // pre-test and is_outside (check_ext_collision), per self pair the strict
@@ -573,6 +592,34 @@ __global__ void __launch_bounds__(256) valid_kernel(KinArgs<float> k32, SceneArg
   v.idle = !active;
   run_fk(k32, qs, taylor, v);
   if (active) valid[b] = v.definite ? 0 : (v.uncertain ? 2 : 1);
+}
+
+// Small batches of a scene without clouds: a lane per (configuration, unit),
+// the unit being grid row y, so that a batch of a few thousand
+// configurations still spreads over the GPU.  Each lane ORs its outcome
+// into flags[b] (1: a certain collision, 2: a test in the band);
+// valid_units_finish_kernel turns the flags into the answer valid_kernel
+// gives (a certain collision anywhere decides, else any band test leaves
+// the configuration to the FP64 pass) and clears them for the next launch.
+__global__ void __launch_bounds__(64) valid_units_kernel(KinArgs<float> k32, SceneArgs<float> s32, const double* q,
+                                                         int64_t ldq, int64_t B, float band, int32_t* flags) {
+  const int64_t b = gid();
+  const bool active = b < B;
+  BatchQ qs{q, ldq, active ? b : B - 1};
+  ValidUnitVis v{{k32, s32, band}};
+  v.u = cload(k32.units, (int)blockIdx.y);
+  v.idle = !active;
+  run_fk(k32, qs, k32.base_type == PMP_BASE_FIXED, v);
+  const int f = (v.definite ? 1 : 0) | (v.uncertain ? 2 : 0);
+  if (active && f != 0) atomicOr(flags + b, f);
+}
+
+__global__ void __launch_bounds__(256) valid_units_finish_kernel(int32_t* flags, int64_t B, uint8_t* valid) {
+  const int64_t b = gid();
+  if (b >= B) return;
+  const int f = flags[b];
+  valid[b] = (f & 1) ? 0 : ((f & 2) ? 2 : 1);
+  flags[b] = 0;
 }
 
 // FP64 re-decision of the configurations left pending by valid_kernel.
@@ -1618,10 +1665,17 @@ hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc
 hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const double* q, int64_t ldq, int64_t B, float band,
                         uint8_t* valid, int32_t* n_recheck, int block, const CloudWork& cw,
-                        const CloudRange* clouds, int n_clouds, int32_t* flags, hipStream_t st) {
+                        const CloudRange* clouds, int n_clouds, int32_t* flags, int32_t* unit_flags,
+                        hipStream_t st) {
   if (block <= 0) block = 256;
-  hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), 0, st, k32, s32, q, ldq, B, band, valid,
-                     cw);
+  if (unit_flags && n_clouds == 0 && k32.n_units > 0) {
+    hipLaunchKernelGGL(valid_units_kernel, dim3(blocks_for(B, 64), (unsigned)k32.n_units), dim3(64), 0, st, k32, s32,
+                       q, ldq, B, band, unit_flags);
+    hipLaunchKernelGGL(valid_units_finish_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, unit_flags, B, valid);
+  } else {
+    hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), 0, st, k32, s32, q, ldq, B, band,
+                       valid, cw);
+  }
   // sphere-vs-cloud queries on the matrix cores, one pass per cloud
   const int64_t Q = (int64_t)k32.n_ext_spheres * B;
   for (int oc = 0; oc < n_clouds && Q > 0; ++oc) {

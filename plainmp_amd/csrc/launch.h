@@ -78,7 +78,12 @@ hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc
 hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
                         const SceneArgs<double>& s64, const double* q, int64_t ldq, int64_t B, float band,
                         uint8_t* valid, int32_t* n_recheck, int block, const CloudWork& cw,
-                        const CloudRange* clouds, int n_clouds, int32_t* flags, hipStream_t st);
+                        const CloudRange* clouds, int n_clouds, int32_t* flags, int32_t* unit_flags,
+                        hipStream_t st);
+// Batches up to this size in scenes without clouds take the pass with a lane
+// per (configuration, evaluate unit); unit_flags: [B] int32, zero on entry
+// and left zero.
+constexpr int64_t kValidUnitsMaxBatch = 16384;
 // cloud.hip
 hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                               const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
