@@ -36,6 +36,20 @@ def test_valid_plain_two_bars_full():
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("n_configs", [16384, 20480])
+def test_valid_both_batch_paths(n_configs):
+    """Cloud-free validity takes a lane per (configuration, unit) up to 16,384
+    configurations and a lane per configuration above: both answers equal
+    the oracle's, bit-exact (the band re-decided in FP64)."""
+    wl = synth.workload("plain_two_bars", scale=n_configs / 1024)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref = oc.is_valid_batch(wl.Q, 8).astype(bool)
+    got, n_re = pc.is_valid_batch_stats(wl.Q)
+    _report("valid_paths", B=len(ref), valid_frac=ref.mean(), rechecked=n_re)
+    assert len(ref) == n_configs
+    assert np.array_equal(got, ref)
+
+
 def test_clearance_and_centers_plain_two_bars():
     wl = synth.workload("plain_two_bars")
     oc, pc = oracle_cst(wl), product_cst(wl)
