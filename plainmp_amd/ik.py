@@ -179,7 +179,9 @@ def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[
             A = torch.cat([A, -w * cjac[:C].permute(2, 0, 1).double() * act[..., None]], dim=1)
             r = torch.cat([e, w * (cfg.margin - v) * act], dim=1)
         At = A.transpose(1, 2)
-        step = -torch.linalg.solve(At @ A + cfg.damping * eye, (At @ r[..., None]))[..., 0]
+        # A^T A + damping I is positive definite: solve_ex skips the error
+        # check, whose read of the result on the host would stall every iteration
+        step = -torch.linalg.solve_ex(At @ A + cfg.damping * eye, (At @ r[..., None]))[0][..., 0]
         norm = step.abs().max(dim=1, keepdim=True).values
         step = step * torch.clamp(cfg.max_step / torch.clamp(norm, min=1e-300), max=1.0)
         Q = torch.minimum(torch.maximum(Q + step, lo), hi)
