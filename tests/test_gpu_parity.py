@@ -50,6 +50,25 @@ def test_valid_both_batch_paths(n_configs):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("n_configs", [1024, 20480])
+def test_valid_nan_configurations(n_configs):
+    """A NaN joint value makes every SDF comparison false, so the reference
+    reports the configuration invalid (AABB pre-test not outside, is_outside
+    false).  Both the unit pass and the one-lane pass agree with the oracle,
+    NaN rows included."""
+    wl = synth.workload("plain_two_bars", scale=n_configs / 1024)
+    Q = wl.Q.copy()
+    rng = np.random.default_rng(4)
+    rows = rng.choice(len(Q), 37, replace=False)
+    Q[rows, rng.integers(0, Q.shape[1], size=37)] = np.nan
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref = oc.is_valid_batch(Q, 8).astype(bool)
+    got, n_re = pc.is_valid_batch_stats(Q)
+    _report("valid_nan", B=len(Q), nan_rows=len(rows), rechecked=n_re)
+    assert not ref[rows].any()
+    assert np.array_equal(got, ref)
+
+
 def test_clearance_and_centers_plain_two_bars():
     wl = synth.workload("plain_two_bars")
     oc, pc = oracle_cst(wl), product_cst(wl)
