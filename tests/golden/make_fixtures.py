@@ -7,7 +7,8 @@ and scene come from plainmp_amd.synth by name; the URDF's SHA-256 is stored to
 catch generator drift) with the FP64 oracle's outputs.  known_answers.json
 holds values the reference's own tests and comments state
 (ompl/motion_validator.cpp:23-37, ompl/generate_sequence.py:37-39,
-tests/test_psdf.py:89-102,201-204).
+tests/test_psdf.py:89-102,201-204).  cloud_tiles.npz holds a small cloud
+and the oracle's tile tables for it.
 """
 
 from __future__ import annotations
@@ -85,6 +86,17 @@ def c5():
                     grad=np.concatenate(grad))
 
 
+def cloud_tiles():
+    """1,000 points of the shelf cloud generator with a run of duplicates, and
+    the oracle's tile tables (field bytes of the 48-byte FP32 records)."""
+    P = synth.shelf_cloud(1000, seed=9)
+    P[200:260] = P[17]
+    t = _oracle.cloud_tiles(P)
+    tiles = np.frombuffer(t["tiles"], np.uint8).reshape(-1, 48)[:, :40]
+    return dict(points=P, tiles=tiles, **{k: np.frombuffer(t[k], np.uint8) for k in
+                                         ("tiles64", "tile_b", "points64", "supers")})
+
+
 def known_answers():
     return {
         "farthest_sequence_8": [0, 7, 4, 2, 6, 5, 3, 1],
@@ -106,6 +118,7 @@ def main():
         print(name, {k: np.shape(v) for k, v in arrays.items()})
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
         json.dump(known_answers(), f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "cloud_tiles.npz"), **cloud_tiles())
 
 
 if __name__ == "__main__":

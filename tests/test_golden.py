@@ -85,3 +85,12 @@ def test_known_answers():
     g = _oracle.Prim.ground(1.0)  # the product's evaluate runs on the GPU: tests/test_gpu_golden.py
     assert g.evaluate(np.zeros(3)) == ka["ground_1_at_origin"]
     assert g.evaluate(np.array([1.0, 0.0, 0.0])) == ka["ground_1_at_x1"]
+
+
+def test_oracle_reproduces_cloud_tiles():
+    """The oracle's tile tables of the stored cloud equal the stored ones."""
+    f = np.load(os.path.join(GOLDEN, "cloud_tiles.npz"), allow_pickle=False)
+    t = _oracle.cloud_tiles(f["points"])
+    assert np.array_equal(np.frombuffer(t["tiles"], np.uint8).reshape(-1, 48)[:, :40], f["tiles"])
+    for k in ("tiles64", "tile_b", "points64", "supers"):
+        assert np.array_equal(np.frombuffer(t[k], np.uint8), f[k]), k

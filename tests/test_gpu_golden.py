@@ -72,3 +72,15 @@ def test_golden_known_answers_on_gpu():
     g = psdf.GroundSDF(1.0)
     assert g.evaluate(np.zeros(3)) == ka["ground_1_at_origin"]
     assert g.evaluate(np.array([1.0, 0.0, 0.0])) == ka["ground_1_at_x1"]
+
+
+def test_golden_cloud_tiles():
+    """The device build of the stored cloud's tile tables equals the stored
+    tables byte for byte."""
+    from plainmp_amd import kdtree
+
+    f = np.load(os.path.join(GOLDEN, "cloud_tiles.npz"), allow_pickle=False)
+    t = kdtree.build_cloud_tiles(f["points"])
+    assert np.array_equal(np.frombuffer(t["tiles"], np.uint8).reshape(-1, 48)[:, :40], f["tiles"])
+    for k in ("tiles64", "tile_b", "points64", "supers"):
+        assert np.array_equal(np.frombuffer(t[k], np.uint8), f[k]), k
