@@ -18,7 +18,16 @@
 //                        (tests/test_psdf.py:27-57,89-102,201-207), invariants
 //   KDTree               brute force (tests/test_kdtree.py:6-20)
 //   SphereCollisionCst   per-pair closed forms, is_valid == all(evaluate > 0)
-//                        (tests/test_constraints.py:47-53), FD Jacobian rows
+//                        (tests/test_constraints.py:47-53), FD Jacobian rows;
+//                        self collision: brute force over every listed sphere
+//                        pair on the 14-DOF dual arm (booleans on Taylor FK,
+//                        values on the independent numpy FK), the self row
+//                        against FP64 central differences of that brute-force
+//                        minimum, a two-sphere closed form with its gradient
+//   sphere reorder       the visiting order of libstdc++'s own
+//                        std::unordered_set<size_t> driven through the
+//                        reference's reserve / insert / begin / erase sequence
+//                        (tests/cpp/reorder_stdset.cpp), ties and duplicates
 //   sequence table       reference known answers (motion_validator.cpp:23-37,
 //                        generate_sequence.py:37-39) + permutation invariants
 //   smoothness cost      closed form of the second-difference matrix
@@ -29,9 +38,8 @@
 //                        sparsity pattern unchanged between evaluations
 //   cloud tile tables    invariants (a partition of the points, exact bounds,
 //                        ascending Morton codes, B operands giving |c - p|^2)
-// Parity unpinned: the libstdc++ unordered_set iteration order that drives
-// the sphere reorder (primitive_sphere_collision.cpp:53-96) is derived from
-// libstdc++'s hashtable rules, not observed from the reference binary.
+// Parity unpinned: OMPL's RealVectorStateSpace::interpolate, restated as the
+// lerp s1 + t (s2 - s1) (no reference test pins interpolated states).
 #pragma once
 
 #include <algorithm>

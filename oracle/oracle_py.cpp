@@ -112,6 +112,12 @@ PYBIND11_MODULE(_oracle, m) {
     V3 r = quat_to_rpy(q);
     return std::vector<double>{r.x, r.y, r.z};
   });
+  m.def("informative_order", [](const Arr& P) {  // (k, 3) local sphere positions -> visiting order
+    if (P.ndim() != 2 || P.shape(1) != 3) throw std::invalid_argument("expected (k, 3) positions");
+    std::vector<V3> pos;
+    for (py::ssize_t i = 0; i < P.shape(0); i++) pos.push_back({P.at(i, 0), P.at(i, 1), P.at(i, 2)});
+    return informative_order(pos);
+  });
   m.def("farthest_sequence", &farthest_sequence);
   m.def("sequence_table", []() { return sequence_table(); });
   m.def("box_step_ratio", &box_step_ratio);

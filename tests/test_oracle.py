@@ -7,6 +7,7 @@ differently, central finite differences and brute force.
 """
 
 import math
+import os
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -470,6 +471,192 @@ def test_sphere_reorder_is_farthest_point():
             dmin = np.linalg.norm(rest[:, None, :] - chosen[None, :, :], axis=-1).min(1)
             assert dmin[0] >= dmin.max() - 1e-12
         start += k
+
+
+def _reorder_groups(rng):
+    groups = [rng.normal(0.0, 0.1, size=(k, 3)) for k in range(1, 21)]
+    groups += [rng.normal(0.0, 0.1, size=(k, 3)) for k in (33, 64, 97)]
+    # ties: lattice points (equal distances everywhere) and repeated positions
+    grid = np.stack(np.meshgrid(*[np.arange(3.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    groups += [grid, np.zeros((5, 3)), np.repeat(rng.normal(size=(3, 3)), 3, axis=0)]
+    return groups
+
+
+def test_reorder_order_matches_libstdcxx_unordered_set(tmp_path):
+    # reorder_spheres_for_informative_collision_check (primitive_sphere_collision.cpp:53-96)
+    # run with libstdc++'s real std::unordered_set<size_t>; the oracle models its
+    # iteration order with a vector
+    import shutil
+    import subprocess
+
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "reorder_stdset.cpp")
+    exe = str(tmp_path / "reorder_stdset")
+    subprocess.run([cxx, "-O2", "-std=c++17", src, "-o", exe], check=True)
+    groups = _reorder_groups(np.random.default_rng(77))
+    text = f"{len(groups)}\n" + "".join(
+        f"{len(g)}\n" + "".join(f"{x:.17g} {y:.17g} {z:.17g}\n" for x, y, z in g) for g in groups)
+    out = subprocess.run([exe], input=text, capture_output=True, text=True, check=True).stdout.split("\n")
+    for g, line in zip(groups, out):
+        std_order = [int(v) for v in line.split()]
+        assert sorted(std_order) == list(range(len(g)))
+        assert _oracle.informative_order(g) == std_order, f"group of {len(g)}"
+
+
+def _group_layout(robot):
+    """Global sphere index ranges per group (spec order) and, per listed self
+    pair, the (i, j) sphere index pairs: every sphere of the first group that
+    carries link a against every sphere of the first group that carries link b
+    (primitive_sphere_collision.cpp:162-177, all k1 x k2 pairs)."""
+    sizes = [len(s[2]) for s in robot.sphere_specs]
+    start = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+    first = {}
+    for gi, s in enumerate(robot.sphere_specs):
+        first.setdefault(s[0], gi)
+    I, J = [], []
+    for a, b in robot.self_pairs:
+        ia = np.arange(start[first[a]], start[first[a] + 1])
+        ib = np.arange(start[first[b]], start[first[b] + 1])
+        I.append(np.repeat(ia, len(ib)))
+        J.append(np.tile(ib, len(ia)))
+    return start, np.concatenate(I), np.concatenate(J)
+
+
+def _numpy_centres(robot, q):
+    """World sphere centres in spec order from the independent numpy FK."""
+    T = numpy_fk(robot.urdf, dict(zip(robot.joint_names, q)))
+    return np.concatenate([(T[ln][:3, :3] @ np.asarray(pos, float).T).T + T[ln][:3, 3]
+                           for ln, pos, _, _ in robot.sphere_specs])
+
+
+def _self_brute(robot, q, radii):
+    """min over listed sphere pairs of |c1 - c2| - (r1 + r2) (exact FK), and the gap to the runner-up."""
+    _, I, J = _group_layout(robot)
+    C = _numpy_centres(robot, q)
+    d = np.sqrt(((C[I] - C[J]) ** 2).sum(-1)) - (radii[I] + radii[J])
+    s = np.sort(d)
+    return s[0], s[1] - s[0]
+
+
+def _dual_arm_configs(n, seed):
+    """n dual-arm configurations, a third each with a self collision, with the
+    nearest listed sphere pair within the 0.1 m cutoff, and uniform; picked
+    from a uniform pool by the oracle's Taylor-FK centres (selection only)."""
+    robot = synth.dual_arm(0)
+    pool = np.random.default_rng(seed).uniform(-2.9, 2.9, size=(80 * n, 14))
+    cst = oracle_cst(synth.Workload("pool", robot, []))
+    C = cst.sphere_centers_batch(pool, False)
+    _, r = cst.get_all_spheres()
+    _, I, J = _group_layout(robot)
+    m = (np.sqrt(((C[:, I] - C[:, J]) ** 2).sum(-1)) - (r[I] + r[J])).min(1)
+    k = n // 3
+    hit, near, far = np.flatnonzero(m < 0), np.flatnonzero((m > 0) & (m < 0.1)), np.flatnonzero(m >= 0.1)
+    assert len(hit) >= k and len(near) >= k
+    return pool[np.concatenate([hit[:k], near[:k], far[:n - 2 * k]])]
+
+
+@pytest.mark.parametrize("with_prims", [False, True])
+def test_self_collision_is_valid_vs_brute_force(with_prims):
+    # check_self_collision (primitive_sphere_collision.cpp:222-254): a listed pair
+    # collides iff |c1 - c2|^2 < (r1 + r2)^2; is_valid = ext ok and self ok
+    robot = synth.dual_arm(0)
+    prims = synth.mixed_primitives(24, seed=5) if with_prims else []
+    wl = synth.Workload("dual_arm_self", robot, prims, Q=_dual_arm_configs(300, 11))
+    cst = oracle_cst(wl)
+    C = cst.sphere_centers_batch(wl.Q, False)  # Taylor FK, as is_valid uses
+    _, r = cst.get_all_spheres()
+    _, I, J = _group_layout(robot)
+    D = C[:, I] - C[:, J]
+    sq = D[..., 0] * D[..., 0] + D[..., 1] * D[..., 1] + D[..., 2] * D[..., 2]
+    rs = (r[I] + r[J]) ** 2
+    self_ok = (sq >= rs).all(1)
+    ext_ok = np.ones(len(wl.Q), bool)
+    if prims:
+        P = [oracle_prim(p) for p in prims]
+        for b in range(len(wl.Q)):
+            ext = np.array([[p.evaluate(C[b, s]) - r[s] for p in P] for s in range(len(r))])
+            ext_ok[b] = bool((ext > 0).all())
+    ok = self_ok & ext_ok
+    got = cst.is_valid_batch(wl.Q, 4).astype(bool)
+    margin = np.abs(sq - rs).min(1)
+    keep = margin > 1e-12
+    assert keep.mean() > 0.99
+    assert np.array_equal(got[keep], ok[keep])
+    # both outcomes of the self test occur, including configurations whose only
+    # collision is a self collision
+    assert 0.2 < self_ok.mean() < 0.8
+    if prims:
+        assert (~self_ok & ext_ok).any() and (self_ok & ~ext_ok).any()
+
+
+def test_self_evaluate_value_vs_brute_force():
+    # evaluate_self_collision (primitive_sphere_collision.cpp:324-383): min over
+    # listed sphere pairs of |c1 - c2| - (r1 + r2), capped at the 0.1 cutoff
+    robot = synth.dual_arm(0)
+    wl = synth.Workload("dual_arm_self", robot, [], Q=_dual_arm_configs(120, 12))
+    cst = oracle_cst(wl, reorder=False)
+    assert cst.cst_dim() == 1
+    radii = np.concatenate([np.asarray(s[2], float) for s in robot.sphere_specs])
+    vals, _ = cst.evaluate_batch(wl.Q, 4)
+    below = 0
+    for b, q in enumerate(wl.Q):
+        dmin, _ = _self_brute(robot, q, radii)
+        assert abs(vals[b, 0] - min(dmin, 0.1)) < 1e-12
+        below += dmin < 0.1
+    assert 0.5 < below / len(wl.Q) < 0.9
+    # reordering the spheres does not change the minimum
+    v2, _ = oracle_cst(wl, reorder=True).evaluate_batch(wl.Q, 4)
+    assert np.allclose(v2, vals, atol=1e-14)
+
+
+def test_self_evaluate_row_vs_central_fd():
+    # the self row (c1 - c2)^T / |c1 - c2| (J1 - J2) (:375-381) against FP64
+    # central differences of the brute-force minimum, away from argmin switches
+    robot = synth.dual_arm(0)
+    wl = synth.Workload("dual_arm_self", robot, [], Q=_dual_arm_configs(200, 13))
+    cst = oracle_cst(wl)
+    radii = np.concatenate([np.asarray(s[2], float) for s in robot.sphere_specs])
+    h = 1e-6
+    checked = 0
+    for q in wl.Q:
+        dmin, gap = _self_brute(robot, q, radii)
+        if not (dmin < 0.09 and gap > 1e-4):
+            continue
+        v, Jrow = cst.evaluate(q)
+        fd = np.array([(_self_brute(robot, q + h * e, radii)[0] - _self_brute(robot, q - h * e, radii)[0]) / (2 * h)
+                       for e in np.eye(14)])
+        np.testing.assert_allclose(Jrow[0], fd, rtol=1e-6, atol=1e-8)
+        assert np.any(np.abs(fd) > 1e-3)
+        checked += 1
+        if checked == 12:
+            break
+    assert checked >= 8
+
+
+def test_self_two_spheres_closed_form():
+    # one revolute link about z carries a sphere at (1, 0, 0); the base carries
+    # one at (bx, by, 0): d(q) = |(cos q - bx, sin q - by)| - (ra + rb),
+    # dd/dq = ((cos q - bx)(-sin q) + (sin q - by) cos q) / |.|
+    urdf = _one_link_robot()
+    ra, rb, bx, by = 0.2, 0.15, 0.9, 0.6
+    spec_a = _oracle.SphereSpec("l1", np.array([[1.0, 0.0, 0.0]]), [ra], False)
+    spec_b = _oracle.SphereSpec("base", np.array([[bx, by, 0.0]]), [rb], False)
+    cst = _oracle.SphereCollisionCst(urdf_parse.kinematic_model(urdf), ["j1"], _oracle.BaseType.FIXED,
+                                     [spec_a, spec_b], [("l1", "base")], [], False)
+    for q in np.linspace(-1.0, 2.0, 31):
+        dx, dy = math.cos(q) - bx, math.sin(q) - by
+        n = math.hypot(dx, dy)
+        d = n - (ra + rb)
+        g = (dx * -math.sin(q) + dy * math.cos(q)) / n
+        v, Jrow = cst.evaluate(np.array([q]))
+        if d < 0.1:
+            assert abs(v[0] - d) < 1e-14 and abs(Jrow[0, 0] - g) < 1e-12
+        else:
+            assert v[0] == 0.1 and Jrow[0, 0] == 0.0
+        if abs(n * n - (ra + rb) ** 2) > 1e-4:  # Taylor FK moves the centre by < 3e-5
+            assert cst.is_valid(np.array([q])) == (n * n >= (ra + rb) ** 2)
 
 
 # ------------------------------------------------------ motion validators
