@@ -19,7 +19,11 @@
  *     column col of configuration b (ldq >= B).  Outputs use the same
  *     "batch index fastest" layout with leading dimension B.
  *   - Work is enqueued on the context's stream (pmp_set_stream); calls do
- *     not synchronise the host unless documented.
+ *     not synchronise the host unless documented.  Calls on one context
+ *     share its device workspaces: a call made on another stream than the
+ *     previous call waits (on the device, through an event) for that call's
+ *     work, so consecutive calls never overlap.  Calls recorded into a
+ *     caller's own stream capture are not ordered this way.
  *   - A context is not re-entrant: use one context per host thread.
  */
 #ifndef PLAINMP_AMD_H_
@@ -260,7 +264,11 @@ pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const doubl
 
 /* Same as pmp_check_edges_batch, captured once into a HIP graph and replayed
  * by pmp_graph_launch on the context stream.  The pointers must stay valid
- * while the graph exists. */
+ * while the graph exists, and the graph must be destroyed before its context.
+ * The graph reads the context's robot, scene and cull tables as they were at
+ * capture: after pmp_upload_robot, pmp_upload_scene or pmp_set_band (e.g. a
+ * set_sdf of the host mirror) pmp_graph_launch returns PMP_ERR_INVALID and
+ * the graph has to be captured again. */
 typedef struct pmp_graph pmp_graph;
 pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
                                  const pmp_validator_desc* validator, uint8_t* valid, int32_t* n_checked,

@@ -6,7 +6,8 @@
   Python bindings, linked against ``libplainmp_amd.so``.
 
 ``python plainmp_amd/build.py`` rebuilds both (run as a file: importing the
-package needs the extension).
+package needs the extension); ``--incremental`` recompiles only the sources
+that changed since their objects were built.
 """
 
 from __future__ import annotations
@@ -47,12 +48,32 @@ def _run(cmd, verbose):
         raise RuntimeError(f"build step failed: {cmd[0]} (exit {r.returncode})")
 
 
-def build_library(verbose: bool = False) -> str:
+def _stale(obj, src):
+    """An object is rebuilt when it is older than its source or any header."""
+    if not os.path.exists(obj):
+        return True
+    heads = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".h", ".hpp"))]
+    heads += [os.path.join(HERE, "..", "include", f) for f in os.listdir(os.path.join(HERE, "..", "include"))]
+    return os.path.getmtime(obj) < max(os.path.getmtime(f) for f in [src] + heads)
+
+
+def build_library(verbose: bool = False, force: bool = True) -> str:
+    """Each source is compiled to an object in parallel, then linked
+    (``force=False`` keeps objects newer than their sources and headers)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
-    srcs = [os.path.join(CSRC, "kernels.hip"), os.path.join(CSRC, "cloud.hip"),
-            os.path.join(CSRC, "cloud_build.hip"), os.path.join(CSRC, "capi.cpp")]
+    srcs = ["kernels.hip", "cloud.hip", "cloud_build.hip", "capi.cpp"]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, s + ".o") for s in srcs]
+    jobs = [[hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(srcs, objs)
+            if force or _stale(o, os.path.join(CSRC, s))]
+    with ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
+        for f in [ex.submit(_run, j, verbose) for j in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
-    _run([hipcc, *HIP_FLAGS, "-shared", *srcs, "-o", tmp], verbose)
+    _run([hipcc, *HIP_FLAGS, "-shared", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 
@@ -92,10 +113,10 @@ def build_extension(verbose: bool = False) -> str:
     return EXT
 
 
-def build(verbose: bool = False) -> None:
-    build_library(verbose)
+def build(verbose: bool = False, force: bool = True) -> None:
+    build_library(verbose, force)
     build_extension(verbose)
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, force="--incremental" not in sys.argv)

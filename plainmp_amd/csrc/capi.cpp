@@ -32,8 +32,20 @@ struct Error : std::runtime_error {
     if (!(cond)) throw Error(PMP_ERR_INVALID, (msg));   \
   } while (0)
 
+// The context whose work the current entry point enqueues (set by
+// set_device); its call is recorded for cross-stream ordering when the entry
+// point returns.
+thread_local pmp_context* g_call_ctx = nullptr;
+void record_call(pmp_context* ctx) noexcept;
+
 template <class F>
 pmp_status guarded(F&& f) {
+  struct Finish {
+    ~Finish() {
+      if (g_call_ctx) record_call(g_call_ctx);
+      g_call_ctx = nullptr;
+    }
+  } finish;
   try {
     f();
     return PMP_OK;
@@ -172,6 +184,13 @@ struct pmp_context {
   EdgeBufs eb;  // edge buffers of direct calls
   DevBuf seq, tmp_prims, tmp_cloud, ws_cen, ws_cloud, ws_flags;
   DevBuf ws_vunit;  // per-configuration flags of the unit validity pass (kValidUnitsMaxBatch)
+  // Cross-stream order of calls (they share the workspaces above): an event
+  // recorded on the stream of the last call, waited on by a call on another.
+  hipEvent_t call_ev = nullptr;
+  hipStream_t call_stream = nullptr;
+  bool call_recorded = false;
+  // Bumped whenever the device tables a captured graph reads are replaced.
+  uint64_t tables_gen = 0;
 };
 
 struct pmp_graph {
@@ -179,11 +198,27 @@ struct pmp_graph {
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
   EdgeBufs eb;
+  const pmp_context* ctx = nullptr;
+  uint64_t tables_gen = 0;  // the context's tables_gen at capture
 };
 
 namespace {
 
-void set_device(pmp_context* ctx) { PMP_HIP(hipSetDevice(ctx->device)); }
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// Every entry point that enqueues work calls this first.  Calls on one
+// context share its workspaces, so a call on another stream than the last
+// call's waits for that call's work first (nothing is ordered while the
+// stream is being captured).
+void set_device(pmp_context* ctx) {
+  PMP_HIP(hipSetDevice(ctx->device));
+  g_call_ctx = ctx;
+  if (ctx->call_recorded && ctx->call_stream != ctx->stream && !capturing(ctx->stream))
+    PMP_HIP(hipStreamWaitEvent(ctx->stream, ctx->call_ev, 0));
+}
 hipStream_t st(pmp_context* ctx) { return ctx->stream; }
 
 // FP64 quaternion product (Eigen order) and q * v (Eigen's formula, which for
@@ -340,6 +375,7 @@ void build_cull(pmp_context* ctx, double margin, std::vector<CullEntry>& ent, st
 // The validity table (boxes grown by the band) and the evaluate table (by
 // the 0.1 m cutoff of evaluate_ext_collision's prefilter).
 void rebuild_cull(pmp_context* ctx) {
+  ++ctx->tables_gen;
   std::vector<CullEntry> ent;
   std::vector<CullRange> rng;
   hipStream_t s0 = st(ctx);
@@ -434,6 +470,17 @@ void require_robot(pmp_context* ctx) {
 
 }  // namespace
 
+namespace {
+// End of an entry point that enqueued work: remember it for set_device.
+void record_call(pmp_context* ctx) noexcept {
+  if (capturing(ctx->stream)) return;
+  if (!ctx->call_ev && hipEventCreateWithFlags(&ctx->call_ev, hipEventDisableTiming) != hipSuccess) return;
+  if (hipEventRecord(ctx->call_ev, ctx->stream) != hipSuccess) return;
+  ctx->call_recorded = true;
+  ctx->call_stream = ctx->stream;
+}
+}  // namespace
+
 extern "C" {
 
 const char* pmp_last_error(void) { return g_last_error.c_str(); }
@@ -478,6 +525,7 @@ pmp_status pmp_destroy(pmp_context* ctx) {
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     if (ctx->edge_ev0) (void)hipEventDestroy(ctx->edge_ev0);
     if (ctx->edge_ev1) (void)hipEventDestroy(ctx->edge_ev1);
+    if (ctx->call_ev) (void)hipEventDestroy(ctx->call_ev);
     delete ctx;
   });
 }
@@ -522,6 +570,7 @@ pmp_status pmp_set_schedule(pmp_context* ctx, const pmp_schedule* s) {
 pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
   return guarded([&] {
     PMP_REQUIRE(ctx != nullptr && r != nullptr, "null argument");
+    ++ctx->tables_gen;  // captured graphs of the old tables must not replay
     const int N = r->n_nodes;
     PMP_REQUIRE(N >= 1 && N <= kMaxNodes, "n_nodes out of range");
     PMP_REQUIRE(r->node_parent && r->node_kind && r->node_q_col && r->node_quat && r->node_trans && r->node_axis &&
@@ -811,6 +860,7 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
 pmp_status pmp_upload_scene(pmp_context* ctx, const pmp_scene_desc* sc) {
   return guarded([&] {
     PMP_REQUIRE(ctx != nullptr && sc != nullptr, "null argument");
+    ++ctx->tables_gen;  // captured graphs of the old tables must not replay
     PMP_REQUIRE(sc->n_prims >= 0 && sc->n_prims <= kMaxPrims, "n_prims out of range");
     PMP_REQUIRE(sc->n_prims == 0 || sc->prims != nullptr, "missing primitive list");
     std::vector<DevPrim<float>> p32;
@@ -985,9 +1035,9 @@ pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, in
       const int64_t Q = (int64_t)ctx->k32.n_ext_spheres * B;
       cw.B = B;
       cw.cen = (float*)ctx->ws_cen.get(sizeof(float) * std::max<int64_t>(3 * Q, 1));
-      float* w = (float*)ctx->ws_cloud.get(sizeof(float) * std::max<int64_t>(3 * Q * ctx->n_clouds, 1));
-      for (int oc = 0; oc < ctx->n_clouds; ++oc) {  // per cloud: cap2, rr, d2
-        cw.cap2[oc] = w + 3 * Q * oc;
+      float* w = (float*)ctx->ws_cloud.get(sizeof(float) * std::max<int64_t>(4 * Q * ctx->n_clouds, 1));
+      for (int oc = 0; oc < ctx->n_clouds; ++oc) {  // per cloud: cap2, rr, then the d2 bracket (upper, lower)
+        cw.cap2[oc] = w + 4 * Q * oc;
         cw.rr[oc] = cw.cap2[oc] + Q;
       }
       flags = (int32_t*)ctx->ws_flags.get(sizeof(int32_t) * B);
@@ -1287,6 +1337,8 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
       PMP_HIP(le);
       PMP_HIP(ce);
       PMP_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+      g->ctx = ctx;
+      g->tables_gen = ctx->tables_gen;
     } catch (...) {
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
       if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -1301,6 +1353,8 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
 pmp_status pmp_graph_launch(pmp_graph* g) {
   return guarded([&] {
     PMP_REQUIRE(g != nullptr, "null graph");
+    PMP_REQUIRE(g->ctx->tables_gen == g->tables_gen,
+                "the context's robot, scene or band changed after this graph was captured; capture it again");
     PMP_HIP(hipGraphLaunch(g->exec, g->stream));
   });
 }

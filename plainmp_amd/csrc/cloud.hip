@@ -6,10 +6,11 @@
 //     |c - p|^2 = |c - o|^2 + ( |p - o|^2 - 2 (c - o) . (p - o) )
 // and the bracket over 16 centres x 16 points is one K = 4 MFMA:
 //     A (16 x 4) = [ c - o , 1 ],   B (4 x 16) = [ -2 (p - o) ; |p - o|^2 ].
-// Working in tile-local coordinates keeps every term near the tile size, so
-// the FP32 cancellation error stays around 1e-9 m^2 for the tiles that can
-// matter.  A wave handles 16 queries; a tile is skipped when no query's cap
-// reaches the tile's AABB.
+// Working in tile-local coordinates keeps every term near the tile size; the
+// FP32 cancellation error is bounded per tile by kCloudGemmErr (|c'|^2 +
+// rho^2) (fk.hpp), which validity decisions add to their band.  A wave
+// handles 16 queries; a tile is skipped when no query's cap reaches the
+// tile's AABB.
 #include <float.h>
 
 #include "fk.hpp"
@@ -21,14 +22,18 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // min |c - p|^2 over the cloud's tiles for queries q < Q whose cap2[q] >= 0;
 // centres in cen[(q / B * 3 + k) * B + q % B].  Writes out_d2[q] = min squared
-// distance (FLT_MAX when no point lies within the cap).  `skip` (optional,
-// per configuration q % B): queries of configurations already decided (0)
-// are not evaluated.  Super-tiles are tested first, then their tiles; a
-// (super-)tile is skipped when no query's cap reaches its AABB.
+// distance (FLT_MAX when no point lies within the cap).  With kBounds, the
+// minimum is bracketed instead: out_d2[q] = min over tiles of (d2 + err) and
+// out_lo[q] = min over tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 +
+// rho^2) per tile, so the exact minimum lies in [out_lo, out_d2].  `skip`
+// (optional, per configuration q % B): queries of configurations already
+// decided (0) are not evaluated.  Super-tiles are tested first, then their
+// tiles; a (super-)tile is skipped when no query's cap reaches its AABB.
+template <bool kBounds>
 __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, const float* tileB,
                                                         const CloudSuper* supers, int super_begin, int n_super,
                                                         const float* cen, const float* cap2, int64_t B, int64_t Q,
-                                                        const uint8_t* skip, float* out_d2) {
+                                                        const uint8_t* skip, float* out_d2, float* out_lo) {
   const int lane = threadIdx.x & 63;
   const int64_t q0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 16;
   if (q0 >= Q) return;  // wave-uniform
@@ -58,6 +63,7 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
     }
   }
   float best[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+  float lo[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   if (__any(capi >= 0.f)) {
     for (int si = 0; si < n_super; ++si) {
@@ -79,12 +85,19 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
         floatx4 d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[64 + lane], zero, 0, 0, 0);
         floatx4 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[128 + lane], zero, 0, 0, 0);
         floatx4 d3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[192 + lane], zero, 0, 0, 0);
+        const float rho2 = kBounds ? tile_rho2(T) : 0.f;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const float ox = rx[r] - T.o[0], oy = ry[r] - T.o[1], oz = rz[r] - T.o[2];
           const float off = ox * ox + oy * oy + oz * oz;
-          const float m = fminf(fminf(d0[r], d1[r]), fminf(d2[r], d3[r]));
-          best[r] = fminf(best[r], m + off);
+          const float m = fminf(fminf(d0[r], d1[r]), fminf(d2[r], d3[r])) + off;
+          if constexpr (kBounds) {
+            const float err = kCloudGemmErr * (off + rho2);
+            best[r] = fminf(best[r], m + err);
+            lo[r] = fminf(lo[r], m - err);
+          } else {
+            best[r] = fminf(best[r], m);
+          }
         }
       }
     }
@@ -92,32 +105,40 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   // minimum over the 16 lanes that share the rows (columns j = lane & 15)
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    for (int s = 1; s < 16; s <<= 1) best[r] = fminf(best[r], __shfl_xor(best[r], s, 64));
+    for (int s = 1; s < 16; s <<= 1) {
+      best[r] = fminf(best[r], __shfl_xor(best[r], s, 64));
+      if constexpr (kBounds) lo[r] = fminf(lo[r], __shfl_xor(lo[r], s, 64));
+    }
   if (i == 0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t q = q0 + 4 * k + r;
-      if (q < Q) out_d2[q] = best[r];
+      if (q < Q) {
+        out_d2[q] = best[r];
+        if constexpr (kBounds) out_lo[q] = lo[r];
+      }
     }
   }
 }
 
 // Collision decision per cloud query: collide iff sqrt(d2) - rc <= r, i.e.
-// d2 <= (r + rc)^2 (CloudSDF::is_outside, primitive_sdf.hpp:597-603).
+// d2 <= (r + rc)^2 (CloudSDF::is_outside, primitive_sdf.hpp:597-603), where
+// the exact minimum d2 lies in [lo[q], hi[q]] (the FP32 pass's bracket).
 // rr[q] = r + rc of the query, or < 0 when not queried.  flags[b]: bit 0 a
-// collision outside the band, bit 1 a decision inside the band.
-__global__ void __launch_bounds__(256) cloud_decide_kernel(const float* d2, const float* rr, int64_t B, int64_t Q,
-                                                           float band, int32_t* flags) {
+// collision outside the band, bit 1 a decision inside the band (the bracket
+// reaches within band (2R + band) of R^2).
+__global__ void __launch_bounds__(256) cloud_decide_kernel(const float* lo, const float* hi, const float* rr,
+                                                           int64_t B, int64_t Q, float band, int32_t* flags) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
   const float R = rr[q];
   if (R < 0.f) return;
-  const float m = d2[q] - R * R;
+  const float R2 = R * R, w = band * (2.f * R + band);
   const int64_t b = q % B;
-  if (fabsf(m) <= band * (2.f * R + band))
-    atomicOr(&flags[b], 2);
-  else if (m < 0.f)
+  if (hi[q] - R2 < -w)
     atomicOr(&flags[b], 1);
+  else if (!(lo[q] - R2 > w))
+    atomicOr(&flags[b], 2);
 }
 
 // valid[b] in {0, 1, 2} merged with the cloud flags.
@@ -155,17 +176,19 @@ hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float
   return hipGetLastError();
 }
 
-// The FP32 validity pass has filled cen / cap2 / rr; d2 is written after rr
-// (the workspace holds Q floats there).
+// The FP32 validity pass has filled cen / cap2 / rr; the bracket of the
+// squared distance is written after rr (the workspace holds 2 Q floats
+// there: upper ends, then lower ends).
 hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                               const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
                               int64_t Q, float band, uint8_t* valid, int32_t* flags, hipStream_t st) {
   auto blocks = [](int64_t n, int t) { return (unsigned)((n + t - 1) / t); };
-  float* d2 = const_cast<float*>(rr) + Q;
+  float* hi = const_cast<float*>(rr) + Q;
+  float* lo = hi + Q;
   hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, flags, 0, B);
-  hipLaunchKernelGGL(cloud_min_kernel, dim3(blocks((Q + 15) / 16 * 64, 256)), dim3(256), 0, st, tiles, tileB, supers,
-                     cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, d2);
-  hipLaunchKernelGGL(cloud_decide_kernel, dim3(blocks(Q, 256)), dim3(256), 0, st, d2, rr, B, Q, band, flags);
+  hipLaunchKernelGGL(cloud_min_kernel<true>, dim3(blocks((Q + 15) / 16 * 64, 256)), dim3(256), 0, st, tiles, tileB,
+                     supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, hi, lo);
+  hipLaunchKernelGGL(cloud_decide_kernel, dim3(blocks(Q, 256)), dim3(256), 0, st, lo, hi, rr, B, Q, band, flags);
   hipLaunchKernelGGL(cloud_merge_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, valid, flags, B);
   return hipGetLastError();
 }
@@ -173,8 +196,9 @@ hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const 
 hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                             const CloudRange& cr, const float* cen, const float* cap2, int64_t B, int64_t Q,
                             float* out_d2, hipStream_t st) {
-  hipLaunchKernelGGL(cloud_min_kernel, dim3((unsigned)(((Q + 15) / 16 * 64 + 255) / 256)), dim3(256), 0, st, tiles,
-                     tileB, supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2);
+  hipLaunchKernelGGL(cloud_min_kernel<false>, dim3((unsigned)(((Q + 15) / 16 * 64 + 255) / 256)), dim3(256), 0, st,
+                     tiles, tileB, supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2,
+                     (float*)nullptr);
   return hipGetLastError();
 }
 

@@ -24,6 +24,31 @@ namespace pmp {
 
 __device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
 
+// Error of the tile-local FP32 form of a squared distance to a cloud point,
+//   d2 = |c'|^2 + (|p'|^2 - 2 c'.p'),   c' = c - o, p' = p - o,
+// against the exact |c - p|^2 (c the FP32 query, p the FP64 point, o the
+// tile's FP32 centre).  With u = 2^-24, a = |c'| and rho >= |p - o| for every
+// point of the tile, the rounding of c', p' and |p'|^2 (u each), the four
+// products and sums of the K = 4 dot product (gamma_4 = 4u of
+// 2 a rho + rho^2), |c'|^2 (3u a^2) and the final sum (u d2), using
+// |c - p| <= a + rho, add up to at most 8u (a + rho)^2 <= 16u (a^2 + rho^2).
+// The band takes twice that, 2e-6 (a^2 + rho^2), in case the matrix cores
+// round less tightly than IEEE single precision.  For the compact tiles of a
+// dense cloud (a, rho of a few cm) it is ~1e-8 m^2; for tiles spread over
+// metres it exceeds the 1e-5 m validity band, so the decisions that use it
+// carry it as a per-tile term.
+constexpr float kCloudGemmErr = 2e-6f;
+
+// rho^2 of a tile: the squared distance from its centre o to the farthest
+// corner of its (outward-rounded) bounds.
+template <class Tile>
+__device__ __forceinline__ float tile_rho2(const Tile& T) {
+  const float ex = fmaxf(T.ub[0] - T.o[0], T.o[0] - T.lb[0]);
+  const float ey = fmaxf(T.ub[1] - T.o[1], T.o[1] - T.lb[1]);
+  const float ez = fmaxf(T.ub[2] - T.o[2], T.o[2] - T.lb[2]);
+  return ex * ex + ey * ey + ez * ez;
+}
+
 // Visitors that declare `static constexpr bool kBlocks = true` receive each
 // node's sphere blocks through blocks(node, pose) instead of one sphere() call
 // per sphere.

@@ -598,14 +598,23 @@ PYBIND11_MODULE(_native, m) {
       .def_readwrite("type", &ValidatorConfig::type)
       .def_readwrite("resolution", &ValidatorConfig::resolution)
       .def_readwrite("box_width", &ValidatorConfig::box_width);
+  // A captured edge batch.  It keeps its constraint (and so the device
+  // context) alive; launch() first brings the context's tables up to date
+  // with the constraint (a set_sdf or a moved primitive re-uploads them), and
+  // a graph captured before such a change then refuses to replay.
   struct Graph {
     pmp_graph* g = nullptr;
+    std::shared_ptr<SphereCollisionCst> cst;
     ~Graph() {
       if (g) pmp_graph_destroy(g);
     }
   };
   py::class_<Graph, std::shared_ptr<Graph>>(mo, "EdgeGraph")
-      .def("launch", [](Graph& g) { check(pmp_graph_launch(g.g)); })
+      .def("launch", [](Graph& g) {
+        if (!g.g) throw std::invalid_argument("graph destroyed");
+        g.cst->sync_batch_tables();
+        check(pmp_graph_launch(g.g));
+      })
       .def("destroy", [](Graph& g) {
         if (g.g) check(pmp_graph_destroy(g.g));
         g.g = nullptr;
@@ -694,6 +703,7 @@ PYBIND11_MODULE(_native, m) {
              pmp_context* ctx = v.constraint()->device_context((void*)stream);
              pmp_validator_desc d = v.desc();
              auto g = std::make_shared<Graph>();
+             g->cst = v.constraint();
              check(pmp_check_edges_graph(ctx, (const double*)s1, (const double*)s2, lds, E, &d, (uint8_t*)valid,
                                          (int32_t*)nchk, &g->g));
              return g;

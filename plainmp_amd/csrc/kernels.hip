@@ -49,12 +49,15 @@ struct alignas(16) Quad4 {
 // (FLT_MAX when none).  Super-tiles, then tiles, are skipped for the whole
 // wave when no `live` lane's g reaches their box; a kept tile's points are
 // compared in tile-local coordinates, |c - p|^2 = |c'|^2 + (|p'|^2 - 2 c'.p'),
-// from the matrix-core pass's operands (-2 p', |p'|^2).
+// from the matrix-core pass's operands (-2 p', |p'|^2).  With kBounds the
+// minimum is bracketed: best = min over tiles of (d2 + err), lo = min over
+// tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 + rho^2) of the tile.
+template <bool kBounds>
 __device__ __forceinline__ void cloud_block_min(const SceneArgs<float>& sc, int oc, const V3<float> (&c)[kBlockSpheres],
                                                 int n, V3<float> g, float reach2, bool live,
-                                                float (&best)[kBlockSpheres]) {
+                                                float (&best)[kBlockSpheres], float (&lo)[kBlockSpheres]) {
 #pragma unroll
-  for (int k = 0; k < kBlockSpheres; ++k) best[k] = FLT_MAX;
+  for (int k = 0; k < kBlockSpheres; ++k) best[k] = lo[k] = FLT_MAX;
   const int s_end = sc.super_at[oc + 1];
   for (int si = sc.super_at[oc]; si < s_end; ++si) {
     const CloudSuper S = cload(sc.supers, si);
@@ -85,8 +88,18 @@ __device__ __forceinline__ void cloud_block_min(const SceneArgs<float>& sc, int 
           }
         }
       }
+      const float rho2 = kBounds ? tile_rho2(T) : 0.f;
 #pragma unroll
-      for (int k = 0; k < kBlockSpheres; ++k) best[k] = fminf(best[k], tb[k] + dot(cl[k], cl[k]));
+      for (int k = 0; k < kBlockSpheres; ++k) {
+        const float off = dot(cl[k], cl[k]);
+        if constexpr (kBounds) {
+          const float err = kCloudGemmErr * (off + rho2);
+          best[k] = fminf(best[k], tb[k] + off + err);
+          lo[k] = fminf(lo[k], tb[k] + off - err);
+        } else {
+          best[k] = fminf(best[k], tb[k] + off);
+        }
+      }
     }
   }
 }
@@ -370,7 +383,8 @@ struct ValidVis32T {
   // the same operands as the matrix-core pass.  The decision is that of the
   // cloud pass (CloudSDF::is_outside, primitive_sdf.hpp:597-603, after the
   // AABB pre-test): collide iff min |c - p|^2 <= (r + rc)^2, with the band
-  // around it left to the FP64 walk.
+  // around it, widened by the tile-local form's error bound, left to the
+  // FP64 walk.
   __device__ __forceinline__ void cloud_inline(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
                                                V3<float> g, float n2_hi) {
     const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
@@ -379,20 +393,20 @@ struct ValidVis32T {
       const DevPrim<float> Pc = cload(vp, p);
       const float rc = Pc.p0;
       const float reach = bk.G * n2_hi + rc + band + 1e-5f;
-      float best[kBlockSpheres];
-      cloud_block_min(sc, (Pc.flags >> 4) & 15, c, n, g, reach * reach, alive(), best);
+      float hi[kBlockSpheres], lo[kBlockSpheres];
+      cloud_block_min<true>(sc, (Pc.flags >> 4) & 15, c, n, g, reach * reach, alive(), hi, lo);
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         if (k < n) {
           const float r = bk.s[k][3];
           if (!alive() || outside_aabb(Pc, c[k], r)) continue;
-          const float R = r + rc;
-          const float m = best[k] - R * R;
-          if (fabsf(m) <= band * (2.0f * R + band)) {
+          // the exact minimum lies in [lo, hi]: certain outside the band around R^2
+          const float R = r + rc, R2 = R * R, w = band * (2.0f * R + band);
+          if (hi[k] - R2 < -w) {
+            definite = true;
+          } else if (!(lo[k] - R2 > w)) {
             uncertain = true;
             n_inb = 2;
-          } else if (m < 0.0f) {
-            definite = true;
           }
         }
       }
@@ -1139,8 +1153,8 @@ struct EvalUnitVis {
       const int orig = cload(sc.vorig, p);
       const float rc = Pc.p0;
       const float reach = bk.G * n2_hi + rc + kCutoff + 1e-5f;
-      float d2[kBlockSpheres];
-      cloud_block_min(sc, (Pc.flags >> 4) & 15, c, n, g, all ? FLT_MAX : reach * reach, true, d2);
+      float d2[kBlockSpheres], unused[kBlockSpheres];
+      cloud_block_min<false>(sc, (Pc.flags >> 4) & 15, c, n, g, all ? FLT_MAX : reach * reach, true, d2, unused);
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         if (k < n) {

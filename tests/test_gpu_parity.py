@@ -434,6 +434,49 @@ def test_edges_cloud(band):
     assert np.array_equal(n_got, n_ref)
 
 
+def _sparse_cloud_workload(n_configs, seed=21):
+    """The 8-DOF mobile torso arm against 1,000 points spread uniformly over a
+    4 m cube around it: 64-point tiles span metres, where the FP32 tile-local
+    squared distance carries an error far above the 1e-5 m band."""
+    import dataclasses
+
+    rng = np.random.default_rng(seed)
+    base = synth.workload("cloud_shelves", scale=n_configs / 16384)
+    cloud = rng.uniform([-2.0, -2.0, -1.4], [2.0, 2.0, 2.6], size=(1000, 3))
+    return dataclasses.replace(base, cloud=cloud)
+
+
+def test_sparse_cloud_valid_edges_and_clearance():
+    """Validity, motion checks and clearances against a sparse cloud: the
+    validity decisions bracket the FP32 distance by the per-tile error bound
+    (|c'|^2 + rho^2 term), so the booleans stay exact; the clearance error is
+    the FP32 GEMM form's (stated bound 2e-6 (|c'|^2 + rho^2) / (2 d))."""
+    from plainmp_amd import kdtree as kd
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    wl = _sparse_cloud_workload(4096)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    sizes = kd.cloud_tile_sizes(len(wl.cloud))
+    ref = oc.is_valid_batch(wl.Q, 8).astype(bool)
+    got, n_re = pc.is_valid_batch_stats(wl.Q)
+    _report("sparse_cloud_valid", B=len(ref), tiles=sizes[0], valid_frac=ref.mean(), rechecked=n_re)
+    assert 0.05 < ref.mean() < 0.95
+    assert np.array_equal(got, ref)
+    E = 512
+    S1, S2 = wl.Q[:E], wl.Q[E: 2 * E]
+    w = [0.05] * S1.shape[1]
+    eref, n_ref = oc.check_motions(S1, S2, True, w, 0.0, 8)
+    mv = BoxMotionValidator(pc, w)
+    egot, n_got = mv.check_motions(S1, S2)
+    _report("sparse_cloud_edges", E=E, valid_frac=eref.mean(), rechecked=mv.last_recheck_count())
+    assert np.array_equal(egot, eref.astype(bool))
+    assert np.array_equal(n_got, n_ref)
+    Q = wl.Q[:64]
+    cerr = np.abs(pc.sphere_clearance_batch(Q, False) - oc.sphere_clearance_batch(Q, False, 8)).max()
+    _report("sparse_cloud_clearance", max_err=cerr)
+    assert cerr <= 5e-5
+
+
 @pytest.mark.parametrize("height", [0.0, 0.3])
 def test_valid_every_kind_with_ground(height):
     """Boxes (axis-aligned and rotated), cylinders (z-aligned and rotated),

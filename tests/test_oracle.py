@@ -335,6 +335,63 @@ def test_cloud_tiles_invariants(n):
         assert np.array_equal(sup[s, 3:6], t32[first:first + count, 3:6].max(0))
 
 
+@pytest.mark.parametrize("extent", [0.5, 2.0, 4.0])
+def test_cloud_tile_local_fp32_error_bound(extent):
+    """The FP32 tile-local squared distance |c'|^2 + (|p'|^2 - 2 c'.p') from
+    the tile tables, emulated with every product and sum rounded to single
+    precision (the matrix-core K = 4 order) and as an FMA chain (the edge
+    pass's order), stays within 8u (|c'| + rho)^2 of the exact |c - p|^2 and so
+    within half the kernels' band term 2e-6 (|c'|^2 + rho^2).  For tiles
+    spread over metres the error exceeds the 1e-5 m validity band
+    (1e-5 (2R + 1e-5) ~ 6.4e-7 m^2 at R = 0.032), which is why the decisions
+    carry the per-tile term."""
+    rng = np.random.default_rng(int(extent * 10))
+    P = rng.uniform(-extent / 2, extent / 2, (1000, 3)) + np.array([0.3, -0.2, 0.7])
+    d = _oracle.cloud_tiles(P)
+    t32 = np.frombuffer(d["tiles"], np.float32).reshape(-1, 12)
+    tb = np.frombuffer(d["tile_b"], np.float32).reshape(-1, 4, 4, 16)
+    p64 = np.frombuffer(d["points64"], np.float64).reshape(-1, 64, 4)
+    f32, u = np.float32, 2.0 ** -24
+    worst, worst_ratio = 0.0, 0.0
+    for t in range(len(t32)):
+        lb, ub, o = t32[t, 0:3], t32[t, 3:6], t32[t, 6:9]
+        n = int(np.frombuffer(t32[t].tobytes(), np.int32)[9])
+        B = tb[t].transpose(0, 2, 1).reshape(64, 4)[:n]  # (-2p', |p'|^2) per point, float32
+        pts = p64[t, :n, :3]
+        e = np.maximum(ub - o, o - lb).astype(f32)
+        rho2 = float(e[0] * e[0] + e[1] * e[1] + e[2] * e[2])
+        # queries: near the tile's points (the decisions that matter) and anywhere in the cloud
+        near = pts[rng.integers(0, n, 40)] + rng.normal(0, 0.05, (40, 3))
+        far = rng.uniform(P.min(0), P.max(0), (20, 3))
+        C = np.concatenate([near, far]).astype(f32)
+        cl = (C - o).astype(f32)  # c' rounded once, as in the kernels
+        off = (cl[:, 0] * cl[:, 0] + cl[:, 1] * cl[:, 1] + cl[:, 2] * cl[:, 2]).astype(f32)
+        # rounded K = 4 dot product in matrix-core order
+        m = (cl[:, None, 0] * B[None, :, 0]).astype(f32)
+        for k in (1, 2):
+            m = (m + (cl[:, None, k] * B[None, :, k]).astype(f32)).astype(f32)
+        m = (m + B[None, :, 3]).astype(f32)
+        d_mfma = (m + off[:, None]).astype(f32).astype(np.float64)
+        # FMA chain fma(x, c'x, fma(y, c'y, fma(z, c'z, w))): products exact in FP64, one rounding per step
+        acc = B[None, :, 3].astype(np.float64)
+        for k in (2, 1, 0):
+            acc = (B[None, :, k].astype(np.float64) * cl[:, None, k].astype(np.float64) + acc).astype(f32).astype(np.float64)
+        d_fma = (acc + off[:, None].astype(np.float64)).astype(f32).astype(np.float64)
+        exact = ((C.astype(np.float64)[:, None, :] - pts[None]) ** 2).sum(2)
+        a = np.sqrt(off.astype(np.float64))[:, None]
+        bound8 = 8 * u * (a + math.sqrt(rho2)) ** 2
+        band_term = 2e-6 * (off.astype(np.float64)[:, None] + rho2)
+        for dd in (d_mfma, d_fma):
+            err = np.abs(dd - exact)
+            assert np.all(err <= bound8)
+            assert np.all(err <= 0.5 * band_term)
+            worst = max(worst, err.max())
+            worst_ratio = max(worst_ratio, (err / band_term).max())
+    if extent >= 4.0:
+        assert worst > 1e-5 * (2 * 0.032 + 1e-5)  # beyond the plain band: the per-tile term is needed
+    assert worst_ratio < 0.5
+
+
 def test_is_valid_consistent_with_evaluate():
     # tests/test_constraints.py:47-53
     wl = synth.workload("grad_mixed32", scale=1 / 64)
