@@ -153,6 +153,7 @@ struct pmp_context {
   float band = 1e-5f;
   pmp_schedule sched{0, 0, 0};
   int n_cu = 256;
+  int max_lds = 65536;  // LDS bytes per workgroup on this device
   // robot
   bool has_robot = false;
   DevBuf nodes32, nodes64, sph32, sph64, meta, pairs, blocks, items, units, unit_all, ws_vjp;
@@ -499,6 +500,7 @@ pmp_status pmp_create(int32_t device, pmp_context** out) {
       hipDeviceProp_t prop;
       PMP_HIP(hipGetDeviceProperties(&prop, device));
       ctx->n_cu = prop.multiProcessorCount;
+      ctx->max_lds = (int)prop.sharedMemPerBlock;
       ctx->seq.upload(sequence_table(), nullptr);
       PMP_HIP(hipDeviceSynchronize());
     } catch (...) {
@@ -644,26 +646,26 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
             f = k;
             break;
           }
-        PMP_REQUIRE(f >= 0, "kinematic tree too branched: more than 8 live poses");
+        PMP_REQUIRE(f >= 0, "kinematic tree too branched: more than 32 live poses");
         used[f] = true;
         slot[i] = f;
       }
     }
     // q columns that move each node (reference rptable_), base columns move all
-    std::vector<uint32_t> mask(N, 0);
-    uint32_t base_bits = 0;
-    for (int c = n_q; c < n_cols; ++c) base_bits |= 1u << c;
-    uint32_t rev = 0;
-    if (r->base_type == PMP_BASE_PLANAR) rev |= 1u << (n_q + 2);
-    if (r->base_type == PMP_BASE_FLOATING) rev |= (1u << (n_q + 3)) | (1u << (n_q + 4)) | (1u << (n_q + 5));
+    std::vector<uint64_t> mask(N, 0);
+    uint64_t base_bits = 0;
+    for (int c = n_q; c < n_cols; ++c) base_bits |= 1ull << c;
+    uint64_t rev = 0;
+    if (r->base_type == PMP_BASE_PLANAR) rev |= 1ull << (n_q + 2);
+    if (r->base_type == PMP_BASE_FLOATING) rev |= (1ull << (n_q + 3)) | (1ull << (n_q + 4)) | (1ull << (n_q + 5));
     for (int i = 0; i < N; ++i) {
-      uint32_t m = base_bits;
+      uint64_t m = base_bits;
       for (int a = i; a > 0; a = r->node_parent[a]) {
         const int k = r->node_kind[a];
-        if (k == PMP_NODE_REVOLUTE || k == PMP_NODE_PRISMATIC) m |= 1u << r->node_q_col[a];
+        if (k == PMP_NODE_REVOLUTE || k == PMP_NODE_PRISMATIC) m |= 1ull << r->node_q_col[a];
       }
       mask[i] = m;
-      if (r->node_kind[i] == PMP_NODE_REVOLUTE) rev |= 1u << r->node_q_col[i];
+      if (r->node_kind[i] == PMP_NODE_REVOLUTE) rev |= 1ull << r->node_q_col[i];
     }
     // spheres sorted by node (stable: groups stay contiguous and in order)
     std::vector<int> order(S);
@@ -834,6 +836,14 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     a.base_type = d.base_type = r->base_type;
     a.n_ext_spheres = d.n_ext_spheres = n_ext;
     a.rev_mask = d.rev_mask = rev;
+    int n_slots = 0;
+    for (int i = 0; i < N; ++i) n_slots = std::max(n_slots, slot[i] + 1);
+    a.n_slots = d.n_slots = n_slots;
+    a.slot_off = d.slot_off = 0;
+    a.lds_limit = d.lds_limit = ctx->max_lds;
+    PMP_REQUIRE(slot_bytes(d, 64) <= (size_t)ctx->max_lds,
+                "kinematic tree too branched for this device: " + std::to_string(n_slots) +
+                    " live poses need more LDS than a workgroup has");
     for (int k = 0; k < 3; ++k) {
       a.base_t[k] = (float)r->base_pose[k];
       d.base_t[k] = r->base_pose[k];
@@ -1286,7 +1296,7 @@ void prepare_edges(pmp_context* ctx, const double* s1, const double* s2, int64_t
 // edge by index) is launched only when all of it is resident; otherwise the
 // resident grid pulls edges from the counter.
 int edge_grid(pmp_context* ctx, int block, int64_t E) {
-  const int occ = edges_resident_waves_per_cu(ctx->s32.n_cloud > 0, block);
+  const int occ = edges_resident_waves_per_cu(ctx->s32.n_cloud > 0, block, slot_bytes(ctx->k64, block));
   int wpc = ctx->sched.waves_per_cu > 0 ? ctx->sched.waves_per_cu : 16;
   if (occ > 0) wpc = std::min(wpc, occ);
   const int64_t by_edges = (E * 64 + block - 1) / block;
@@ -1303,7 +1313,7 @@ pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const doubl
     EdgeArgs ea{};
     prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, ctx->eb, ea);
     if (E == 0) return;
-    const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
+    const int block = slot_block(ctx->k64, ctx->sched.block > 0 ? ctx->sched.block : 256);
     if (!ctx->edge_ev0) {
       PMP_HIP(hipEventCreate(&ctx->edge_ev0));
       PMP_HIP(hipEventCreate(&ctx->edge_ev1));
@@ -1328,7 +1338,7 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
         if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
         cs = ctx->capture_stream;
       }
-      const int block = ctx->sched.block > 0 ? ctx->sched.block : 256;
+      const int block = slot_block(ctx->k64, ctx->sched.block > 0 ? ctx->sched.block : 256);
       g->stream = cs;
       PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       hipError_t le = launch_edges(ctx->k32, ctx->k64, ctx->s32, ctx->s64, ea, ctx->band, edge_grid(ctx, block, E),

@@ -4,9 +4,8 @@
 // records, sphere tables and primitives are read with uniform indices
 // (scalar loads into SGPRs), while the pose chain and the per-configuration
 // results stay in VGPRs.  Poses that later nodes (or the self-collision pass)
-// need are kept in a small register file of slots whose indices the host
-// assigns; every slot index is uniform, so the accesses lower to register
-// moves rather than scratch memory.
+// need are kept in slots whose indices the host assigns (wave-uniform); the
+// slots live in LDS (SlotFile).
 //
 // The chain follows KinematicModel::set_joint_angles_impl and
 // build_cache_until(_inner) (kinematics/algorithm.cpp:109-210):
@@ -73,33 +72,39 @@ struct is_ranged<V, decltype((void)V::kRanged)> {
   static constexpr bool value = V::kRanged;
 };
 
-// Register file of kept poses.  The slot index is wave-uniform, so the
-// switch compiles to scalar branches around plain register moves.
+// Kept poses (ancestors of later branches, links of self pairs) live in LDS:
+// the launch reserves slot_bytes(ka, block) bytes of dynamic shared memory
+// from byte offset ka.slot_off, laid out [slot][component][lane] so that the
+// 64 lanes of a wave touch 64 consecutive words (no bank conflicts) and
+// nothing goes to scratch.  Slot indices are wave-uniform; each lane reads
+// only what it wrote, so no barrier is needed.
 template <class T>
 struct SlotFile {
-  Pose<T> s0, s1, s2, s3, s4, s5, s6, s7;  // named members: no private array, no scratch
-  __device__ __forceinline__ Pose<T> get(int i) const {
-    if (i == 0) return s0;
-    if (i == 1) return s1;
-    if (i == 2) return s2;
-    if (i == 3) return s3;
-    if (i == 4) return s4;
-    if (i == 5) return s5;
-    if (i == 6) return s6;
-    return s7;
+  T* base;
+  int stride;
+  __device__ __forceinline__ explicit SlotFile(int32_t off) {
+    HIP_DYNAMIC_SHARED(unsigned char, pmp_dyn_lds)
+    base = reinterpret_cast<T*>(pmp_dyn_lds + off) + threadIdx.x;
+    stride = (int)blockDim.x;
   }
-  __device__ __forceinline__ void set(int i, const Pose<T>& p) {
-    if (i == 0) s0 = p;
-    else if (i == 1) s1 = p;
-    else if (i == 2) s2 = p;
-    else if (i == 3) s3 = p;
-    else if (i == 4) s4 = p;
-    else if (i == 5) s5 = p;
-    else if (i == 6) s6 = p;
-    else s7 = p;
+  __device__ __forceinline__ Pose<T> get(int i) const {
+    const T* p = base + i * 7 * stride;
+    Pose<T> r;
+    r.q = {p[0], p[stride], p[2 * stride], p[3 * stride]};
+    r.t = {p[4 * stride], p[5 * stride], p[6 * stride]};
+    return r;
+  }
+  __device__ __forceinline__ void set(int i, const Pose<T>& P) {
+    T* p = base + i * 7 * stride;
+    p[0] = P.q.x;
+    p[stride] = P.q.y;
+    p[2 * stride] = P.q.z;
+    p[3 * stride] = P.q.w;
+    p[4 * stride] = P.t.x;
+    p[5 * stride] = P.t.y;
+    p[6 * stride] = P.t.z;
   }
 };
-static_assert(kMaxSlots == 8, "SlotFile holds eight poses");
 
 // Configuration sources.  FK reads each column once, in FP64, and converts.
 struct BatchQ {  // q[col * ldq + b]
@@ -154,7 +159,7 @@ __device__ __forceinline__ Pose<T> base_pose(const KinArgs<T>& ka, const QS& qs)
 // `taylor` selects the collision check's sin/cos (a runtime, wave-uniform flag).
 template <class T, class QS, class Vis>
 __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool taylor, Vis& vis) {
-  SlotFile<T> slots;
+  SlotFile<T> slots(ka.slot_off);
   const Pose<T> base = base_pose(ka, qs);
   vis.begin(base, qs);
   Pose<T> prev = base;

@@ -937,8 +937,9 @@ __global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinAr
       __syncthreads();
     } else {
       const int64_t n_chunks = (n_wp + 63) / 64;
-      const int per_round = n_chunks < kRecheckWaves ? (int)n_chunks : kRecheckWaves;
-      const int n_split = kRecheckWaves / per_round;
+      const int nw = (int)(blockDim.x >> 6);  // kRecheckWaves, fewer when the kept poses need the LDS
+      const int per_round = n_chunks < nw ? (int)n_chunks : nw;
+      const int n_split = nw / per_round;
       for (int64_t c0 = 0; c0 < n_chunks; c0 += per_round) {
         const int64_t chunk = c0 + w / n_split;
         if (w / n_split < per_round && chunk < n_chunks) {
@@ -1023,13 +1024,13 @@ struct Frames {
       set(n + 5, {0.f, 0.f, 1.f}, base.t);
     }
   }
-  __device__ __forceinline__ V3<float> col(int j, uint32_t mask, uint32_t rev, V3<float> c) const {
+  __device__ __forceinline__ V3<float> col(int j, uint64_t mask, uint64_t rev, V3<float> c) const {
     V3<float> r{0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < MAXC; ++k)
-      if (k == j && ((mask >> k) & 1u)) {
+      if (k == j && ((mask >> k) & 1ull)) {
         const V3<float> a{ax[k], ay[k], az[k]};
-        r = ((rev >> k) & 1u) ? cross(a, V3<float>{c.x - ox[k], c.y - oy[k], c.z - oz[k]}) : a;
+        r = ((rev >> k) & 1ull) ? cross(a, V3<float>{c.x - ox[k], c.y - oy[k], c.z - oz[k]}) : a;
       }
     return r;
   }
@@ -1302,7 +1303,7 @@ __device__ __forceinline__ void eval_finish(const KinArgs<float>& ka, const Scen
   if (NP > 0) {
     o.vals[b] = best;
     V3<float> g{0.f, 0.f, 0.f};
-    uint32_t mask = 0;
+    uint64_t mask = 0;
     if (bidx >= 0) {
       const DevPrim<float> P = prim_at(sc, bidx % kMaxPrims);
       if (P.type == PMP_PRIM_CLOUD) {
@@ -1324,7 +1325,7 @@ __device__ __forceinline__ void eval_finish(const KinArgs<float>& ka, const Scen
   if (ka.n_pairs > 0) {
     o.vals[(int64_t)row * o.ld + b] = sbest;
     V3<float> n{0.f, 0.f, 0.f};
-    uint32_t m1 = 0, m2 = 0;
+    uint64_t m1 = 0, m2 = 0;
     if (sidx >= 0) {
       const V3<float> d = sub(rv.c1, rv.c2);
       n = scale(1.0f / sqrtf(dot(d, d)), d);
@@ -1421,7 +1422,7 @@ struct PairVis {
   __device__ void sphere(int s, V3<float> c, float r) {
     if (s < it.begin || s >= it.end) return;
     const DevSphereMeta m = cload(ka.meta, s);
-    const uint32_t mask = cload(ka.nodes, m.node).col_mask;
+    const uint64_t mask = cload(ka.nodes, m.node).col_mask;
     const int D = ka.n_cols, NP = sc.n_prims;
     V3<float> J[MAXC];
 #pragma unroll
@@ -1475,7 +1476,7 @@ struct LinkJacVis {
   int target;
   V3<float> off;
   V3<float> p{0.f, 0.f, 0.f};
-  uint32_t mask = 0;
+  uint64_t mask = 0;
   bool done = false;
   __device__ bool alive() const { return !done; }
   template <class QS>
@@ -1540,7 +1541,7 @@ struct VjpVis {
   __device__ void sphere(int s, V3<float> c, float) {
     if (s < it.begin || s >= it.end) return;
     const DevSphereMeta m = cload(ka.meta, s);
-    const uint32_t mask = cload(ka.nodes, m.node).col_mask;
+    const uint64_t mask = cload(ka.nodes, m.node).col_mask;
     const int NP = sc.n_prims;
     V3<float> g{0.f, 0.f, 0.f};
     for (int p = 0; p < NP; ++p) {
@@ -1662,18 +1663,23 @@ static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t -
 
 hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int taylor, float* out,
                           hipStream_t st) {
-  hipLaunchKernelGGL(centers_kernel<float>, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, q, ldq, B, taylor, out);
+  const int bk = slot_block(ka, 256);
+  hipLaunchKernelGGL(centers_kernel<float>, dim3(blocks_for(B, bk)), dim3(bk), slot_bytes(ka, bk), st, ka, q, ldq, B,
+                     taylor, out);
   return hipGetLastError();
 }
 hipError_t launch_poses(const KinArgs<double>& ka, const double* q, int64_t ldq, int64_t B, int taylor, double* out,
                         hipStream_t st) {
-  hipLaunchKernelGGL(poses_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, q, ldq, B, taylor, out);
+  const int bk = slot_block(ka, 256);
+  hipLaunchKernelGGL(poses_kernel, dim3(blocks_for(B, bk)), dim3(bk), slot_bytes(ka, bk), st, ka, q, ldq, B, taylor,
+                     out);
   return hipGetLastError();
 }
 hipError_t launch_clearance(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                             int64_t B, int taylor, float* out, float* cen, hipStream_t st) {
-  hipLaunchKernelGGL(clearance_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, sc, q, ldq, B, taylor, out,
-                     cen);
+  const int bk = slot_block(ka, 256);
+  hipLaunchKernelGGL(clearance_kernel, dim3(blocks_for(B, bk)), dim3(bk), slot_bytes(ka, bk), st, ka, sc, q, ldq, B,
+                     taylor, out, cen);
   return hipGetLastError();
 }
 hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
@@ -1682,13 +1688,14 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                         const CloudRange* clouds, int n_clouds, int32_t* flags, int32_t* unit_flags,
                         hipStream_t st) {
   if (block <= 0) block = 256;
+  block = slot_block(k32, block);
   if (unit_flags && n_clouds == 0 && k32.n_units > 0) {
-    hipLaunchKernelGGL(valid_units_kernel, dim3(blocks_for(B, 64), (unsigned)k32.n_units), dim3(64), 0, st, k32, s32,
-                       q, ldq, B, band, unit_flags);
+    hipLaunchKernelGGL(valid_units_kernel, dim3(blocks_for(B, 64), (unsigned)k32.n_units), dim3(64),
+                       slot_bytes(k32, 64), st, k32, s32, q, ldq, B, band, unit_flags);
     hipLaunchKernelGGL(valid_units_finish_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, unit_flags, B, valid);
   } else {
-    hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), 0, st, k32, s32, q, ldq, B, band,
-                       valid, cw);
+    hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), slot_bytes(k32, block), st, k32, s32, q,
+                       ldq, B, band, valid, cw);
   }
   // sphere-vs-cloud queries on the matrix cores, one pass per cloud
   const int64_t Q = (int64_t)k32.n_ext_spheres * B;
@@ -1697,8 +1704,8 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                                       B, Q, band, valid, flags, st);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(valid_recheck_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, k64, s64, q, ldq, B, valid,
-                     n_recheck);
+  hipLaunchKernelGGL(valid_recheck_kernel, dim3(blocks_for(B, 64)), dim3(64), slot_bytes(k64, 64), st, k64, s64, q,
+                     ldq, B, valid, n_recheck);
   return hipGetLastError();
 }
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
@@ -1708,17 +1715,20 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
     hipError_t e = hipEventRecord(ev_before, st);
     if (e != hipSuccess) return e;
   }
-  // scenes with a point cloud take the variant that tests clouds in the walk
+  // scenes with a point cloud take the variant that tests clouds in the walk;
+  // the FP32 walk and the in-kernel FP64 re-decision share the slot area
+  const size_t lds = slot_bytes(k64, block);
   if (s32.n_cloud > 0)
-    hipLaunchKernelGGL(edges_kernel<true>, dim3(grid), dim3(block), 0, st, k32, s32, k64, s64, ea, band);
+    hipLaunchKernelGGL(edges_kernel<true>, dim3(grid), dim3(block), lds, st, k32, s32, k64, s64, ea, band);
   else
-    hipLaunchKernelGGL(edges_kernel<false>, dim3(grid), dim3(block), 0, st, k32, s32, k64, s64, ea, band);
+    hipLaunchKernelGGL(edges_kernel<false>, dim3(grid), dim3(block), lds, st, k32, s32, k64, s64, ea, band);
   if (ev_after) {
     hipError_t e = hipEventRecord(ev_after, st);
     if (e != hipSuccess) return e;
   }
   const unsigned rgrid = (unsigned)(ea.E < kRecheckGrid ? ea.E : kRecheckGrid);
-  hipLaunchKernelGGL(edges_recheck_kernel, dim3(rgrid), dim3(64 * kRecheckWaves), 0, st, k64, s64, ea);
+  const int rblock = slot_block(k64, 64 * kRecheckWaves, sizeof(int));
+  hipLaunchKernelGGL(edges_recheck_kernel, dim3(rgrid), dim3(rblock), slot_bytes(k64, rblock), st, k64, s64, ea);
   return hipGetLastError();
 }
 
@@ -1728,8 +1738,10 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
       hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__);                                             \
     else if ((ncols) <= 16)                                                                   \
       hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__);                                            \
-    else                                                                                      \
+    else if ((ncols) <= 32)                                                                   \
       hipLaunchKernelGGL(KERNEL<32>, __VA_ARGS__);                                            \
+    else                                                                                      \
+      hipLaunchKernelGGL(KERNEL<64>, __VA_ARGS__);                                            \
   } while (0)
 
 #define PMP_DISPATCH_MAXC_CLOUD(ncols, KERNEL, CLOUD, ...)                                   \
@@ -1738,8 +1750,10 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
       hipLaunchKernelGGL((KERNEL<8, CLOUD>), __VA_ARGS__);                                    \
     else if ((ncols) <= 16)                                                                   \
       hipLaunchKernelGGL((KERNEL<16, CLOUD>), __VA_ARGS__);                                   \
-    else                                                                                      \
+    else if ((ncols) <= 32)                                                                   \
       hipLaunchKernelGGL((KERNEL<32, CLOUD>), __VA_ARGS__);                                   \
+    else                                                                                      \
+      hipLaunchKernelGGL((KERNEL<64, CLOUD>), __VA_ARGS__);                                   \
   } while (0)
 
 __global__ void fill_f32_kernel(float* p, float v, int64_t n) {
@@ -1820,17 +1834,19 @@ __global__ void __launch_bounds__(256) sqp_ineq_kernel(SqpArgs a) {
 
 hipError_t launch_link_jacobian(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int link,
                                 const float (&off)[3], float* jac, hipStream_t st) {
-  if (B > 0)
-    PMP_DISPATCH_MAXC(ka.n_cols, link_jac_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, ka, q, ldq, B, link,
-                      off[0], off[1], off[2], jac);
+  if (B > 0) {
+    const int bk = slot_block(ka, 256);
+    PMP_DISPATCH_MAXC(ka.n_cols, link_jac_kernel, dim3(blocks_for(B, bk)), dim3(bk), slot_bytes(ka, bk), st, ka, q,
+                      ldq, B, link, off[0], off[1], off[2], jac);
+  }
   return hipGetLastError();
 }
 
-int edges_resident_waves_per_cu(bool cloud, int block) {
+int edges_resident_waves_per_cu(bool cloud, int block, size_t lds) {
   int n = 0;
   const hipError_t err =
-      cloud ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, edges_kernel<true>, block, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, edges_kernel<false>, block, 0);
+      cloud ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, edges_kernel<true>, block, lds)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, edges_kernel<false>, block, lds);
   return err == hipSuccess ? n * (block / 64) : 0;
 }
 
@@ -1856,7 +1872,10 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
   // launch fills the GPU; from kWholeUnitChunks chunks on (or beyond
   // kEvalGroupMaxUnits units), one unit walks the whole configuration.
   const int64_t whole = e.whole_chunks > 0 ? e.whole_chunks : kWholeUnitChunks;
-  const bool split = chunks < whole && ka.n_units > 1 && ka.n_units <= kEvalGroupMaxUnits;
+  const int waves = ka.n_units < kEvalGroupWaves ? ka.n_units : kEvalGroupWaves;
+  const size_t own = sizeof(int32_t) * 4 * 64 * (size_t)ka.n_units;  // (value, winner) pairs per unit and lane
+  const bool split = chunks < whole && ka.n_units > 1 && ka.n_units <= kEvalGroupMaxUnits &&
+                     own + slot_bytes(ka, 64 * waves) <= (size_t)ka.lds_limit;
   o.units = split ? ka.units : ka.unit_all;
   o.n_units = split ? ka.n_units : 1;
   o.offsets = e.offsets;
@@ -1869,31 +1888,36 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
                        (int64_t)e.n_traj);
   if (B == 0) return hipGetLastError();
   if (split) {
-    const int waves = o.n_units < kEvalGroupWaves ? o.n_units : kEvalGroupWaves;
-    const size_t lds = sizeof(int32_t) * 4 * 64 * (size_t)o.n_units;
-    PMP_DISPATCH_MAXC(ka.n_cols, eval_group_kernel, dim3((unsigned)chunks), dim3(64 * waves), lds, st, ka, sc, q, ldq,
-                      B, o);
+    KinArgs<float> kg = ka;  // slots after the unit partials
+    kg.slot_off = (int32_t)own;
+    PMP_DISPATCH_MAXC(ka.n_cols, eval_group_kernel, dim3((unsigned)chunks), dim3(64 * waves),
+                      own + slot_bytes(ka, 64 * waves), st, kg, sc, q, ldq, B, o);
   } else {
-    PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks), dim3(64), 0, st, ka, sc, q, ldq, B, o);
+    PMP_DISPATCH_MAXC(ka.n_cols, eval_units_kernel, dim3((unsigned)chunks), dim3(64), slot_bytes(ka, 64), st, ka, sc,
+                      q, ldq, B, o);
   }
   return hipGetLastError();
 }
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                         int64_t B, float* dist, float* grad, hipStream_t st) {
   if (ka.n_items == 0) return hipSuccess;
+  const int bk = slot_block(ka, 256);
+  const size_t lds = slot_bytes(ka, bk);
   if (sc.n_cloud > 0)
-    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, true, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256), 0,
-                            st, ka, sc, q, ldq, B, dist, grad);
+    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, true, dim3(blocks_for(B, bk), (unsigned)ka.n_items), dim3(bk),
+                            lds, st, ka, sc, q, ldq, B, dist, grad);
   else
-    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, false, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256),
-                            0, st, ka, sc, q, ldq, B, dist, grad);
+    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, false, dim3(blocks_for(B, bk), (unsigned)ka.n_items), dim3(bk),
+                            lds, st, ka, sc, q, ldq, B, dist, grad);
   return hipGetLastError();
 }
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
                       const float* gd, float* part, float* grad_q, hipStream_t st) {
-  if (ka.n_items > 0)
-    PMP_DISPATCH_MAXC(ka.n_cols, vjp_kernel, dim3(blocks_for(B, 256), (unsigned)ka.n_items), dim3(256), 0, st, ka,
-                      sc, q, ldq, B, gd, part);
+  if (ka.n_items > 0) {
+    const int bk = slot_block(ka, 256);
+    PMP_DISPATCH_MAXC(ka.n_cols, vjp_kernel, dim3(blocks_for(B, bk), (unsigned)ka.n_items), dim3(bk),
+                      slot_bytes(ka, bk), st, ka, sc, q, ldq, B, gd, part);
+  }
   hipLaunchKernelGGL(vjp_reduce_kernel, dim3(blocks_for((int64_t)ka.n_cols * B, 256)), dim3(256), 0, st, part,
                      ka.n_items, ka.n_cols, B, grad_q);
   return hipGetLastError();

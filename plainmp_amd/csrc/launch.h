@@ -58,7 +58,7 @@ struct SqpArgs {
 hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st);
 // Waves of the edge kernel variant (cloud or not) one CU holds at once with
 // workgroups of `block` threads (0 when the runtime cannot tell).
-int edges_resident_waves_per_cu(bool cloud, int block);
+int edges_resident_waves_per_cu(bool cloud, int block, size_t lds);
 hipError_t launch_link_jacobian(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int link,
                                 const float (&off)[3], float* jac, hipStream_t st);
 // p[0, n) = v by a kernel (safe under stream capture, unlike a memset node).

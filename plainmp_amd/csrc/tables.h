@@ -10,8 +10,8 @@
 
 namespace pmp {
 
-constexpr int kMaxSlots = 8;      // live ancestor poses kept in registers during FK
-constexpr int kMaxCols = 32;      // q columns (joints + base)
+constexpr int kMaxSlots = 32;     // live poses kept during FK (LDS; the device's LDS may allow fewer)
+constexpr int kMaxCols = 64;      // q columns (joints + base)
 constexpr int kMaxNodes = 256;
 constexpr int kMaxSpheres = 4096;
 constexpr int kMaxPrims = 512;
@@ -33,10 +33,10 @@ struct alignas(16) DevNode {
   int32_t sphere_begin;  // node-sorted sphere range
   int32_t sphere_end;
   int32_t identity;      // local rotation skipped on compose
-  uint32_t col_mask;     // q columns that move this node (reference rptable_ row)
   int32_t block_begin;   // sphere blocks of this node (primitive-checked spheres only)
   int32_t block_end;
-  int32_t ipad[2];
+  int32_t ipad;
+  uint64_t col_mask;     // q columns that move this node (reference rptable_ row)
   T quat[4];
   T trans[3];
   T axis[3];
@@ -145,10 +145,29 @@ struct KinArgs {
   const DevEvalUnit* unit_all;  // one unit: every sphere and self pair
   int32_t n_nodes, n_spheres, n_pairs, n_cols;
   int32_t n_q_cols, base_type, n_ext_spheres;
-  uint32_t rev_mask;  // q columns that rotate (revolute joints, base yaw/roll/pitch)
+  uint64_t rev_mask;  // q columns that rotate (revolute joints, base yaw/roll/pitch)
+  int32_t n_slots;    // kept-pose slots the FK walk uses
+  int32_t slot_off;   // byte offset of the kept-pose slots in the launch's dynamic LDS
+  int32_t lds_limit;  // LDS bytes one workgroup may use on this device (launch sizing)
   T base_q[4];
   T base_t[3];
 };
+
+// Dynamic LDS a launch of `block` lanes reserves for the kept poses of FK in
+// precision T (nothing for a robot that keeps none).
+template <class T>
+inline size_t slot_bytes(const KinArgs<T>& ka, int block) {
+  return (size_t)ka.n_slots * 7 * sizeof(T) * (size_t)block;
+}
+// The largest workgroup (a multiple of 64 lanes, at most `want`) whose
+// kept-pose slots in precision T fit the LDS left beside `own` bytes; 64 at
+// least (the robot upload checks that 64 FP64 lanes fit).
+template <class T>
+inline int slot_block(const KinArgs<T>& ka, int want, size_t own = 0) {
+  int b = want;
+  while (b > 64 && own + slot_bytes(ka, b) > (size_t)ka.lds_limit) b -= 64;
+  return b;
+}
 
 template <class T>
 struct SceneArgs {

@@ -167,6 +167,107 @@ def dual_arm(seed: int = 0) -> Robot:
     return Robot(urdf, joints, specs, pairs, np.full(14, -2.9), np.full(14, 2.9))
 
 
+def fetch_like(seed: int = 0) -> Robot:
+    """Mobile-manipulator topology of plainmp's fetch configuration
+    (conf/fetch.yaml): a prismatic torso lift carrying a pan-tilt head and a
+    7R arm with x / y joint axes, a fixed gripper link and two prismatic
+    fingers (not controlled, at 0).  Spheres on the torso and head links are
+    only-self; the 14 self pairs of that configuration join the torso and head
+    to the arm links and fingers (9 distinct links, each keeping its pose).
+    Control joints: the torso lift and the 7 arm joints."""
+    rng = np.random.default_rng(7000 + seed)
+    j = lambda lo, hi: float(rng.uniform(lo, hi))  # noqa: E731
+    names = ["base_link", "torso_lift_link", "head_pan_link", "head_tilt_link", "shoulder_pan_link",
+             "shoulder_lift_link", "upperarm_roll_link", "elbow_flex_link", "forearm_roll_link", "wrist_flex_link",
+             "wrist_roll_link", "gripper_link", "r_gripper_finger_link", "l_gripper_finger_link"]
+    urdf = '<?xml version="1.0"?>\n<robot name="fetch_like">\n' + "".join(_link(n) for n in names)
+    arm = [  # (joint, parent, child, xyz, axis)
+        ("shoulder_pan_joint", "torso_lift_link", "shoulder_pan_link", (j(0.1, 0.14), 0.0, j(0.32, 0.38)), (0, 0, 1)),
+        ("shoulder_lift_joint", "shoulder_pan_link", "shoulder_lift_link", (j(0.1, 0.13), 0.0, j(0.05, 0.07)), (0, 1, 0)),
+        ("upperarm_roll_joint", "shoulder_lift_link", "upperarm_roll_link", (j(0.2, 0.24), 0.0, 0.0), (1, 0, 0)),
+        ("elbow_flex_joint", "upperarm_roll_link", "elbow_flex_link", (j(0.12, 0.15), 0.0, 0.0), (0, 1, 0)),
+        ("forearm_roll_joint", "elbow_flex_link", "forearm_roll_link", (j(0.18, 0.21), 0.0, 0.0), (1, 0, 0)),
+        ("wrist_flex_joint", "forearm_roll_link", "wrist_flex_link", (j(0.11, 0.14), 0.0, 0.0), (0, 1, 0)),
+        ("wrist_roll_joint", "wrist_flex_link", "wrist_roll_link", (j(0.12, 0.15), 0.0, 0.0), (1, 0, 0)),
+    ]
+    urdf += _joint("torso_lift_joint", "prismatic", "base_link", "torso_lift_link", (j(-0.09, -0.08), 0.0, j(0.36, 0.39)),
+                   (0, 0, 0), axis=(0, 0, 1), lower=0.0, upper=0.386)
+    urdf += _joint("head_pan_joint", "revolute", "torso_lift_link", "head_pan_link", (j(0.04, 0.06), 0.0, j(0.58, 0.62)),
+                   (0, 0, 0), axis=(0, 0, 1), lower=-1.57, upper=1.57)
+    urdf += _joint("head_tilt_joint", "revolute", "head_pan_link", "head_tilt_link", (j(0.13, 0.15), 0.0, j(0.05, 0.06)),
+                   (0, 0, 0), axis=(0, 1, 0), lower=-0.76, upper=1.45)
+    for jn, par, ch, xyz, ax in arm:
+        urdf += _joint(jn, "revolute", par, ch, xyz, (0, 0, 0), axis=ax, lower=-2.0, upper=2.0)
+    urdf += _joint("gripper_axis", "fixed", "wrist_roll_link", "gripper_link", (j(0.15, 0.18), 0.0, 0.0), (0, 0, 0))
+    urdf += _joint("r_gripper_finger_joint", "prismatic", "gripper_link", "r_gripper_finger_link", (0.0, 0.0154, 0.0),
+                   (0, 0, 0), axis=(0, 1, 0), lower=0.0, upper=0.05)
+    urdf += _joint("l_gripper_finger_joint", "prismatic", "gripper_link", "l_gripper_finger_link", (0.0, -0.0154, 0.0),
+                   (0, 0, 0), axis=(0, -1, 0), lower=0.0, upper=0.05)
+    urdf += "</robot>\n"
+    seg = {ch: np.array(xyz, float) for _, _, ch, xyz, _ in arm[1:]}
+    seg["shoulder_pan_link"] = np.array(arm[1][3], float)
+    seg["wrist_roll_link"] = np.array([0.15, 0.0, 0.0])
+    specs = [
+        ("torso_lift_link", np.array([0.0, 0.0, 0.0]) + rng.normal(0, 0.03, (6, 3)) * [1, 1, 4], rng.uniform(0.08, 0.14, 6), True),
+        ("head_pan_link", np.array([0.1, 0.0, 0.1]) + rng.normal(0, 0.04, (4, 3)), rng.uniform(0.06, 0.12, 4), True),
+    ]
+    for ln in ["shoulder_pan_link", "shoulder_lift_link", "upperarm_roll_link", "elbow_flex_link", "forearm_roll_link",
+               "wrist_flex_link", "wrist_roll_link"]:
+        link, pos, r, only = _spheres_on(rng, ln, seg.get(ln, np.array([0.12, 0.0, 0.0])), 5, 0.04, 0.07)
+        specs.append((link, pos, r, only))
+    specs.append(("gripper_link", np.array([[0.0, 0.0, 0.0], [0.05, 0.0, 0.0]]) + rng.normal(0, 0.005, (2, 3)),
+                  rng.uniform(0.03, 0.05, 2), False))
+    for ln in ("r_gripper_finger_link", "l_gripper_finger_link"):
+        specs.append((ln, np.array([[0.02, 0.0, 0.0], [0.05, 0.0, 0.0]]), np.array([0.015, 0.012]), False))
+    arm_links = ["upperarm_roll_link", "elbow_flex_link", "forearm_roll_link", "wrist_flex_link", "gripper_link",
+                 "l_gripper_finger_link", "r_gripper_finger_link"]
+    pairs = [(a, b) for a in ("head_pan_link", "torso_lift_link") for b in arm_links]
+    joints = ["torso_lift_joint"] + [a[0] for a in arm]
+    lower = np.concatenate([[0.0], np.full(7, -2.0)])
+    upper = np.concatenate([[0.386], np.full(7, 2.0)])
+    return Robot(urdf, joints, specs, pairs, lower, upper)
+
+
+def humanoid31(seed: int = 0) -> Robot:
+    """A 31-joint humanoid tree in the shape of plainmp's jaxon configuration
+    (conf/jaxon.yaml): a 3-joint waist, a 2-joint neck, two 7R arms off the
+    chest and two 6R legs off the pelvis, meant for a floating base (31 + 6 =
+    37 q columns).  Two to three spheres per link (91 in all); self pairs
+    between the hands and between each hand and the opposite knee."""
+    rng = np.random.default_rng(8000 + seed)
+    links = ["BODY"]
+    urdf_j = ""
+    joints, seg = [], {}
+    axes = {"x": (1, 0, 0), "y": (0, 1, 0), "z": (0, 0, 1)}
+
+    def chain(prefix, parent, mount, pattern, step):
+        nonlocal urdf_j
+        prev = parent
+        for i, ax in enumerate(pattern):
+            ln, jn = f"{prefix}_LINK{i}", f"{prefix}_JOINT{i}"
+            xyz = mount if i == 0 else tuple(float(v) for v in step * rng.uniform(0.7, 1.3))
+            urdf_j += _joint(jn, "revolute", prev, ln, xyz, (0, 0, 0), axis=axes[ax], lower=-1.5, upper=1.5)
+            links.append(ln)
+            joints.append(jn)
+            seg[ln] = step
+            prev = ln
+        return prev
+
+    chain("CHEST", "BODY", (0.0, 0.0, 0.1), "zyx", np.array([0.0, 0.0, 0.12]))
+    chain("HEAD", "CHEST_LINK2", (0.0, 0.0, 0.3), "zy", np.array([0.0, 0.0, 0.08]))
+    chain("LARM", "CHEST_LINK2", (0.0, 0.22, 0.25), "yxzyzyx", np.array([0.0, 0.0, -0.13]))
+    chain("RARM", "CHEST_LINK2", (0.0, -0.22, 0.25), "yxzyzyx", np.array([0.0, 0.0, -0.13]))
+    chain("LLEG", "BODY", (0.0, 0.1, -0.05), "zxyyyx", np.array([0.0, 0.0, -0.16]))
+    chain("RLEG", "BODY", (0.0, -0.1, -0.05), "zxyyyx", np.array([0.0, 0.0, -0.16]))
+    urdf = '<?xml version="1.0"?>\n<robot name="humanoid31">\n' + "".join(_link(n) for n in links) + urdf_j + "</robot>\n"
+    specs = [("BODY", rng.normal(0, 0.05, (3, 3)), rng.uniform(0.08, 0.12, 3), False)]
+    for ln in links[1:]:
+        k = 3 if ("ARM" in ln or "LEG" in ln) else 2
+        specs.append(_spheres_on(rng, ln, seg[ln], k, 0.03, 0.07))
+    pairs = [("LARM_LINK6", "RARM_LINK6"), ("LARM_LINK6", "RLEG_LINK3"), ("RARM_LINK6", "LLEG_LINK3")]
+    return Robot(urdf, joints, specs, pairs, np.full(31, -1.5), np.full(31, 1.5))
+
+
 def self_pair_count(robot: Robot) -> int:
     sizes = {s[0]: len(s[2]) for s in robot.sphere_specs}
     return int(sum(sizes[a] * sizes[b] for a, b in robot.self_pairs))

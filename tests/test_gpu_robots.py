@@ -1,0 +1,181 @@
+"""Robots beyond the benchmark arms, HIP path vs the FP64 oracle (GPU):
+
+* a Fetch-like mobile manipulator (plainmp's fetch configuration: torso and
+  head spheres only-self, 14 self pairs over 9 links, self-body primitives
+  merged into the scene) -- ten kept poses during FK;
+* a 31-joint humanoid on a floating base (37 q columns, the 64-column
+  kernels);
+* the 14-DOF dual arm on a floating base (20 columns, the 32-column kernels);
+* edges long enough (n_test > 128) for checkMotion's linear visiting branch.
+
+Bars as in test_gpu_parity: booleans exact, values within 5e-6 m, rows within
+1e-4 relative (2e-4 where the reference takes base columns by forward
+differences) wherever the argmin pair is unambiguous.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import _oracle
+from plainmp_amd import synth
+from support import oracle_cst, product_cst
+
+pytestmark = pytest.mark.gpu
+
+
+def _report(name, **kv):
+    print(f"[{name}] " + " ".join(f"{k}={v}" for k, v in kv.items()), flush=True)
+
+
+def _second_gap(d):
+    s = np.sort(d.reshape(len(d), -1), axis=1)
+    return s[:, 1] - s[:, 0]
+
+
+def _self_distances(robot, oc, Q):
+    """|c1 - c2| - (r1 + r2) over every listed sphere pair (oracle centres, exact FK)."""
+    sizes = [len(s[2]) for s in robot.sphere_specs]
+    start = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+    first = {}
+    for gi, s in enumerate(robot.sphere_specs):
+        first.setdefault(s[0], gi)
+    I, J = [], []
+    for a, b in robot.self_pairs:
+        ia = np.arange(start[first[a]], start[first[a] + 1])
+        ib = np.arange(start[first[b]], start[first[b] + 1])
+        I.append(np.repeat(ia, len(ib)))
+        J.append(np.tile(ib, len(ia)))
+    I, J = np.concatenate(I), np.concatenate(J)
+    C = oc.sphere_centers_batch(Q, True)
+    _, r = oc.get_all_spheres()
+    return np.sqrt(((C[:, I] - C[:, J]) ** 2).sum(-1)) - (r[I] + r[J])
+
+
+def _check_evaluate(wl, oc, pc, Q, rtol=1e-4):
+    vref, jref = oc.evaluate_batch(Q, 8)
+    vgot, jgot = pc.evaluate_batch(Q)
+    assert vgot.shape == vref.shape and jgot.shape == jref.shape
+    verr = np.abs(vgot - vref).max()
+    assert verr <= 5e-6
+    row = 0
+    checked = 0
+    if len(wl.prims) or oc.n_prims():
+        clear = (_second_gap(oc.pair_clearance_batch(Q, True, 8)) > 1e-5) & (vref[:, 0] < 0.1)
+        scale = np.maximum(np.abs(jref[clear, 0]), 1.0)
+        assert (np.abs(jgot[clear, 0] - jref[clear, 0]) <= rtol * scale).all()
+        checked += int(clear.sum())
+        row = 1
+    if wl.robot.self_pairs:
+        sd = _self_distances(wl.robot, oc, Q)
+        clear = (_second_gap(sd) > 1e-5) & (vref[:, row] < 0.1)
+        scale = np.maximum(np.abs(jref[clear, row]), 1.0)
+        assert (np.abs(jgot[clear, row] - jref[clear, row]) <= rtol * scale).all()
+        checked += int(clear.sum())
+    return verr, checked
+
+
+FETCH_BODY = [  # self-body primitives of plainmp's fetch configuration (base and torso columns)
+    {"type": "cylinder", "radius": 0.29, "height": 0.32, "position": [0.005, 0.0, 0.2], "rotation": np.eye(3)},
+    {"type": "box", "width": [0.16, 0.16, 1.0], "position": [-0.12, 0.0, 0.5], "rotation": np.eye(3)},
+]
+
+
+def test_fetch_like_valid_evaluate_edges():
+    robot = synth.fetch_like(0)
+    Q = synth.uniform_configs(robot, 2048, seed=31)
+    wl = synth.Workload("fetch_like", robot, synth.mixed_primitives(16, seed=9), Q=Q)
+    oc, pc = oracle_cst(wl, fixed_prims=FETCH_BODY), product_cst(wl, fixed_prims=FETCH_BODY)
+    ref = oc.is_valid_batch(Q, 8).astype(bool)
+    got = pc.is_valid_batch(Q)
+    _report("fetch_valid", B=len(Q), valid_frac=ref.mean(), self_pairs=synth.self_pair_count(robot))
+    assert 0.05 < ref.mean() < 0.95
+    assert np.array_equal(got, ref)
+    verr, checked = _check_evaluate(wl, oc, pc, Q[:512])
+    _report("fetch_evaluate", val_err=verr, rows_checked=checked)
+    assert checked > 100
+    E = 256
+    w = [0.05] * robot.dof
+    eref, n_ref = oc.check_motions(Q[:E], Q[E:2 * E], True, w, 0.0, 8)
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    egot, n_got = BoxMotionValidator(pc, w).check_motions(Q[:E], Q[E:2 * E])
+    assert np.array_equal(egot, eref.astype(bool))
+    assert np.array_equal(n_got, n_ref)
+
+
+def test_humanoid31_floating_base():
+    from plainmp_amd.kinematics import BaseType
+
+    robot = synth.humanoid31(0)
+    rng = np.random.default_rng(32)
+    B = 512
+    Q = np.concatenate([synth.uniform_configs(robot, B, seed=33), rng.normal(0, 0.1, (B, 6))], axis=1)
+    prims = synth.mixed_primitives(24, seed=10)
+    for p in prims:  # around the humanoid, whose pelvis is at the origin
+        p["position"] = np.asarray(p["position"], float) - np.array([0.5, 0.0, 0.6])
+    wl = synth.Workload("humanoid31", robot, prims, Q=Q)
+    oc = oracle_cst(wl, base_type=_oracle.BaseType.FLOATING)
+    pc = product_cst(wl, base_type=BaseType.FLOATING)
+    assert pc.q_dim() == 37
+    ref = oc.is_valid_batch(Q, 8).astype(bool)
+    got = pc.is_valid_batch(Q)
+    _report("humanoid_valid", B=B, valid_frac=ref.mean())
+    assert 0.05 < ref.mean() < 0.95
+    assert np.array_equal(got, ref)
+    verr, checked = _check_evaluate(wl, oc, pc, Q, rtol=2e-4)
+    _report("humanoid_evaluate", val_err=verr, rows_checked=checked)
+    assert checked > 50
+    dref = oc.pair_clearance_batch(Q[:64], True, 8)
+    dist, _ = pc.pair_clearance_batch(Q[:64], True)
+    assert np.abs(dist - dref).max() <= 5e-6
+
+
+def test_dual_arm_floating_base_20_columns():
+    """17-32 columns: the dual arm on a floating base; pair distances and
+    their gradient against FP64 central differences, evaluate rows."""
+    from plainmp_amd.kinematics import BaseType
+
+    base = synth.workload("traj_dual_arm", scale=0.002)
+    rng = np.random.default_rng(34)
+    Q = np.concatenate([base.traj_X[:32], rng.normal(0, 0.1, (32, 6))], axis=1)
+    wl = synth.Workload("dual_arm_floating", base.robot, base.prims, Q=Q)
+    oc = oracle_cst(wl, base_type=_oracle.BaseType.FLOATING)
+    pc = product_cst(wl, base_type=BaseType.FLOATING)
+    assert pc.q_dim() == 20
+    assert np.array_equal(pc.is_valid_batch(Q), oc.is_valid_batch(Q, 8).astype(bool))
+    verr, _ = _check_evaluate(wl, oc, pc, Q, rtol=2e-4)
+    dref = oc.pair_clearance_batch(Q, True, 8)
+    dist, grad = pc.pair_clearance_batch(Q, True)
+    assert np.abs(dist - dref).max() <= 5e-6
+    h = 1e-6
+    D = Q.shape[1]
+    fd = np.empty(grad.shape)
+    fd2 = np.empty(grad.shape)
+    for j in range(D):
+        e = np.zeros(D)
+        e[j] = 1.0
+        fd[..., j] = (oc.pair_clearance_batch(Q + h * e, True, 8) - oc.pair_clearance_batch(Q - h * e, True, 8)) / (2 * h)
+        fd2[..., j] = (oc.pair_clearance_batch(Q + 4 * h * e, True, 8) - oc.pair_clearance_batch(Q - 4 * h * e, True, 8)) / (8 * h)
+    smooth = np.abs(fd - fd2) <= 1e-6
+    rel = np.abs(grad - fd) / np.maximum(np.abs(fd), 1e-2)
+    _report("dual_arm_floating", val_err=verr, smooth=int(smooth.sum()), max_rel=rel[smooth].max())
+    assert rel[smooth].max() <= 1e-4
+
+
+def test_edges_linear_visiting_branch():
+    """checkMotion visits n_test - 1 > 127 states in order s2, then i * ratio
+    (motion_validator.cpp:66-76) instead of the farthest-first table: a 0.02
+    rad box validator on the edges_mixed16 scene gives n_test up to ~290."""
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    wl = synth.workload("edges_mixed16", scale=1 / 8)
+    w = np.full(7, 0.02)
+    n_test = synth.edge_waypoint_counts(wl.S1, wl.S2, w) + 1
+    assert n_test.max() > 200 and (n_test <= 128).any() and (n_test > 128).sum() > 100
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(w), 0.0, 8)
+    got, n_got = BoxMotionValidator(pc, w).check_motions(wl.S1, wl.S2)
+    long = n_test > 128
+    _report("edges_long", E=len(ref), long=int(long.sum()), valid_long=float(ref[long].mean()))
+    assert np.array_equal(got, ref.astype(bool))
+    assert np.array_equal(n_got, n_ref)

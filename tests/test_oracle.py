@@ -100,7 +100,8 @@ def _quat_to_matrix(qx, qy, qz, qw):
     ])
 
 
-@pytest.mark.parametrize("robot_fn", [synth.panda_like, synth.mobile_torso, synth.dual_arm])
+@pytest.mark.parametrize("robot_fn", [synth.panda_like, synth.mobile_torso, synth.dual_arm, synth.fetch_like,
+                                      synth.humanoid31])
 def test_fk_exact_matches_numpy_homogeneous(robot_fn):
     robot = robot_fn(0)
     kin = urdf_parse.kinematic_model(robot.urdf)
