@@ -111,14 +111,19 @@ def main():
     kernel_ms = float(np.mean(kernel_ms))
     stream.synchronize()
     n_recheck = mv.last_recheck_count()
+    # the graph's own outputs of its last replay, checked against the oracle
+    # after the timed region (direct calls above rewrite them with the same work)
+    graph.launch()
+    stream.synchronize()
     checked = int(nchk.sum().item())
     ok = valid.cpu().numpy()
+    parity = _parity_headline(wl, ok, nchk.cpu().numpy())
     edges_per_s = E / (ms_step * 1e-3)
     configs_per_s = checked / (ms_step * 1e-3)
     n_wp_total = _total_waypoints(wl)
     wp_flops = roofline.waypoint_flops(wl.robot, wl.prims)
     achieved_tflops = checked * wp_flops / (kernel_ms * 1e-3) / 1e12
-    traffic = _traffic_per_launch("edges_kernel")
+    traffic, traffic_note = _traffic_per_launch("edges_kernel<false>")
 
     out = {
         "metric": "collision-checked configs/sec + Jacobian evals/sec on 1 MI355X vs plain GPU & CPU",
@@ -144,6 +149,8 @@ def main():
             "edges_valid_frac": float(ok.astype(bool).mean()),
             "edges_rechecked_fp64": n_recheck,
         },
+        "parity_checked": parity["equal"],
+        "parity": parity,
         "edges_per_s": edges_per_s,
         "schedule": {**schedule.as_dict(), "autotune": tune_report},
         "covered_configs_per_s": n_wp_total / (ms_step * 1e-3),
@@ -154,6 +161,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tflops / roofline.PEAK_FP32_VALU_TFLOPS,
             "traffic": traffic,
+            "traffic_note": traffic_note,
             "flops_per_waypoint": wp_flops,
             "note": "algorithmic flops = reference-order checked states x full-evaluation flops per state "
                     "(no culling, no early exit inside a state) / edge-kernel time; FP32 VALU peak with "
@@ -163,8 +171,15 @@ def main():
 
     if not args.no_extra:
         out["extra"] = _extra(torch, stream, dev)
+        out["rooflines"] = out["extra"].pop("_rooflines")
         out["plain_per_call_gpu"] = _plain_per_call(wl, mv)
         out["speedup_vs_plain_per_call_edges"] = edges_per_s / out["plain_per_call_gpu"]["edges_per_s"]
+        # north-star comparison: Panda-class arm vs 32 primitives, batched
+        # configurations against one configuration per call (BASELINE "B")
+        b32 = _plain_per_call_configs()
+        out["plain_per_call_gpu_configs"] = b32
+        out["speedup_vs_plain_per_call_configs"] = (out["extra"]["grad_mixed32_is_valid"]["configs_per_s"]
+                                                    / b32["configs_per_s"])
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_baseline(wl)
         out["speedup_vs_cpu_configs"] = configs_per_s / out["cpu_baseline"]["value"]
@@ -177,14 +192,75 @@ def main():
 
 
 def _traffic_per_launch(kernel):
-    """HBM bytes per launch of `kernel` from the committed PMC summary
-    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a plain copy's reading)."""
+    """HBM bytes per launch of `kernel` from the committed PMC table
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a plain copy's reading),
+    or None when the table was measured on other kernel sources than these."""
+    from plainmp_amd import roofline
+
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, "no traffic table"
     with open(path) as f:
-        rec = json.load(f).get(kernel)
-    return None if rec is None else rec.get("bytes_per_launch")
+        table = json.load(f)
+    if table.get("_calibration", {}).get("sources_digest") != roofline.sources_digest():
+        return None, "traffic table measured on other kernel sources (stale): not used"
+    rec = table.get(kernel)
+    if rec is None:
+        return None, "kernel not in the traffic table"
+    return rec.get("bytes_per_launch"), "profiles/traffic.json, this build's sources"
+
+
+def _parity_headline(wl, got_valid, got_n):
+    """The headline replay's per-edge validity and reference-order checked
+    counts against the FP64 oracle (checkMotion, motion_validator.cpp:17-79)
+    on every host thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from support import oracle_cst
+
+    oc = oracle_cst(wl)
+    t0 = time.perf_counter()
+    ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(wl.box_width), 0.0, _host_threads())
+    v_eq = bool(np.array_equal(got_valid.astype(bool), ref.astype(bool)))
+    n_eq = bool(np.array_equal(got_n, n_ref))
+    if not (v_eq and n_eq):
+        print(f"PARITY FAILURE: valid equal {v_eq}, n_checked equal {n_eq}", file=sys.stderr, flush=True)
+    return {"equal": v_eq and n_eq, "valid_equal": v_eq, "n_checked_equal": n_eq, "edges": int(len(ref)),
+            "oracle_s": time.perf_counter() - t0}
+
+
+def _host_threads():
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def _host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def _plain_per_call_configs(n=400):
+    """One configuration per call on the Panda-class arm against 32 primitives
+    (grad_mixed32): synchronous host->device copy, the validity launches and
+    the device->host copy, as a per-call drop-in would do."""
+    from plainmp_amd import synth
+    from plainmp_amd.synth import build_constraint as product_cst
+
+    wl = synth.workload("grad_mixed32")
+    cst = product_cst(wl)
+    for q in wl.Q[:20]:
+        cst.is_valid(q)
+    t0 = time.perf_counter()
+    for q in wl.Q[:n]:
+        cst.is_valid(q)
+    dt = time.perf_counter() - t0
+    return {"configs": n, "s": dt, "configs_per_s": n / dt, "workload": "grad_mixed32 (7R, 60 spheres, 32 primitives)"}
 
 
 def _total_waypoints(wl):
@@ -240,6 +316,19 @@ def _extra(torch, stream, dev):
         "hbm_gbs_algorithmic": bytes_out / (ms * 1e-3) / 1e9,
         "frac_of_hbm_peak": bytes_out / (ms * 1e-3) / 1e9 / roofline.PEAK_HBM_GBS,
     }
+    rl = {}
+    rl["pair_kernel"] = {"workload": "grad_mixed32 pair distances + d/dq, 8192 configs", "bound": "hbm",
+                         "achieved": bytes_out / (ms * 1e-3) / 1e9, "peak": roofline.PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": bytes_out / (ms * 1e-3) / 1e9 / roofline.PEAK_HBM_GBS,
+                         "bytes_per_launch": bytes_out, "launch_ms": ms,
+                         "note": "algorithmic bytes: dist S x P x B + grad S x P x D x B (FP32) written, q read"}
+    # validity of the same 8,192 configurations (the north-star batched rate,
+    # against one configuration per call below)
+    with torch.cuda.stream(stream):
+        valid = torch.empty(B, dtype=torch.uint8, device=dev)
+    ms_v = _time_stream(torch, stream, lambda: cst._dev_is_valid(q.data_ptr(), B, B, valid.data_ptr(), 0, sh))
+    res["grad_mixed32_is_valid"] = {"configs": B, "ms": ms_v, "configs_per_s": B / (ms_v * 1e-3),
+                                    "valid_frac": float(valid.float().mean().item())}
     # reference-semantics evaluate (value + 1 Jacobian row) on the same batch
     C = cst.cst_dim()
     with torch.cuda.stream(stream):
@@ -250,6 +339,11 @@ def _extra(torch, stream, dev):
     ms = _time_stream(torch, stream, lambda: cst._dev_evaluate(q.data_ptr(), B, B, vals.data_ptr(), jac.data_ptr(), sh))
     res["grad_mixed32_evaluate"] = {"configs": B, "ms": ms, "jacobians_per_s": B / (ms * 1e-3),
                                     "autotune": tune_eval}
+    ev_flops = roofline.evaluate_flops(wl.robot, wl.prims)
+    rl["eval_group_kernel"] = {"workload": "grad_mixed32 evaluate (value + row), 8192 configs", "bound": "valu",
+                               "achieved": B * ev_flops / (ms * 1e-3) / 1e12, "peak": roofline.PEAK_FP32_VALU_TFLOPS,
+                               "unit": "TFLOP/s", "frac": B * ev_flops / (ms * 1e-3) / 1e12 / roofline.PEAK_FP32_VALU_TFLOPS,
+                               "flops_per_config": ev_flops, "launch_ms": ms}
     # adjoint (VJP) used by the autograd backward
     with torch.cuda.stream(stream):
         gd = torch.ones((S, P, B), dtype=torch.float32, device=dev)
@@ -274,6 +368,11 @@ def _extra(torch, stream, dev):
         x.data_ptr(), N, N, offl, [], vals.data_ptr(), jac.data_ptr(), cost.data_ptr(), grad.data_ptr(), sh))
     res["traj_dual_arm_cost_grad"] = {"trajectories": T, "waypoints": N, "ms": ms,
                                       "waypoints_per_s": N / (ms * 1e-3), "trajectories_per_s": T / (ms * 1e-3)}
+    tw = roofline.traj_waypoint_flops(wl.robot, wl.prims, synth.self_pair_count(wl.robot))
+    rl["eval_units_kernel"] = {"workload": "traj_dual_arm cost + gradient, 122138 waypoints", "bound": "valu",
+                               "achieved": N * tw / (ms * 1e-3) / 1e12, "peak": roofline.PEAK_FP32_VALU_TFLOPS,
+                               "unit": "TFLOP/s", "frac": N * tw / (ms * 1e-3) / 1e12 / roofline.PEAK_FP32_VALU_TFLOPS,
+                               "flops_per_waypoint": tw, "launch_ms": ms}
     # the same trajectories as the SQP solver's sparse inequality sequence:
     # evaluate rows of every waypoint + motion-step box rows, CSC per trajectory
     msbox = [0.1] * D
@@ -304,9 +403,8 @@ def _extra(torch, stream, dev):
     res["cloud_shelves_is_valid"] = {
         "configs": B, "cloud_points": n_pts, "spheres": cst.n_spheres(), "ms": ms, "configs_per_s": B / (ms * 1e-3),
         "valid_frac": float(valid.float().mean().item()),
-        "dense_equivalent_tflops": B * cst.n_spheres() * n_pts * 8 / (ms * 1e-3) / 1e12,
-        "note": "dense = every (sphere, point) pair as the K=4 product (8 flop); tile culling skips most "
-                "tiles, so this equivalent rate exceeds the 157.3 TFLOP/s FP32 MFMA peak",
+        "note": "whole validity call (FP32 walk, matrix-core cloud pass, decisions, FP64 re-decisions); the "
+                "cloud kernel's matrix-core utilisation is measured with counters (profiles/)",
     }
     # the same arm and cloud, 4,096 motions between consecutive configurations
     # (box validator 0.05 rad): the cloud variant of the edge kernel
@@ -377,6 +475,7 @@ def _extra(torch, stream, dev):
     res["ik_batched_restarts"] = {"restarts": 256, "iterations": 60, "ms": t_ik * 1e3, "success": bool(ik.success),
                                   "first_success_restart": int(ik.n_trial),
                                   "restart_iterations_per_s": 256 * 60 / t_ik}
+    res["_rooflines"] = rl
     res["path_check_and_shortcut"] = {"states": len(P), "check_path_ms": t_path * 1e3,
                                       "segments_per_s": (len(P) - 1) / t_path,
                                       "valid_segment_frac": float(seg.mean()), "shortcut_ms": t_short * 1e3,
@@ -414,11 +513,11 @@ def _cpu_baseline(wl, budget_s=2.0, one_thread_s=1.0):
             k += 1
         return edges, checked, time.perf_counter() - t0
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = _host_threads()
     edges, checked, dt = sample(threads, budget_s, 1024)
     e1, c1, dt1 = sample(1, one_thread_s, 64)
     return {"value": checked / dt, "unit": "configs/s", "cores": threads, "kind": "port",
-            "edges_per_s": edges / dt,
+            "host": _host_info(), "edges_per_s": edges / dt,
             "sample": f"{edges} edges of edges_mixed16 ({checked} waypoints checked in reference order), "
                       f"{dt:.1f} s wall on {threads} threads",
             "one_thread": {"value": c1 / dt1, "unit": "configs/s", "edges_per_s": e1 / dt1,
@@ -432,7 +531,7 @@ def _cpu_evaluate_baseline(budget_s=2.0):
     from plainmp_amd import synth
     from support import oracle_cst
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = _host_threads()
     wl = synth.workload("grad_mixed32")
     oc = oracle_cst(wl)
     done = 0
@@ -444,7 +543,7 @@ def _cpu_evaluate_baseline(budget_s=2.0):
         done += len(wl.Q[sl])
         k += 1
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "jacobians/s", "cores": threads, "kind": "port",
+    return {"value": done / dt, "unit": "jacobians/s", "cores": threads, "kind": "port", "host": _host_info(),
             "sample": f"{done} grad_mixed32 configurations (evaluate: value + Jacobian row), {dt:.1f} s on {threads} threads"}
 
 

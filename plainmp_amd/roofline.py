@@ -80,7 +80,129 @@ def waypoint_flops(robot, prims, self_pairs: int = 0) -> int:
     return 3 * robot.dof + config_flops(robot, prims, self_pairs)
 
 
+def evaluate_flops(robot, prims, self_pairs: int = 0) -> int:
+    """Reference-semantics evaluate of one configuration (constraint values
+    and rows, primitive_sphere_collision.cpp:256-399): every pair's value plus
+    the Jacobian row of the external and of the self argmin (analytic SDF
+    gradient 12 + 17 per column each)."""
+    rows = (1 if prims else 0) + (1 if self_pairs else 0)
+    return config_flops(robot, prims, self_pairs) + rows * (12 + 17 * robot.dof)
+
+
+def traj_waypoint_flops(robot, prims, self_pairs: int = 0) -> int:
+    """One trajectory waypoint of the cost / gradient launch: evaluate plus
+    the smoothness value and gradient (16 per dof)."""
+    return evaluate_flops(robot, prims, self_pairs) + 16 * robot.dof
+
+
+def sources_digest() -> str:
+    """SHA-256 of the kernel and C-ABI sources, to tie measured figures (the
+    committed traffic table) to the build they were measured on."""
+    import hashlib
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(here)):
+        if f.endswith((".hip", ".hpp", ".h", ".cpp")):
+            with open(os.path.join(here, f), "rb") as fh:
+                h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
 def pair_grad_flops(robot, prims, n_rel_mean: float) -> float:
     """Per configuration: FK + every (sphere, primitive) value and its adjoint."""
     n_pairs = robot.n_spheres * len(prims)
     return fk_flops(robot) + sum(robot.n_spheres * _pair_cost(p) for p in prims) + n_pairs * (12 + 17 * n_rel_mean)
+
+
+# ------------------------------------------------- HBM traffic from counters
+def _counter_rows(path):
+    """Rows of a rocprofv3 counter-collection CSV (a file, or a directory
+    searched for *counter_collection.csv)."""
+    import csv
+    import glob
+    import os
+
+    if os.path.isdir(path):
+        hits = sorted(glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True))
+        if not hits:
+            raise FileNotFoundError(f"no counter_collection.csv under {path}")
+        path = hits[0]
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _kernel_base(name: str) -> str:
+    import re
+
+    n = re.sub(r"\(.*", "", name)  # drop the argument list
+    n = re.sub(r"^void ", "", n).strip()
+    return n.split("::")[-1] if "<" not in n else n.split("::", 1)[-1]
+
+
+def traffic_table(fetch_path, write_path, calib_mb: int) -> dict:
+    """Per kernel and grid, HBM bytes per launch from a FETCH_SIZE run and a
+    WRITE_SIZE run (KiB per dispatch), each scaled by what the same counter
+    reads for the plain device copy of `calib_mb` MiB the run made
+    (bench.py --calib-copy-mb): the copy moves calib_mb MiB each way."""
+    import statistics
+
+    def per_kernel(rows, counter):
+        acc = {}
+        for r in rows:
+            if r.get("Counter_Name") != counter:
+                continue
+            key = (_kernel_base(r["Kernel_Name"]), int(r["Grid_Size"]))
+            acc.setdefault(key, []).append(float(r["Counter_Value"]))
+        return acc
+
+    fetch = per_kernel(_counter_rows(fetch_path), "FETCH_SIZE")
+    write = per_kernel(_counter_rows(write_path), "WRITE_SIZE")
+    kib = calib_mb * 1024.0
+
+    def calib(acc):  # the launch whose counter reads closest to the copy size (the copy itself)
+        best = None
+        for (k, g), v in acc.items():
+            m = max(v)
+            if 0.25 * kib <= m <= 2.0 * kib and (best is None or abs(m - kib) < abs(best - kib)):
+                best = m
+        if best is None:
+            raise ValueError("calibration copy not found in the counter run")
+        return kib / best
+
+    ff, wf = calib(fetch), calib(write)
+    out = {"_calibration": {"copy_MiB": calib_mb, "fetch_factor": ff, "write_factor": wf,
+                            "sources_digest": sources_digest()}}
+    for (k, g) in sorted(set(fetch) & set(write)):
+        fv, wv = statistics.mean(fetch[(k, g)]), statistics.mean(write[(k, g)])
+        rec = out.setdefault(k, {"grids": {}})
+        rec["grids"][str(g)] = {"launches": len(fetch[(k, g)]), "fetch_KiB_raw": round(fv, 2),
+                                "write_KiB_raw": round(wv, 2),
+                                "bytes_per_launch": int(round((fv * ff + wv * wf) * 1024))}
+    for k, rec in out.items():
+        if k.startswith("_"):
+            continue
+        grids = rec["grids"]
+        big = max(grids, key=lambda g: int(g))  # the largest grid of a kernel is its benchmark launch
+        rec["bytes_per_launch"] = grids[big]["bytes_per_launch"]
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="HBM traffic per kernel launch from two rocprofv3 --pmc runs")
+    ap.add_argument("--fetch", required=True, help="output of the FETCH_SIZE run (CSV or directory)")
+    ap.add_argument("--write", required=True, help="output of the WRITE_SIZE run (CSV or directory)")
+    ap.add_argument("--calib-mb", type=int, default=1024)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    table = traffic_table(a.fetch, a.write, a.calib_mb)
+    with open(a.out, "w") as f:
+        json.dump(table, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

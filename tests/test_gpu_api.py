@@ -39,9 +39,11 @@ def test_set_sdf_aliasing():
     oc.set_sdf(oprims)
     assert np.array_equal(cst.is_valid_batch(wl.Q), oc.is_valid_batch(wl.Q, 8).astype(bool))
     before = cst.is_valid_batch(wl.Q)
-    for i, (dx, yaw) in enumerate([((0.0, 0.0, -0.3), 0.0), ((0.15, -0.1, 0.05), 0.7), ((0.0, 0.2, 0.0), -1.1)]):
-        sdfs[i].translate(np.array(dx))
-        oprims[i].translate(np.array(dx))
+    # a box and a cylinder moved into the arm's reach, the cylinder also turned
+    for i, target, yaw in [(0, (0.3, 0.0, 0.45), 0.0), (2, (0.0, 0.35, 0.6), 0.7)]:
+        dx = np.array(target) - np.asarray(wl.prims[i]["position"], float)
+        sdfs[i].translate(dx)
+        oprims[i].translate(dx)
         if yaw:
             sdfs[i].rotate_z(yaw)
             oprims[i].rotate_z(yaw)
@@ -87,8 +89,9 @@ def test_graph_refuses_stale_tables():
     oc.set_sdf(oprims)
     ref, n_ref = oc.check_motions(wl.S1, wl.S2, True, list(wl.box_width), 0.0, 8)
     assert np.array_equal(valid.cpu().numpy().astype(bool), ref.astype(bool))
-    sdfs[0].translate(np.array([0.1, 0.0, 0.0]))
-    oprims[0].translate(np.array([0.1, 0.0, 0.0]))
+    dx = np.array([0.3, 0.0, 0.45]) - np.asarray(wl.prims[0]["position"], float)
+    sdfs[0].translate(dx)
+    oprims[0].translate(dx)
     with pytest.raises(ValueError, match="capture it again"):
         g.launch()
     g.destroy()
