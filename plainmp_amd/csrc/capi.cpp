@@ -840,6 +840,8 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
     for (int i = 0; i < N; ++i) n_slots = std::max(n_slots, slot[i] + 1);
     a.n_slots = d.n_slots = n_slots;
     a.slot_off = d.slot_off = 0;
+    a.slot_wave_bytes = n_slots * 7 * 64 * (int)sizeof(float);
+    d.slot_wave_bytes = n_slots * 7 * 64 * (int)sizeof(double);
     a.lds_limit = d.lds_limit = ctx->max_lds;
     PMP_REQUIRE(slot_bytes(d, 64) <= (size_t)ctx->max_lds,
                 "kinematic tree too branched for this device: " + std::to_string(n_slots) +

@@ -74,35 +74,37 @@ struct is_ranged<V, decltype((void)V::kRanged)> {
 
 // Kept poses (ancestors of later branches, links of self pairs) live in LDS:
 // the launch reserves slot_bytes(ka, block) bytes of dynamic shared memory
-// from byte offset ka.slot_off, laid out [slot][component][lane] so that the
-// 64 lanes of a wave touch 64 consecutive words (no bank conflicts) and
-// nothing goes to scratch.  Slot indices are wave-uniform; each lane reads
-// only what it wrote, so no barrier is needed.
+// from byte offset ka.slot_off, ka.slot_wave_bytes per wave, laid out
+// [wave][slot][component][lane] so that the 64 lanes of a wave touch 64
+// consecutive words (no bank conflicts) and nothing goes to scratch.  A wave
+// owns its chunk: a kernel that runs FK in FP32 and then in FP64 in the same
+// wave (the edge pass) sizes the chunk for FP64 and reuses it.  Slot indices
+// are wave-uniform; each lane reads only what it wrote, so no barrier is
+// needed.
 template <class T>
 struct SlotFile {
   T* base;
-  int stride;
-  __device__ __forceinline__ explicit SlotFile(int32_t off) {
+  __device__ __forceinline__ explicit SlotFile(const KinArgs<T>& ka) {
     HIP_DYNAMIC_SHARED(unsigned char, pmp_dyn_lds)
-    base = reinterpret_cast<T*>(pmp_dyn_lds + off) + threadIdx.x;
-    stride = (int)blockDim.x;
+    base = reinterpret_cast<T*>(pmp_dyn_lds + ka.slot_off + (threadIdx.x >> 6) * ka.slot_wave_bytes) +
+           (threadIdx.x & 63);
   }
   __device__ __forceinline__ Pose<T> get(int i) const {
-    const T* p = base + i * 7 * stride;
+    const T* p = base + i * 7 * 64;
     Pose<T> r;
-    r.q = {p[0], p[stride], p[2 * stride], p[3 * stride]};
-    r.t = {p[4 * stride], p[5 * stride], p[6 * stride]};
+    r.q = {p[0], p[64], p[128], p[192]};
+    r.t = {p[256], p[320], p[384]};
     return r;
   }
   __device__ __forceinline__ void set(int i, const Pose<T>& P) {
-    T* p = base + i * 7 * stride;
+    T* p = base + i * 7 * 64;
     p[0] = P.q.x;
-    p[stride] = P.q.y;
-    p[2 * stride] = P.q.z;
-    p[3 * stride] = P.q.w;
-    p[4 * stride] = P.t.x;
-    p[5 * stride] = P.t.y;
-    p[6 * stride] = P.t.z;
+    p[64] = P.q.y;
+    p[128] = P.q.z;
+    p[192] = P.q.w;
+    p[256] = P.t.x;
+    p[320] = P.t.y;
+    p[384] = P.t.z;
   }
 };
 
@@ -159,7 +161,7 @@ __device__ __forceinline__ Pose<T> base_pose(const KinArgs<T>& ka, const QS& qs)
 // `taylor` selects the collision check's sin/cos (a runtime, wave-uniform flag).
 template <class T, class QS, class Vis>
 __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool taylor, Vis& vis) {
-  SlotFile<T> slots(ka.slot_off);
+  SlotFile<T> slots(ka);
   const Pose<T> base = base_pose(ka, qs);
   vis.begin(base, qs);
   Pose<T> prev = base;

@@ -1716,12 +1716,15 @@ hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, c
     if (e != hipSuccess) return e;
   }
   // scenes with a point cloud take the variant that tests clouds in the walk;
-  // the FP32 walk and the in-kernel FP64 re-decision share the slot area
+  // a wave's FP32 walk and its FP64 re-decisions after it share the wave's
+  // kept-pose chunk, sized for FP64
+  KinArgs<float> kw = k32;
+  kw.slot_wave_bytes = k64.slot_wave_bytes;
   const size_t lds = slot_bytes(k64, block);
   if (s32.n_cloud > 0)
-    hipLaunchKernelGGL(edges_kernel<true>, dim3(grid), dim3(block), lds, st, k32, s32, k64, s64, ea, band);
+    hipLaunchKernelGGL(edges_kernel<true>, dim3(grid), dim3(block), lds, st, kw, s32, k64, s64, ea, band);
   else
-    hipLaunchKernelGGL(edges_kernel<false>, dim3(grid), dim3(block), lds, st, k32, s32, k64, s64, ea, band);
+    hipLaunchKernelGGL(edges_kernel<false>, dim3(grid), dim3(block), lds, st, kw, s32, k64, s64, ea, band);
   if (ev_after) {
     hipError_t e = hipEventRecord(ev_after, st);
     if (e != hipSuccess) return e;

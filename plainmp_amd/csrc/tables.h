@@ -147,8 +147,9 @@ struct KinArgs {
   int32_t n_q_cols, base_type, n_ext_spheres;
   uint64_t rev_mask;  // q columns that rotate (revolute joints, base yaw/roll/pitch)
   int32_t n_slots;    // kept-pose slots the FK walk uses
-  int32_t slot_off;   // byte offset of the kept-pose slots in the launch's dynamic LDS
-  int32_t lds_limit;  // LDS bytes one workgroup may use on this device (launch sizing)
+  int32_t slot_off;         // byte offset of the kept-pose slots in the launch's dynamic LDS
+  int32_t slot_wave_bytes;  // kept-pose bytes per wave (n_slots * 7 * 64 * sizeof(T), or more)
+  int32_t lds_limit;        // LDS bytes one workgroup may use on this device (launch sizing)
   T base_q[4];
   T base_t[3];
 };
@@ -157,7 +158,7 @@ struct KinArgs {
 // precision T (nothing for a robot that keeps none).
 template <class T>
 inline size_t slot_bytes(const KinArgs<T>& ka, int block) {
-  return (size_t)ka.n_slots * 7 * sizeof(T) * (size_t)block;
+  return (size_t)ka.slot_wave_bytes * (size_t)(block / 64);
 }
 // The largest workgroup (a multiple of 64 lanes, at most `want`) whose
 // kept-pose slots in precision T fit the LDS left beside `own` bytes; 64 at
