@@ -192,6 +192,18 @@ pmp_status pmp_sphere_clearance_batch(pmp_context* ctx, const double* q, int64_t
 pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, uint8_t* valid,
                               int32_t* n_recheck);
 
+/* pmp_is_valid_batch without any allocation: no hipMalloc, hipFree or
+ * allocating library call, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops).  No counterpart in the reference.
+ * Its workspaces come from a static device array of 32 MiB that every GPU
+ * the library runs on holds until the process ends; a batch that needs more
+ * (scenes with a point cloud: 28 B per primitive-checked sphere and
+ * configuration plus 4 B per configuration) is refused with PMP_ERR_INVALID.
+ * All calls on one GPU share that array: calls that overlap on one GPU (from
+ * several contexts, or captured into graphs that replay concurrently) race. */
+pmp_status pmp_is_valid_batch_noalloc(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, uint8_t* valid,
+                                      int32_t* n_recheck);
+
 /* Reference-semantics value and Jacobian rows, vals[k * B + b],
  * jac[(k * q_dim + j) * B + b] for k < cst_dim (ext first, then self).
  * Replaces ConstraintBase::evaluate -> SphereCollisionCst::evaluate_dirty

@@ -475,7 +475,7 @@ namespace {
 // End of an entry point that enqueued work: remember it for set_device.
 void record_call(pmp_context* ctx) noexcept {
   if (capturing(ctx->stream)) return;
-  if (!ctx->call_ev && hipEventCreateWithFlags(&ctx->call_ev, hipEventDisableTiming) != hipSuccess) return;
+  if (!ctx->call_ev) return;  // created with the context
   if (hipEventRecord(ctx->call_ev, ctx->stream) != hipSuccess) return;
   ctx->call_recorded = true;
   ctx->call_stream = ctx->stream;
@@ -501,6 +501,8 @@ pmp_status pmp_create(int32_t device, pmp_context** out) {
       PMP_HIP(hipGetDeviceProperties(&prop, device));
       ctx->n_cu = prop.multiProcessorCount;
       ctx->max_lds = (int)prop.sharedMemPerBlock;
+      PMP_HIP(noalloc_scratch_init(device));
+      PMP_HIP(hipEventCreateWithFlags(&ctx->call_ev, hipEventDisableTiming));
       ctx->seq.upload(sequence_table(), nullptr);
       PMP_HIP(hipDeviceSynchronize());
     } catch (...) {
@@ -1062,6 +1064,51 @@ pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, in
       unit_flags = (int32_t*)ctx->ws_vunit.get(sizeof(int32_t) * kValidUnitsMaxBatch);
       if (fresh) PMP_HIP(launch_fill_i32(unit_flags, 0, kValidUnitsMaxBatch, st(ctx)));
     }
+    PMP_HIP(launch_valid(ctx->k32, ctx->k64, ctx->s32, ctx->s64, q, ldq, B, ctx->band, valid, n_recheck,
+                         ctx->sched.block, cw, ctx->clouds.data(), ctx->n_clouds, flags, unit_flags, st(ctx)));
+  });
+}
+
+pmp_status pmp_is_valid_batch_noalloc(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, uint8_t* valid,
+                                      int32_t* n_recheck) {
+  return guarded([&] {
+    PMP_BATCH_CHECKS(q, ldq, B);
+    PMP_REQUIRE(B == 0 || valid, "null output");
+    if (B == 0) return;
+    set_device(ctx);
+    unsigned char* ws = (unsigned char*)noalloc_scratch(ctx->device);
+    PMP_REQUIRE(ws != nullptr, "internal: no-allocation scratch not set up on this device");
+    // The same workspaces as pmp_is_valid_batch, carved from the static
+    // scratch (256-byte aligned pieces).
+    auto carve = [&](size_t& at, size_t bytes) {
+      unsigned char* p = ws + at;
+      at += (bytes + 255) & ~(size_t)255;
+      return p;
+    };
+    size_t need = 0;
+    CloudWork cw{};
+    int32_t* flags = nullptr;
+    int32_t* unit_flags = nullptr;
+    if (ctx->n_clouds) {
+      const int64_t Q = (int64_t)ctx->k32.n_ext_spheres * B;
+      cw.B = B;
+      cw.cen = (float*)carve(need, sizeof(float) * 3 * (size_t)Q);
+      float* w = (float*)carve(need, sizeof(float) * 4 * (size_t)Q * ctx->n_clouds);
+      for (int oc = 0; oc < ctx->n_clouds; ++oc) {
+        cw.cap2[oc] = w + 4 * Q * oc;
+        cw.rr[oc] = cw.cap2[oc] + Q;
+      }
+      flags = (int32_t*)carve(need, sizeof(int32_t) * (size_t)B);
+    } else if (B <= kValidUnitsMaxBatch && ctx->k32.n_units > 0) {
+      unit_flags = (int32_t*)carve(need, sizeof(int32_t) * (size_t)B);
+    }
+    PMP_REQUIRE(need <= kNoAllocScratchBytes,
+                "batch needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                    std::to_string(kNoAllocScratchBytes) +
+                    " bytes pmp_is_valid_batch_noalloc has; split the batch or call pmp_is_valid_batch");
+    // the scratch keeps the previous call's contents: the unit pass needs its
+    // flags at zero (a fill kernel, not a memset node)
+    if (unit_flags) PMP_HIP(launch_fill_i32(unit_flags, 0, B, st(ctx)));
     PMP_HIP(launch_valid(ctx->k32, ctx->k64, ctx->s32, ctx->s64, q, ldq, B, ctx->band, valid, n_recheck,
                          ctx->sched.block, cw, ctx->clouds.data(), ctx->n_clouds, flags, unit_flags, st(ctx)));
   });

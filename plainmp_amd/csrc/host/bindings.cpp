@@ -537,6 +537,12 @@ PYBIND11_MODULE(_native, m) {
              check(pmp_is_valid_batch(c.device_context((void*)stream), (const double*)q, ldq, B, (uint8_t*)valid,
                                       (int32_t*)nre));
            })
+      .def("_dev_is_valid_noalloc",
+           [](SphereCollisionCst& c, uintptr_t q, int64_t ldq, int64_t B, uintptr_t valid, uintptr_t nre,
+              uintptr_t stream) {
+             check(pmp_is_valid_batch_noalloc(c.device_context((void*)stream), (const double*)q, ldq, B,
+                                              (uint8_t*)valid, (int32_t*)nre));
+           })
       .def("_dev_sphere_clearance",
            [](SphereCollisionCst& c, uintptr_t q, int64_t ldq, int64_t B, int accurate, uintptr_t out,
               uintptr_t stream) {

@@ -84,6 +84,13 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
 // per (configuration, evaluate unit); unit_flags: [B] int32, zero on entry
 // and left zero.
 constexpr int64_t kValidUnitsMaxBatch = 16384;
+// noalloc.hip: the scratch of pmp_is_valid_batch_noalloc, a static device
+// array of kNoAllocScratchBytes per GPU.  noalloc_scratch_init looks up this
+// device's copy (call with `device` current); noalloc_scratch returns it
+// (null before the lookup).
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+hipError_t noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
 // cloud.hip
 hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                               const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
