@@ -173,6 +173,36 @@ pmp_status pmp_fk_spheres_batch(pmp_context* ctx, const double* q, int64_t ldq, 
 pmp_status pmp_link_jacobian_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t link,
                                    const double* point, float* jac);
 
+/* One damped Gauss-Newton update of R restarts of a multi-start IK, a lane
+ * per restart, in place on q[j * R + r] (FP64, device).  No counterpart in
+ * the reference, whose solve_ik (src/plainmp/ik.py:42-156) runs SLSQP one
+ * seed after another on the LinkPoseCst residual
+ * (constraints/primitive_link_pose.cpp:32-57): here the residual rows are
+ * that constraint's (position; + getRPY() - rpy or quaternion coefficients
+ * for 6- or 7-vector targets, with rpy_derivative / q_derivative rows,
+ * kinematics/algorithm.cpp:21-47) plus collision values below `margin` as
+ * penalty rows; the step solves (A^T A + damping I) s = -A^T r (Cholesky,
+ * FP64), is scaled to at most max_step in the max norm and projected on
+ * [lo, hi].  Inputs are device arrays of this iteration: poses from
+ * pmp_link_poses_batch, jac0 from pmp_link_jacobian_batch at the link origin
+ * and, for kind 6 / 7, jac_u[k] at the link's unit axes e_k; vals / cjac from
+ * pmp_evaluate_batch (n_cst rows, or NULL).  q_dim <= 16. */
+typedef struct pmp_ik_step_desc {
+  int32_t kind;          /* 3, 6 or 7 */
+  int32_t link;          /* node index of the target link */
+  const double* target;  /* device [kind] */
+  const float* jac0;
+  const float* jac_u[3];
+  const float* vals;
+  const float* cjac;
+  int32_t n_cst;
+  const double* lo;      /* device [q_dim] */
+  const double* hi;
+  double damping, max_step, margin, penalty;
+} pmp_ik_step_desc;
+pmp_status pmp_ik_step_batch(pmp_context* ctx, double* q, int64_t R, const double* poses,
+                             const pmp_ik_step_desc* step);
+
 /* Link poses of all nodes, poses[(n*7 + k) * B + b] as x,y,z,qx,qy,qz,qw
  * (FP64).  Replaces set_joint_angles + get_link_pose
  * (kinematics/algorithm.cpp:88-107, kinematics/kinematics.hpp:131-136). */
@@ -287,6 +317,16 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
                                  pmp_graph** out);
 pmp_status pmp_graph_launch(pmp_graph* graph);
 pmp_status pmp_graph_destroy(pmp_graph* graph);
+/* Same as pmp_traj_cost_grad_batch, captured once into a HIP graph (its
+ * offsets and weights copied into the graph's own device buffers) and
+ * replayed by pmp_graph_launch: trajectories of uneven length in one launch
+ * per replay, the loop of SequentialCst::evaluate over waypoints
+ * (constraints/sequential_constraint.cpp:63-127) for every trajectory.  X and
+ * the outputs must stay valid while the graph exists; the context rules of
+ * pmp_check_edges_graph apply (a table change makes launch refuse). */
+pmp_status pmp_traj_cost_grad_graph(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
+                                    int32_t n_traj, const double* weights, float* vals, float* jac, float* cost,
+                                    float* grad, pmp_graph** out);
 /* Duration of the FP32 edge kernel of the most recent pmp_check_edges_batch
  * on this context, from HIP events recorded around it on the context stream
  * (waits for that kernel).  No reference counterpart: it serves throughput

@@ -63,7 +63,7 @@ def build_library(verbose: bool = False, force: bool = True) -> str:
     from concurrent.futures import ThreadPoolExecutor
 
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
-    srcs = ["kernels.hip", "cloud.hip", "cloud_build.hip", "noalloc.hip", "capi.cpp"]
+    srcs = ["kernels.hip", "cloud.hip", "cloud_build.hip", "noalloc.hip", "ik.hip", "capi.cpp"]
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = [os.path.join(objdir, s + ".o") for s in srcs]

@@ -14,6 +14,18 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from ._native.constraint import SphereAttachmentSpec, SphereCollisionCst  # noqa: F401
+from ._native.ompl import _dev_capture_traj_graph
+
+
+def capture_traj_graph(cst: SphereCollisionCst, X_ptr: int, ldx: int, N: int, offsets: Sequence[int],
+                       weights: Sequence[float], vals_ptr: int, jac_ptr: int, cost_ptr: int, grad_ptr: int,
+                       stream: int):
+    """``traj_cost_grad`` over device buffers (pointers as integers, layouts of
+    pmp_traj_cost_grad_batch) captured once into a HIP graph: ``launch()``
+    replays every trajectory's evaluate rows, smoothness cost and gradient in
+    one launch; ``destroy()`` frees it (the constraint outlives the graph)."""
+    return _dev_capture_traj_graph(cst, X_ptr, ldx, N, [int(o) for o in offsets], [float(w) for w in weights],
+                                   vals_ptr, jac_ptr, cost_ptr, grad_ptr, stream)
 
 
 def sqp_ineq_matrices(cst: SphereCollisionCst, X: np.ndarray, offsets: Sequence[int],
@@ -42,4 +54,4 @@ def sqp_ineq_matrices(cst: SphereCollisionCst, X: np.ndarray, offsets: Sequence[
     return out
 
 
-__all__ = ["SphereAttachmentSpec", "SphereCollisionCst", "sqp_ineq_matrices"]
+__all__ = ["SphereAttachmentSpec", "SphereCollisionCst", "sqp_ineq_matrices", "capture_traj_graph"]

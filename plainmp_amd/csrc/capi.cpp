@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "ik.h"
 #include "launch.h"
 
 using namespace pmp;
@@ -199,6 +200,7 @@ struct pmp_graph {
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
   EdgeBufs eb;
+  DevBuf traj_off, traj_w2;  // a trajectory graph's offsets and squared weights
   const pmp_context* ctx = nullptr;
   uint64_t tables_gen = 0;  // the context's tables_gen at capture
 };
@@ -1018,6 +1020,43 @@ pmp_status pmp_link_jacobian_batch(pmp_context* ctx, const double* q, int64_t ld
   });
 }
 
+pmp_status pmp_ik_step_batch(pmp_context* ctx, double* q, int64_t R, const double* poses,
+                             const pmp_ik_step_desc* d) {
+  return guarded([&] {
+    require_robot(ctx);
+    PMP_REQUIRE(d != nullptr && R >= 0, "null step descriptor or negative batch");
+    PMP_REQUIRE(d->kind == 3 || d->kind == 6 || d->kind == 7, "All poses must be 3 or 6 or 7 dimensional");
+    PMP_REQUIRE(ctx->n_cols <= 16, "the batched IK step handles at most 16 q columns");
+    PMP_REQUIRE(d->link >= 0 && d->link < ctx->n_nodes, "link index out of range");
+    PMP_REQUIRE(d->n_cst >= 0 && d->n_cst <= 2, "n_cst out of range");
+    if (R == 0) return;
+    PMP_REQUIRE(q && poses && d->target && d->jac0 && d->lo && d->hi, "null argument");
+    PMP_REQUIRE(d->kind == 3 || (d->jac_u[0] && d->jac_u[1] && d->jac_u[2]), "rotation targets need jac_u");
+    PMP_REQUIRE(d->n_cst == 0 || (d->vals && d->cjac), "null collision rows");
+    set_device(ctx);
+    IkArgs a{};
+    a.R = R;
+    a.D = ctx->n_cols;
+    a.C = d->n_cst;
+    a.kind = d->kind;
+    a.link = d->link;
+    a.poses = poses;
+    a.tgt = d->target;
+    a.J0 = d->jac0;
+    for (int k = 0; k < 3; ++k) a.Ju[k] = d->jac_u[k];
+    a.vals = d->vals;
+    a.cjac = d->cjac;
+    a.lo = d->lo;
+    a.hi = d->hi;
+    a.damping = d->damping;
+    a.max_step = d->max_step;
+    a.margin = d->margin;
+    a.wpen = std::sqrt(d->penalty);
+    a.q = q;
+    PMP_HIP(launch_ik_step(a, st(ctx)));
+  });
+}
+
 pmp_status pmp_sphere_clearance_batch(pmp_context* ctx, const double* q, int64_t ldq, int64_t B, int32_t accurate,
                                       float* clearance) {
   return guarded([&] {
@@ -1128,16 +1167,76 @@ pmp_status pmp_evaluate_batch(pmp_context* ctx, const double* q, int64_t ldq, in
   });
 }
 
+namespace {
+// Argument checks shared by the direct and the captured trajectory call.
+void check_traj_args(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
+                     int32_t n_traj, float* vals, float* jac, float* cost, float* grad) {
+  PMP_BATCH_CHECKS(X, ldx, N);
+  PMP_REQUIRE(n_traj >= 0 && offsets != nullptr, "missing trajectory offsets");
+  PMP_REQUIRE(offsets[0] == 0 && offsets[n_traj] == N, "offsets must start at 0 and end at N");
+  for (int32_t i = 0; i < n_traj; ++i) PMP_REQUIRE(offsets[i + 1] >= offsets[i], "offsets must not decrease");
+  PMP_REQUIRE(N == 0 || (vals && jac && grad), "null output");
+  PMP_REQUIRE(n_traj == 0 || cost, "null output");
+}
+}  // namespace
+
+pmp_status pmp_traj_cost_grad_graph(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
+                                    int32_t n_traj, const double* weights, float* vals, float* jac, float* cost,
+                                    float* grad, pmp_graph** out) {
+  return guarded([&] {
+    PMP_REQUIRE(out != nullptr, "null output pointer");
+    check_traj_args(ctx, X, ldx, N, offsets, n_traj, vals, jac, cost, grad);
+    PMP_REQUIRE(n_traj > 0, "graph of an empty trajectory batch");
+    set_device(ctx);
+    auto* g = new pmp_graph();
+    try {
+      // the graph's own copies of the offsets and weights, in place before capture
+      std::vector<int64_t> off(offsets, offsets + n_traj + 1);
+      std::vector<double> w2(std::max(ctx->n_cols, 1), 1.0);
+      for (int j = 0; j < ctx->n_cols; ++j) w2[j] = weights ? weights[j] * weights[j] : 1.0;
+      g->traj_off.upload(off, st(ctx));
+      g->traj_w2.upload(w2, st(ctx));
+      PMP_HIP(hipStreamSynchronize(st(ctx)));
+      hipStream_t cs = ctx->stream;
+      if (!cs) {
+        if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        cs = ctx->capture_stream;
+      }
+      g->stream = cs;
+      EvalArgs e{};
+      e.vals = vals;
+      e.jac = jac;
+      e.offsets = (const int64_t*)g->traj_off.ptr;
+      e.n_traj = n_traj;
+      e.w2 = (const double*)g->traj_w2.ptr;
+      e.cost = cost;
+      e.grad = grad;
+      e.whole_chunks = ctx->sched.eval_whole_chunks;
+      PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      hipError_t le = launch_eval_units(ctx->k32, ctx->s32, X, ldx, N, e, cs);
+      hipError_t ce = hipStreamEndCapture(cs, &g->graph);
+      PMP_HIP(le);
+      PMP_HIP(ce);
+      PMP_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+      g->ctx = ctx;
+      g->tables_gen = ctx->tables_gen;
+    } catch (...) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      g->traj_off.release();
+      g->traj_w2.release();
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
 pmp_status pmp_traj_cost_grad_batch(pmp_context* ctx, const double* X, int64_t ldx, int64_t N, const int64_t* offsets,
                                     int32_t n_traj, const double* weights, float* vals, float* jac, float* cost,
                                     float* grad) {
   return guarded([&] {
-    PMP_BATCH_CHECKS(X, ldx, N);
-    PMP_REQUIRE(n_traj >= 0 && offsets != nullptr, "missing trajectory offsets");
-    PMP_REQUIRE(offsets[0] == 0 && offsets[n_traj] == N, "offsets must start at 0 and end at N");
-    for (int32_t i = 0; i < n_traj; ++i) PMP_REQUIRE(offsets[i + 1] >= offsets[i], "offsets must not decrease");
-    PMP_REQUIRE(N == 0 || (vals && jac && grad), "null output");
-    PMP_REQUIRE(n_traj == 0 || cost, "null output");
+    check_traj_args(ctx, X, ldx, N, offsets, n_traj, vals, jac, cost, grad);
     if (n_traj == 0) return;
     set_device(ctx);
     std::vector<int64_t> off(offsets, offsets + n_traj + 1);
@@ -1425,6 +1524,8 @@ pmp_status pmp_graph_destroy(pmp_graph* g) {
     if (g->exec) PMP_HIP(hipGraphExecDestroy(g->exec));
     if (g->graph) PMP_HIP(hipGraphDestroy(g->graph));
     g->eb.release();
+    g->traj_off.release();
+    g->traj_w2.release();
     delete g;
   });
 }

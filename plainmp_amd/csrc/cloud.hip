@@ -20,43 +20,75 @@ namespace pmp {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// min |c - p|^2 over the cloud's tiles for queries q < Q whose cap2[q] >= 0;
-// centres in cen[(q / B * 3 + k) * B + q % B].  Writes out_d2[q] = min squared
-// distance (FLT_MAX when no point lies within the cap).  With kBounds, the
-// minimum is bracketed instead: out_d2[q] = min over tiles of (d2 + err) and
-// out_lo[q] = min over tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 +
-// rho^2) per tile, so the exact minimum lies in [out_lo, out_d2].  `skip`
-// (optional, per configuration q % B): queries of configurations already
-// decided (0) are not evaluated.  Super-tiles are tested first, then their
-// tiles; a (super-)tile is skipped when no query's cap reaches its AABB.
+// Squared distance between two boxes (0 when they overlap).
+__device__ __forceinline__ float box_box_d2(float alx, float aly, float alz, float aux, float auy, float auz,
+                                            float blx, float bly, float blz, float bux, float buy, float buz) {
+  const float dx = fmaxf(fmaxf(alx - bux, blx - aux), 0.f);
+  const float dy = fmaxf(fmaxf(aly - buy, bly - auy), 0.f);
+  const float dz = fmaxf(fmaxf(alz - buz, blz - auz), 0.f);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ float wave_min16(float v) {
+  for (int s = 1; s < 16; s <<= 1) v = fminf(v, __shfl_xor(v, s, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max16(float v) {
+  for (int s = 1; s < 16; s <<= 1) v = fmaxf(v, __shfl_xor(v, s, 64));
+  return v;
+}
+
+// min |c - p|^2 over the cloud's tiles for the queries q = e * B + b
+// (e < Q / B spheres, b < B configurations) whose cap2[q] >= 0; centres in
+// cen[(e * 3 + k) * B + b].  Writes out_d2[q] = min squared distance
+// (FLT_MAX when no point lies within the cap).  With kBounds, the minimum is
+// bracketed instead: out_d2[q] = min over tiles of (d2 + err) and out_lo[q] =
+// min over tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 + rho^2) per
+// tile, so the exact minimum lies in [out_lo, out_d2].  `skip` (optional, per
+// configuration b): configurations already decided (0) are not evaluated.
+//
+// A wave takes 16 consecutive spheres of one configuration (spheres are
+// ordered by link, so they are close together; consecutive waves take the
+// next configurations) and culls by the box around
+// its queries grown by their largest reach: 64 super-tiles at a time are
+// tested against that box, a lane each, and a ballot lists the survivors;
+// then a survivor's tiles, a lane each; a surviving tile is then tested per
+// query and, if any query reaches it, compared on the matrix cores.
 template <bool kBounds>
 __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, const float* tileB,
                                                         const CloudSuper* supers, int super_begin, int n_super,
                                                         const float* cen, const float* cap2, int64_t B, int64_t Q,
                                                         const uint8_t* skip, float* out_d2, float* out_lo) {
   const int lane = threadIdx.x & 63;
-  const int64_t q0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 16;
-  if (q0 >= Q) return;  // wave-uniform
+  const int64_t S = Q / B;  // spheres per configuration
+  const int64_t groups = (S + 15) / 16;
+  // Waves in configuration-fastest order, workgroups handed to the XCDs in
+  // turn and each XCD given a contiguous range: neighbouring configurations
+  // read the same lines of the [sphere][B] query arrays, in one L2.
+  const int g = blockIdx.x, xcd = g % kXcds;
+  int64_t first = 0;
+  for (int x = 0; x < xcd; ++x) first += ((int64_t)gridDim.x - x + kXcds - 1) / kXcds;
+  const int64_t w = (first + g / kXcds) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= groups * B) return;  // wave-uniform
+  const int64_t b = w % B, e0 = (w / B) * 16;
   const int i = lane & 15, k = lane >> 4;
+  const bool skipped = skip && skip[b] == 0;
   // the A-row query of this lane
-  const int64_t qi = q0 + i;
   float cx = 0.f, cy = 0.f, cz = 0.f, capi = -1.f;
-  if (qi < Q) {
-    const int64_t e = qi / B, b = qi % B;
+  if (e0 + i < S) {
+    const int64_t e = e0 + i;
     cx = cen[(e * 3 + 0) * B + b];
     cy = cen[(e * 3 + 1) * B + b];
     cz = cen[(e * 3 + 2) * B + b];
-    capi = cap2 ? cap2[qi] : FLT_MAX;
-    if (skip && skip[b] == 0) capi = -1.f;
+    capi = skipped ? -1.f : (cap2 ? cap2[e * B + b] : FLT_MAX);
   }
   // centres of the four output rows 4k + r held by this lane
   float rx[4], ry[4], rz[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int64_t q = q0 + 4 * k + r;
+    const int64_t e = e0 + 4 * k + r;
     rx[r] = ry[r] = rz[r] = 0.f;
-    if (q < Q) {
-      const int64_t e = q / B, b = q % B;
+    if (e < S) {
       rx[r] = cen[(e * 3 + 0) * B + b];
       ry[r] = cen[(e * 3 + 1) * B + b];
       rz[r] = cen[(e * 3 + 2) * B + b];
@@ -66,37 +98,61 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   float lo[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   if (__any(capi >= 0.f)) {
-    for (int si = 0; si < n_super; ++si) {
-      const CloudSuper S = cload(supers, super_begin + si);
-      const float sx = fmaxf(fmaxf(S.lb[0] - cx, cx - S.ub[0]), 0.f);
-      const float sy = fmaxf(fmaxf(S.lb[1] - cy, cy - S.ub[1]), 0.f);
-      const float sz = fmaxf(fmaxf(S.lb[2] - cz, cz - S.ub[2]), 0.f);
-      if (!__any(sx * sx + sy * sy + sz * sz <= capi)) continue;
-      for (int t = S.first; t < S.first + S.count; ++t) {
-        const CloudTile T = cload(tiles, t);
-        const float dx = fmaxf(fmaxf(T.lb[0] - cx, cx - T.ub[0]), 0.f);
-        const float dy = fmaxf(fmaxf(T.lb[1] - cy, cy - T.ub[1]), 0.f);
-        const float dz = fmaxf(fmaxf(T.lb[2] - cz, cz - T.ub[2]), 0.f);
-        const bool need = dx * dx + dy * dy + dz * dz <= capi;
-        if (!__any(need)) continue;
-        const float a = k == 0 ? cx - T.o[0] : (k == 1 ? cy - T.o[1] : (k == 2 ? cz - T.o[2] : 1.f));
-        const float* Bt = tileB + (int64_t)t * 256;
-        floatx4 d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[lane], zero, 0, 0, 0);
-        floatx4 d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[64 + lane], zero, 0, 0, 0);
-        floatx4 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[128 + lane], zero, 0, 0, 0);
-        floatx4 d3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[192 + lane], zero, 0, 0, 0);
-        const float rho2 = kBounds ? tile_rho2(T) : 0.f;
+    // the box around the live queries and their largest cap
+    const bool live = capi >= 0.f;
+    const float qlx = wave_min16(live ? cx : FLT_MAX), qly = wave_min16(live ? cy : FLT_MAX),
+                qlz = wave_min16(live ? cz : FLT_MAX);
+    const float qux = wave_max16(live ? cx : -FLT_MAX), quy = wave_max16(live ? cy : -FLT_MAX),
+                quz = wave_max16(live ? cz : -FLT_MAX);
+    const float cap = wave_max16(capi);
+    for (int s0 = 0; s0 < n_super; s0 += 64) {
+      bool near = false;
+      if (s0 + lane < n_super) {
+        const CloudSuper Sv = supers[super_begin + s0 + lane];
+        near = box_box_d2(Sv.lb[0], Sv.lb[1], Sv.lb[2], Sv.ub[0], Sv.ub[1], Sv.ub[2], qlx, qly, qlz, qux, quy, quz) <=
+               cap;
+      }
+      uint64_t smask = __ballot(near);
+      while (smask) {
+        const int sj = __builtin_ctzll(smask);
+        smask &= smask - 1;
+        const CloudSuper Su = cload(supers, super_begin + s0 + sj);
+        // the super-tile's tiles, a lane each
+        bool tnear = false;
+        if (lane < Su.count) {
+          const CloudTile Tv = tiles[Su.first + lane];
+          tnear = box_box_d2(Tv.lb[0], Tv.lb[1], Tv.lb[2], Tv.ub[0], Tv.ub[1], Tv.ub[2], qlx, qly, qlz, qux, quy,
+                             quz) <= cap;
+        }
+        uint64_t tmask = __ballot(tnear);
+        while (tmask) {
+          const int tj = __builtin_ctzll(tmask);
+          tmask &= tmask - 1;
+          const int t = Su.first + tj;
+          const CloudTile T = cload(tiles, t);
+          const float dx = fmaxf(fmaxf(T.lb[0] - cx, cx - T.ub[0]), 0.f);
+          const float dy = fmaxf(fmaxf(T.lb[1] - cy, cy - T.ub[1]), 0.f);
+          const float dz = fmaxf(fmaxf(T.lb[2] - cz, cz - T.ub[2]), 0.f);
+          if (!__any(dx * dx + dy * dy + dz * dz <= capi)) continue;
+          const float a = k == 0 ? cx - T.o[0] : (k == 1 ? cy - T.o[1] : (k == 2 ? cz - T.o[2] : 1.f));
+          const float* Bt = tileB + (int64_t)t * 256;
+          floatx4 d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[lane], zero, 0, 0, 0);
+          floatx4 d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[64 + lane], zero, 0, 0, 0);
+          floatx4 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[128 + lane], zero, 0, 0, 0);
+          floatx4 d3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bt[192 + lane], zero, 0, 0, 0);
+          const float rho2 = kBounds ? tile_rho2(T) : 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float ox = rx[r] - T.o[0], oy = ry[r] - T.o[1], oz = rz[r] - T.o[2];
-          const float off = ox * ox + oy * oy + oz * oz;
-          const float m = fminf(fminf(d0[r], d1[r]), fminf(d2[r], d3[r])) + off;
-          if constexpr (kBounds) {
-            const float err = kCloudGemmErr * (off + rho2);
-            best[r] = fminf(best[r], m + err);
-            lo[r] = fminf(lo[r], m - err);
-          } else {
-            best[r] = fminf(best[r], m);
+          for (int r = 0; r < 4; ++r) {
+            const float ox = rx[r] - T.o[0], oy = ry[r] - T.o[1], oz = rz[r] - T.o[2];
+            const float off = ox * ox + oy * oy + oz * oz;
+            const float m = fminf(fminf(d0[r], d1[r]), fminf(d2[r], d3[r])) + off;
+            if constexpr (kBounds) {
+              const float err = kCloudGemmErr * (off + rho2);
+              best[r] = fminf(best[r], m + err);
+              lo[r] = fminf(lo[r], m - err);
+            } else {
+              best[r] = fminf(best[r], m);
+            }
           }
         }
       }
@@ -112,10 +168,10 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   if (i == 0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t q = q0 + 4 * k + r;
-      if (q < Q) {
-        out_d2[q] = best[r];
-        if constexpr (kBounds) out_lo[q] = lo[r];
+      const int64_t e = e0 + 4 * k + r;
+      if (e < S) {
+        out_d2[e * B + b] = best[r];
+        if constexpr (kBounds) out_lo[e * B + b] = lo[r];
       }
     }
   }
@@ -186,8 +242,9 @@ hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const 
   float* hi = const_cast<float*>(rr) + Q;
   float* lo = hi + Q;
   hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, flags, 0, B);
-  hipLaunchKernelGGL(cloud_min_kernel<true>, dim3(blocks((Q + 15) / 16 * 64, 256)), dim3(256), 0, st, tiles, tileB,
-                     supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, hi, lo);
+  const int64_t waves = (Q / B + 15) / 16 * B;  // 16 spheres of one configuration per wave
+  hipLaunchKernelGGL(cloud_min_kernel<true>, dim3(blocks(waves * 64, 256)), dim3(256), 0, st, tiles, tileB, supers,
+                     cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, hi, lo);
   hipLaunchKernelGGL(cloud_decide_kernel, dim3(blocks(Q, 256)), dim3(256), 0, st, lo, hi, rr, B, Q, band, flags);
   hipLaunchKernelGGL(cloud_merge_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, valid, flags, B);
   return hipGetLastError();
@@ -196,8 +253,9 @@ hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const 
 hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                             const CloudRange& cr, const float* cen, const float* cap2, int64_t B, int64_t Q,
                             float* out_d2, hipStream_t st) {
-  hipLaunchKernelGGL(cloud_min_kernel<false>, dim3((unsigned)(((Q + 15) / 16 * 64 + 255) / 256)), dim3(256), 0, st,
-                     tiles, tileB, supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2,
+  const int64_t waves = (Q / B + 15) / 16 * B;  // 16 spheres of one configuration per wave
+  hipLaunchKernelGGL(cloud_min_kernel<false>, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0, st, tiles,
+                     tileB, supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2,
                      (float*)nullptr);
   return hipGetLastError();
 }

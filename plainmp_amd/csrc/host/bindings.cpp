@@ -579,6 +579,28 @@ PYBIND11_MODULE(_native, m) {
              check(pmp_link_jacobian_batch(c.device_context((void*)stream), (const double*)q, ldq, B, link,
                                            point.empty() ? nullptr : point.data(), (float*)jac));
            })
+      .def("_dev_ik_step",
+           [](SphereCollisionCst& c, uintptr_t q, int64_t R, uintptr_t poses, int kind, int link, uintptr_t target,
+              uintptr_t jac0, const std::vector<uintptr_t>& jac_u, uintptr_t vals, uintptr_t cjac, int n_cst,
+              uintptr_t lo, uintptr_t hi, double damping, double max_step, double margin, double penalty,
+              uintptr_t stream) {
+             pmp_ik_step_desc d{};
+             d.kind = kind;
+             d.link = link;
+             d.target = (const double*)target;
+             d.jac0 = (const float*)jac0;
+             for (size_t k = 0; k < 3 && k < jac_u.size(); ++k) d.jac_u[k] = (const float*)jac_u[k];
+             d.vals = (const float*)vals;
+             d.cjac = (const float*)cjac;
+             d.n_cst = n_cst;
+             d.lo = (const double*)lo;
+             d.hi = (const double*)hi;
+             d.damping = damping;
+             d.max_step = max_step;
+             d.margin = margin;
+             d.penalty = penalty;
+             check(pmp_ik_step_batch(c.device_context((void*)stream), (double*)q, R, (const double*)poses, &d));
+           })
       .def("_dev_evaluate",
            [](SphereCollisionCst& c, uintptr_t q, int64_t ldq, int64_t B, uintptr_t vals, uintptr_t jac,
               uintptr_t stream) {
@@ -625,6 +647,20 @@ PYBIND11_MODULE(_native, m) {
         if (g.g) check(pmp_graph_destroy(g.g));
         g.g = nullptr;
       });
+  // trajectories of uneven length (cost, gradient and evaluate rows of every
+  // waypoint) captured once and replayed by launch()
+  mo.def("_dev_capture_traj_graph",
+         [](std::shared_ptr<SphereCollisionCst> c, uintptr_t X, int64_t ldx, int64_t N,
+            const std::vector<int64_t>& offsets, const std::vector<double>& weights, uintptr_t vals, uintptr_t jac,
+            uintptr_t cost, uintptr_t grad, uintptr_t stream) {
+           pmp_context* ctx = c->device_context((void*)stream);
+           auto g = std::make_shared<Graph>();
+           g->cst = c;
+           check(pmp_traj_cost_grad_graph(ctx, (const double*)X, ldx, N, offsets.data(), (int32_t)offsets.size() - 1,
+                                          weights.empty() ? nullptr : weights.data(), (float*)vals, (float*)jac,
+                                          (float*)cost, (float*)grad, &g->g));
+           return g;
+         });
   py::class_<MotionValidator, std::shared_ptr<MotionValidator>>(mo, "MotionValidator")
       .def(py::init<std::shared_ptr<SphereCollisionCst>, const ValidatorConfig&>())
       .def("checkMotion", [](MotionValidator& v, const DArr& s1, const DArr& s2) {

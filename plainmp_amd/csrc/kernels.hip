@@ -20,9 +20,6 @@ constexpr int64_t kWholeUnitChunks = 512;
 // Edges are walked 64 states at a time; an edge needing more states than this
 // (a step ratio below 2^-24) is reported invalid with n_checked = -1 instead.
 constexpr double kMaxEdgeStates = 16777216.0;
-// XCDs of an MI355X; workgroups of a launch are handed to them in turn.  Only
-// the locality of the edge pass's index mapping depends on it.
-constexpr int kXcds = 8;
 
 template <class T>
 __device__ __forceinline__ DevPrim<T> prim_at(const SceneArgs<T>& sc, int p) {

@@ -19,6 +19,9 @@ constexpr int kMaxSelfPairs = 1024;
 constexpr int kMaxClouds = 4;
 constexpr int kCloudTile = 64;  // points per cloud tile
 constexpr int kBlockSpheres = 8;  // spheres of one link tested together against a primitive
+// XCDs of an MI355X; workgroups of a launch are handed to them in turn.  Only
+// the locality of index mappings depends on it.
+constexpr int kXcds = 8;
 
 // Primitive kinds of the validity pass; the scene keeps a copy of its
 // primitives sorted by kind so that each kind is a loop without branches.

@@ -5,10 +5,12 @@ The reference's ``solve_ik`` runs SLSQP from one random seed at a time, up to
 is at most ``acceptable_error`` and which the inequality constraint reports
 valid; its expensive inner calls are the pose and collision Jacobians.  Here
 every restart runs at once: each iteration makes one link-pose launch (FP64
-FK), one link-Jacobian launch (four with a rotation target) and one evaluate
-launch (collision values and rows) for all restarts, and takes a damped
-Gauss-Newton step per restart on the pose residual with the collision values
-below ``margin`` as penalty residuals, projected onto the joint bounds.  The
+FK), one link-Jacobian launch (four with a rotation target), one evaluate
+launch (collision values and rows) and one step launch for all restarts
+(pmp_ik_step_batch: a damped Gauss-Newton step per restart on the pose
+residual with the collision values below ``margin`` as penalty residuals,
+its normal equations solved in registers, projected onto the joint bounds;
+robots with more than 16 q columns take the same step in torch).  The
 answer is the first restart, in restart order, that passes the reference's
 acceptance test (squared pose residual <= ``acceptable_error``,
 ``is_valid``), so a result is reproducible from the seed.
@@ -156,7 +158,24 @@ def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[
             e = torch.cat([e, quat - tgt[3:]], dim=1)
         return e, quat, rpy
 
-    for _ in range(cfg.n_iter):
+    if D <= 16:
+        # fused: per iteration the pose, Jacobian and evaluate launches and one
+        # step kernel (the normal equations solved per restart in registers)
+        qs = Q.T.contiguous()
+        pen = float(cfg.penalty)
+        C_used = C if use_col else 0
+        for _ in range(cfg.n_iter):
+            cst._dev_link_poses(qs.data_ptr(), R, R, 1, poses.data_ptr(), sh)
+            cst._dev_link_jacobian(qs.data_ptr(), R, R, link, [], J.data_ptr(), sh)
+            for k in range(len(Ju)):
+                cst._dev_link_jacobian(qs.data_ptr(), R, R, link, units[k], Ju[k].data_ptr(), sh)
+            if use_col:
+                cst._dev_evaluate(qs.data_ptr(), R, R, vals.data_ptr(), cjac.data_ptr(), sh)
+            cst._dev_ik_step(qs.data_ptr(), R, poses.data_ptr(), kind, link, tgt.data_ptr(), J.data_ptr(),
+                             [u.data_ptr() for u in Ju], vals.data_ptr(), cjac.data_ptr(), C_used, lo.data_ptr(),
+                             hi.data_ptr(), cfg.damping, cfg.max_step, cfg.margin, pen, sh)
+        Q = qs.T.contiguous()
+    for _ in range(cfg.n_iter if D > 16 else 0):
         qs = Q.T.contiguous()
         e, quat, rpy = pose_error(qs)
         cst._dev_link_jacobian(qs.data_ptr(), R, R, link, [], J.data_ptr(), sh)

@@ -134,11 +134,24 @@ def _counter_rows(path):
 
 
 def _kernel_base(name: str) -> str:
-    import re
-
-    n = re.sub(r"\(.*", "", name)  # drop the argument list
-    n = re.sub(r"^void ", "", n).strip()
-    return n.split("::")[-1] if "<" not in n else n.split("::", 1)[-1]
+    """A kernel's name without return type, namespaces and argument list
+    (template arguments kept): 'void pmp::edges_kernel<false>(...)' ->
+    'edges_kernel<false>'."""
+    n = name.replace("(anonymous namespace)::", "")
+    if n.startswith("void "):
+        n = n[5:]
+    depth, cut, last_sep = 0, len(n), 0
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+        elif ch == ":" and depth == 0 and n[i + 1:i + 2] == ":":
+            last_sep = i + 2
+    return n[last_sep:cut]
 
 
 def traffic_table(fetch_path, write_path, calib_mb: int) -> dict:

@@ -59,3 +59,28 @@ def oracle_min_margin(ocst, Q):
     to flipping."""
     d = ocst.sphere_clearance_batch(Q, False, 8)
     return np.abs(d).min(axis=1) if d.size else np.full(len(Q), np.inf)
+
+
+def rows_agree(oc, X, got, ref, sel, row, tol=1e-4, h=1e-6):
+    """Evaluate rows of the selected waypoints against the reference's rows
+    (relative error <= tol, scale max(|ref|, 1)).  The reference takes the SDF
+    gradient by a forward difference of step 1e-6 (primitive_sphere_collision.cpp:308-315):
+    where that step straddles a kink of the SDF (inside a box or cylinder,
+    near its medial surface) the reference row is not the derivative.  A row
+    that differs from the reference's is accepted only if it matches the FP64
+    central difference of the oracle's value instead.  Returns (rows checked,
+    rows settled by the central difference)."""
+    idx = np.flatnonzero(sel)
+    g, r = got[idx, row], ref[idx, row]
+    bad = ~(np.abs(g - r) <= tol * np.maximum(np.abs(r), 1.0)).all(axis=1)
+    if bad.any():
+        Xb = X[idx[bad]]
+        D = X.shape[1]
+        fd = np.empty((len(Xb), D))
+        for j in range(D):
+            e = np.zeros(D)
+            e[j] = h
+            fd[:, j] = (oc.evaluate_batch(Xb + e, 8)[0][:, row] - oc.evaluate_batch(Xb - e, 8)[0][:, row]) / (2 * h)
+        ok = (np.abs(g[bad] - fd) <= tol * np.maximum(np.abs(fd), 1.0)).all(axis=1)
+        assert ok.all(), f"{int((~ok).sum())} rows differ from both the reference row and the central difference"
+    return len(idx), int(bad.sum())

@@ -128,4 +128,42 @@ def test_calls_on_alternating_streams(name):
     torch.cuda.synchronize()
     for v, ref in zip(outs, refs):
         assert np.array_equal(v.cpu().numpy().astype(bool), ref)
-    _report("alternating_streams", name=name, calls=len(batches))
+    _report("alternating_streams", workload=name, calls=len(batches))
+
+
+def test_traj_graph_replays_match_direct_call():
+    """Trajectories of uneven length captured into a graph: every replay
+    (outputs overwritten in between) gives the direct call's rows, costs and
+    gradients, and the rows equal the oracle's evaluate."""
+    import torch
+
+    from plainmp_amd.constraint import capture_traj_graph
+
+    wl = synth.workload("traj_dual_arm", scale=0.02)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    off, X = [int(o) for o in wl.traj_offsets], wl.traj_X
+    N, D, C, T = len(X), pc.q_dim(), pc.cst_dim(), len(off) - 1
+    w = list(np.linspace(0.5, 2.0, D))
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    with torch.cuda.stream(stream):
+        x = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+        outs = [torch.empty(s, dtype=torch.float32, device="cuda") for s in ((C, N), (C, D, N), (T,), (D, N))]
+    stream.synchronize()
+    ptrs = [o.data_ptr() for o in outs]
+    pc._dev_traj_cost_grad(x.data_ptr(), N, N, off, w, *ptrs, sh)
+    stream.synchronize()
+    direct = [o.cpu().numpy().copy() for o in outs]
+    g = capture_traj_graph(pc, x.data_ptr(), N, N, off, w, *ptrs, sh)
+    for rep in range(3):
+        with torch.cuda.stream(stream):
+            for o in outs:
+                o.fill_(float("nan"))
+        g.launch()
+        stream.synchronize()
+        for o, d in zip(outs, direct):
+            assert np.array_equal(o.cpu().numpy(), d), rep
+    g.destroy()
+    vref, _ = oc.evaluate_batch(X, 8)
+    assert np.abs(direct[0].T - vref).max() <= 5e-6
+    _report("traj_graph", N=N, T=T)

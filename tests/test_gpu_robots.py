@@ -18,7 +18,7 @@ import pytest
 
 from oracle import _oracle
 from plainmp_amd import synth
-from support import oracle_cst, product_cst
+from support import oracle_cst, product_cst, rows_agree
 
 pytestmark = pytest.mark.gpu
 
@@ -61,16 +61,12 @@ def _check_evaluate(wl, oc, pc, Q, rtol=1e-4):
     checked = 0
     if len(wl.prims) or oc.n_prims():
         clear = (_second_gap(oc.pair_clearance_batch(Q, True, 8)) > 1e-5) & (vref[:, 0] < 0.1)
-        scale = np.maximum(np.abs(jref[clear, 0]), 1.0)
-        assert (np.abs(jgot[clear, 0] - jref[clear, 0]) <= rtol * scale).all()
-        checked += int(clear.sum())
+        checked += rows_agree(oc, Q, jgot, jref, clear, 0, tol=rtol)[0]
         row = 1
     if wl.robot.self_pairs:
         sd = _self_distances(wl.robot, oc, Q)
         clear = (_second_gap(sd) > 1e-5) & (vref[:, row] < 0.1)
-        scale = np.maximum(np.abs(jref[clear, row]), 1.0)
-        assert (np.abs(jgot[clear, row] - jref[clear, row]) <= rtol * scale).all()
-        checked += int(clear.sum())
+        checked += rows_agree(oc, Q, jgot, jref, clear, row, tol=rtol)[0]
     return verr, checked
 
 
@@ -112,7 +108,7 @@ def test_humanoid31_floating_base():
     Q = np.concatenate([synth.uniform_configs(robot, B, seed=33), rng.normal(0, 0.1, (B, 6))], axis=1)
     prims = synth.mixed_primitives(24, seed=10)
     for p in prims:  # around the humanoid, whose pelvis is at the origin
-        p["position"] = np.asarray(p["position"], float) - np.array([0.5, 0.0, 0.6])
+        p["position"] = np.asarray(p["position"], float) - np.array([0.0, 0.0, 0.6])
     wl = synth.Workload("humanoid31", robot, prims, Q=Q)
     oc = oracle_cst(wl, base_type=_oracle.BaseType.FLOATING)
     pc = product_cst(wl, base_type=BaseType.FLOATING)
