@@ -308,6 +308,8 @@ def _extra(torch, stream, dev):
         q = _soa(torch, wl.Q)
         dist = torch.empty((S, P, B), dtype=torch.float32, device=dev)
         grad = torch.empty((S, P, D, B), dtype=torch.float32, device=dev)
+    _, tune_pair = autotune.tune_pairs(cst, q.data_ptr(), B, dist.data_ptr(), grad.data_ptr(), stream,
+                                       key="grad_mixed32", cache=None)
     ms = _time_stream(torch, stream,
                       lambda: cst._dev_pair_clearance(q.data_ptr(), B, B, dist.data_ptr(), grad.data_ptr(), sh))
     bytes_out = (dist.numel() + grad.numel()) * 4 + q.numel() * 8
@@ -315,6 +317,7 @@ def _extra(torch, stream, dev):
         "configs": B, "ms": ms, "jacobians_per_s": B / (ms * 1e-3),
         "hbm_gbs_algorithmic": bytes_out / (ms * 1e-3) / 1e9,
         "frac_of_hbm_peak": bytes_out / (ms * 1e-3) / 1e9 / roofline.PEAK_HBM_GBS,
+        "autotune": tune_pair,
     }
     rl = {}
     rl["pair_kernel"] = {"workload": "grad_mixed32 pair distances + d/dq, 8192 configs", "bound": "hbm",

@@ -135,6 +135,10 @@ typedef struct pmp_schedule {
                                  64-configuration chunks walk each configuration as one
                                  unit, smaller ones split it per sphere block and self
                                  pair; 0 = default (512)                               */
+  int32_t pair_shares;      /* pair distances: the primitives are split into this many
+                               shares, each walked by its own waves (more stores in
+                               flight for a small batch); 0 = default (about four
+                               waves per SIMD), clamped to half the primitive count */
 } pmp_schedule;
 
 typedef struct pmp_context pmp_context;

@@ -487,7 +487,7 @@ void record_call(pmp_context* ctx) noexcept {
 extern "C" {
 
 const char* pmp_last_error(void) { return g_last_error.c_str(); }
-int32_t pmp_abi_version(void) { return 1; }
+int32_t pmp_abi_version(void) { return 2; }
 
 pmp_status pmp_create(int32_t device, pmp_context** out) {
   return guarded([&] {
@@ -569,6 +569,7 @@ pmp_status pmp_set_schedule(pmp_context* ctx, const pmp_schedule* s) {
     PMP_REQUIRE(s->block == 0 || (s->block % 64 == 0 && s->block <= 256), "block must be a multiple of 64, <= 256");
     PMP_REQUIRE(s->waves_per_cu >= 0 && s->waves_per_cu <= 32, "waves_per_cu must be in [0, 32]");
     PMP_REQUIRE(s->eval_whole_chunks >= 0, "eval_whole_chunks must be >= 0");
+    PMP_REQUIRE(s->pair_shares >= 0 && s->pair_shares <= 256, "pair_shares must be in [0, 256]");
     ctx->sched = *s;
   });
 }
@@ -1365,7 +1366,7 @@ pmp_status pmp_pair_clearance_batch(pmp_context* ctx, const double* q, int64_t l
     PMP_REQUIRE(B == 0 || dist, "null output");
     if (B == 0 || ctx->n_spheres == 0 || ctx->s32.n_prims == 0) return;
     set_device(ctx);
-    PMP_HIP(launch_pairs(ctx->k32, ctx->s32, q, ldq, B, dist, grad, st(ctx)));
+    PMP_HIP(launch_pairs(ctx->k32, ctx->s32, q, ldq, B, dist, grad, ctx->sched.pair_shares, st(ctx)));
   });
 }
 

@@ -610,10 +610,13 @@ PYBIND11_MODULE(_native, m) {
       .def("set_band",
            [](SphereCollisionCst& c, double band) { check(pmp_set_band(c.device_context(nullptr), band)); },
            "Width (m) of the band around zero clearance whose FP32 decisions are re-made in FP64 (default 1e-5).")
-      .def("_dev_set_schedule", [](SphereCollisionCst& c, int block, int waves_per_cu, int whole_chunks) {
-        pmp_schedule s{block, waves_per_cu, whole_chunks};
-        check(pmp_set_schedule(c.device_context(nullptr), &s));
-      });
+      .def(
+          "_dev_set_schedule",
+          [](SphereCollisionCst& c, int block, int waves_per_cu, int whole_chunks, int pair_shares) {
+            pmp_schedule s{block, waves_per_cu, whole_chunks, pair_shares};
+            check(pmp_set_schedule(c.device_context(nullptr), &s));
+          },
+          py::arg("block"), py::arg("waves_per_cu"), py::arg("whole_chunks"), py::arg("pair_shares") = 0);
 
   // -------------------------------------------------------------------- ompl
   auto mo = m.def_submodule("ompl");

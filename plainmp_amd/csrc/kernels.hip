@@ -36,6 +36,12 @@ __device__ __forceinline__ float box_dist2(const float (&lb)[3], const float (&u
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Maximum of v over the wave's active lanes (wave-uniform result).
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int m = 32; m > 0; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Four consecutive floats of a table read as one wave-uniform load.
 struct alignas(16) Quad4 {
   float v[4];
@@ -237,7 +243,7 @@ __global__ void __launch_bounds__(256) clearance_kernel(KinArgs<float> ka, Scene
 // FP64 re-decision ("uncertain").
 // kInlineCloud: clouds are tested inside the walk (the edge pass, whose
 // states have no batch-wide cloud pass of their own); otherwise the walk
-// writes cloud queries for the matrix-core pass when cw is set.
+// writes cloud queries for the matrix-core pass when use_cw is set.
 template <bool kInlineCloud>
 struct ValidVis32T {
   static constexpr bool kSelf = true;
@@ -245,7 +251,8 @@ struct ValidVis32T {
   const KinArgs<float>& ka;
   const SceneArgs<float>& sc;
   float band;
-  const CloudWork* cw = nullptr;  // cloud queries are written here when the scene holds clouds
+  CloudWork cw{};        // cloud queries are written here when use_cw (the scene holds clouds)
+  bool use_cw = false;
   int64_t b = 0;
   bool idle = false;
   bool definite = false;
@@ -331,15 +338,15 @@ struct ValidVis32T {
       }
       const V3<float> g = add(mv(R, V3<float>{bk.g[0], bk.g[1], bk.g[2]}), P.t);
       if (bad && n_solid > 0) definite = true;  // NaN centres collide with every primitive
-      if (cw && !idle) {
-        const int64_t B = cw->B;
+      if (use_cw && !idle) {
+        const int64_t B = cw.B;
 #pragma unroll
         for (int k = 0; k < kBlockSpheres; ++k) {
           if (k < n) {
             const int64_t e = bk.ext[k];
-            cw->cen[(e * 3 + 0) * B + b] = c[k].x;
-            cw->cen[(e * 3 + 1) * B + b] = c[k].y;
-            cw->cen[(e * 3 + 2) * B + b] = c[k].z;
+            cw.cen[(e * 3 + 0) * B + b] = c[k].x;
+            cw.cen[(e * 3 + 1) * B + b] = c[k].y;
+            cw.cen[(e * 3 + 2) * B + b] = c[k].z;
           }
         }
         // clouds: queried by the matrix-core pass unless the AABB pre-test culls it
@@ -347,6 +354,18 @@ struct ValidVis32T {
         for (int p = sc.vrange[kKindCloud]; p < sc.vrange[kKindCloud + 1]; ++p) {
           const DevPrim<float> Pc = cload(vp, p);
           const int oc = (Pc.flags >> 4) & 15;
+          // the cloud's output arrays, chosen with constant indices so that
+          // the arguments stay in registers (an indexed kernel argument
+          // would be copied to scratch)
+          float* cap2 = cw.cap2[0];
+          float* rro = cw.rr[0];
+#pragma unroll
+          for (int i = 1; i < kMaxClouds; ++i) {
+            if (oc == i) {
+              cap2 = cw.cap2[i];
+              rro = cw.rr[i];
+            }
+          }
 #pragma unroll
           for (int k = 0; k < kBlockSpheres; ++k) {
             if (k < n) {
@@ -354,8 +373,8 @@ struct ValidVis32T {
               const bool qd = alive() && !outside_aabb(Pc, c[k], r);
               const float Rr = r + Pc.p0;
               const int64_t at = (int64_t)bk.ext[k] * B + b;
-              cw->cap2[oc][at] = qd ? (Rr + band) * (Rr + band) : -1.0f;
-              cw->rr[oc][at] = qd ? Rr : -1.0f;
+              cap2[at] = qd ? (Rr + band) * (Rr + band) : -1.0f;
+              rro[at] = qd ? Rr : -1.0f;
             }
           }
         }
@@ -598,7 +617,8 @@ __global__ void __launch_bounds__(256) valid_kernel(KinArgs<float> k32, SceneArg
   BatchQ qs{q, ldq, active ? b : B - 1};
   const bool taylor = k32.base_type == PMP_BASE_FIXED;  // constraints/primitive.hpp:59-61
   ValidVis32 v{k32, s32, band};
-  v.cw = s32.n_cloud > 0 && active ? &cw : nullptr;
+  v.cw = cw;
+  v.use_cw = s32.n_cloud > 0 && active;
   v.b = b;
   v.idle = !active;
   run_fk(k32, qs, taylor, v);
@@ -1180,26 +1200,43 @@ struct EvalUnitVis {
   }
 };
 
-// Full FK walk that keeps the joint frames and the world centres of up to
-// three spheres (the winners whose Jacobian rows are written).
+// FK walk for the Jacobian rows of the winners: it keeps the joint frames
+// and computes the world centres of just the (up to three) winning spheres,
+// and stops at the last node any lane of the wave needs.
 template <int MAXC>
 struct EvalRowVis {
   static constexpr bool kSelf = false;
+  static constexpr bool kRanged = true;
+  static constexpr bool kBlocks = true;
   const KinArgs<float>& ka;
   Frames<MAXC> fr;
   int s0 = -1, s1 = -1, s2 = -1;
+  int last = 0;  // wave-uniform
   V3<float> c0{0.f, 0.f, 0.f}, c1{0.f, 0.f, 0.f}, c2{0.f, 0.f, 0.f};
   __device__ bool alive() const { return true; }
+  __device__ int last_node() const { return last; }
+  __device__ int pair_lo() const { return 0; }
+  __device__ int pair_hi() const { return 0; }
   template <class QS>
   __device__ void begin(const Pose<float>& base, const QS& qs) { fr.set_base(ka, base, qs); }
   __device__ void node(int, const DevNode<float>& nd, const Pose<float>& P) {
     if (nd.q_col < 0) return;
     fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
   }
-  __device__ void sphere(int s, V3<float> c, float) {
-    if (s == s0) c0 = c;
-    if (s == s1) c1 = c;
-    if (s == s2) c2 = c;
+  __device__ void sphere(int, V3<float>, float) {}
+  __device__ void blocks(const DevNode<float>& nd, const Pose<float>& P) {
+    const bool h0 = s0 >= nd.sphere_begin && s0 < nd.sphere_end;
+    const bool h1 = s1 >= nd.sphere_begin && s1 < nd.sphere_end;
+    const bool h2 = s2 >= nd.sphere_begin && s2 < nd.sphere_end;
+    if (!wave_any(h0 || h1 || h2)) return;
+    const M3<float> R = qmat(P.q);
+    auto centre = [&](int s) {
+      const DevSphere<float> sp = ka.spheres[s];  // per-lane index: a vector load
+      return add(mv(R, V3<float>{sp.x, sp.y, sp.z}), P.t);
+    };
+    if (h0) c0 = centre(s0);
+    if (h1) c1 = centre(s1);
+    if (h2) c2 = centre(s2);
   }
   __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
 };
@@ -1290,12 +1327,23 @@ __device__ __forceinline__ void eval_finish(const KinArgs<float>& ka, const Scen
     }
   }
   EvalRowVis<MAXC> rv{ka};
-  if (bidx >= 0) rv.s0 = bidx / kMaxPrims;
+  int need = -1;  // the last node this lane's winners need
+  if (bidx >= 0) {
+    rv.s0 = bidx / kMaxPrims;
+    need = cload(ka.meta, rv.s0).node;
+  }
   if (sidx >= 0) {
     rv.s1 = sidx / kMaxSpheres;
     rv.s2 = sidx % kMaxSpheres;
+    need = max(need, max(cload(ka.meta, rv.s1).node, cload(ka.meta, rv.s2).node));
   }
-  run_fk(ka, qs, false, rv);
+  // lanes of the wave that reach this point walk together (inactive lanes
+  // have returned); no winner in the wave: the rows are zero without FK
+  const int last = wave_max_i(need);
+  if (last >= 0) {
+    rv.last = last;
+    run_fk(ka, qs, false, rv);
+  }
   int row = 0;
   if (NP > 0) {
     o.vals[b] = best;
@@ -1407,6 +1455,7 @@ struct PairVis {
   int64_t B, b;
   bool active;
   DevSphereItem it;
+  int p_lo, p_hi;  // primitives of this lane's share
   bool done = false;
   __device__ bool alive() const { return !done; }
   template <class QS>
@@ -1424,7 +1473,7 @@ struct PairVis {
     V3<float> J[MAXC];
 #pragma unroll
     for (int j = 0; j < MAXC; ++j) J[j] = fr.col(j, mask, ka.rev_mask, c);
-    for (int p = 0; p < NP; ++p) {
+    for (int p = p_lo; p < p_hi; ++p) {
       const DevPrim<float> P = prim_at(sc, p);
       float d;
       V3<float> g{0.f, 0.f, 0.f};
@@ -1439,12 +1488,14 @@ struct PairVis {
         d = sdf_eval(P, c) - r;
         if (grad) g = sdf_grad(P, c);
       }
+      // the outputs are written once and not read back here: streaming
+      // (non-temporal) stores keep them from displacing the tables in L2
       const int64_t row = (int64_t)m.out * NP + p;
-      if (active) dist[row * B + b] = d;
+      if (active) __builtin_nontemporal_store(d, dist + row * B + b);
       if (grad) {
 #pragma unroll
         for (int j = 0; j < MAXC; ++j)
-          if (j < D && active) grad[(row * D + j) * B + b] = dot(g, J[j]);
+          if (j < D && active) __builtin_nontemporal_store(dot(g, J[j]), grad + (row * D + j) * B + b);
       }
     }
   }
@@ -1457,7 +1508,12 @@ __global__ void __launch_bounds__(256) pair_kernel(KinArgs<float> ka, SceneArgs<
   const int64_t b = gid();
   const bool active = b < B;
   BatchQ qs{q, ldq, active ? b : B - 1};
-  PairVis<MAXC, kCloud> v{ka, sc, {}, dist, grad, B, b, active, cload(ka.items, blockIdx.y)};
+  // grid z splits the primitives (p_per each), so that a batch of a few
+  // thousand configurations still puts several waves on every SIMD: the
+  // launch is bound by its stores, and more waves keep more of them in flight
+  const int p_per = (sc.n_prims + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int p_lo = (int)blockIdx.z * p_per, p_hi = min(sc.n_prims, p_lo + p_per);
+  PairVis<MAXC, kCloud> v{ka, sc, {}, dist, grad, B, b, active, cload(ka.items, blockIdx.y), p_lo, p_hi};
   run_fk(ka, qs, false, v);
 }
 
@@ -1898,17 +1954,23 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
   }
   return hipGetLastError();
 }
+// Waves the pair launch aims for: four per SIMD of a 256-CU MI355X.
+constexpr int64_t kPairWaves = 4096;
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
-                        int64_t B, float* dist, float* grad, hipStream_t st) {
-  if (ka.n_items == 0) return hipSuccess;
+                        int64_t B, float* dist, float* grad, int shares, hipStream_t st) {
+  if (ka.n_items == 0 || sc.n_prims == 0) return hipSuccess;
   const int bk = slot_block(ka, 256);
   const size_t lds = slot_bytes(ka, bk);
+  // primitive shares: the schedule's, or enough for about kPairWaves waves;
+  // at least two primitives a share
+  const int64_t waves = (B + 63) / 64 * ka.n_items;
+  const int64_t want = shares > 0 ? shares : (kPairWaves + waves - 1) / waves, most = (sc.n_prims + 1) / 2;
+  const int nz = (int)(want < most ? want : most) < 1 ? 1 : (int)(want < most ? want : most);
+  const dim3 grid(blocks_for(B, bk), (unsigned)ka.n_items, (unsigned)nz);
   if (sc.n_cloud > 0)
-    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, true, dim3(blocks_for(B, bk), (unsigned)ka.n_items), dim3(bk),
-                            lds, st, ka, sc, q, ldq, B, dist, grad);
+    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, true, grid, dim3(bk), lds, st, ka, sc, q, ldq, B, dist, grad);
   else
-    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, false, dim3(blocks_for(B, bk), (unsigned)ka.n_items), dim3(bk),
-                            lds, st, ka, sc, q, ldq, B, dist, grad);
+    PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, false, grid, dim3(bk), lds, st, ka, sc, q, ldq, B, dist, grad);
   return hipGetLastError();
 }
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,

@@ -117,8 +117,9 @@ struct EvalArgs {
 };
 hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                              int64_t B, const EvalArgs& e, hipStream_t st);
+// shares: primitive shares (grid z), 0 = about four waves per SIMD
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
-                        int64_t B, float* dist, float* grad, hipStream_t st);
+                        int64_t B, float* dist, float* grad, int shares, hipStream_t st);
 // part: workspace of n_items * n_cols * B floats
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
                       const float* gd, float* part, float* grad_q, hipStream_t st);

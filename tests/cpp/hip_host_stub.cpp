@@ -166,7 +166,7 @@ hipError_t launch_nearest(const double*, int64_t, const double*, int64_t, double
   return hipSuccess;
 }
 hipError_t launch_pairs(const KinArgs<float>&, const SceneArgs<float>&, const double*, int64_t, int64_t, float*,
-                        float*, hipStream_t) {
+                        float*, int, hipStream_t) {
   return hipSuccess;
 }
 hipError_t launch_prims_eval(const DevPrim<double>*, int, const double*, const double*, int64_t, int, double,

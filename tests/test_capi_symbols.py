@@ -21,13 +21,13 @@ def test_library_exports_header_symbols():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     lib.pmp_abi_version.restype = ctypes.c_int32
-    assert lib.pmp_abi_version() == 1
+    assert lib.pmp_abi_version() == 2
 
 
 def test_extension_matches_library():
     import plainmp_amd
 
-    assert plainmp_amd._native.abi_version() == 1
+    assert plainmp_amd._native.abi_version() == 2
 
 
 def test_cloud_tile_sizes_and_argument_errors():

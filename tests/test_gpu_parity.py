@@ -758,6 +758,39 @@ def test_autotuned_schedules_keep_results():
     assert np.array_equal(got, ref.astype(bool)) and np.array_equal(n_got, n_ref)
 
 
+def test_pair_shares_keep_results():
+    """The pair-distance launch gives the same distances and gradients,
+    bit for bit, under every primitive split the autotuner tries, and
+    matches the oracle's distances."""
+    import torch
+
+    from plainmp_amd import autotune
+
+    wl = synth.workload("grad_mixed32", scale=1 / 16)  # 512 configurations, 32 primitives
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    B, S, P, D = len(wl.Q), pc.n_spheres(), pc.n_prims(), pc.q_dim()
+    q = torch.from_numpy(np.ascontiguousarray(wl.Q.T)).cuda()
+    sh = torch.cuda.current_stream().cuda_stream
+    outs = []
+    for c in (0,) + autotune.PAIR_SHARES:
+        dist = torch.full((S, P, B), float("nan"), dtype=torch.float32, device="cuda")
+        grad = torch.full((S, P, D, B), float("nan"), dtype=torch.float32, device="cuda")
+        autotune.Schedule(pair_shares=c).apply(pc)
+        pc._dev_pair_clearance(q.data_ptr(), B, B, dist.data_ptr(), grad.data_ptr(), sh)
+        torch.cuda.synchronize()
+        outs.append((c, dist.cpu().numpy(), grad.cpu().numpy()))
+    for c, d, g in outs[1:]:
+        assert np.array_equal(d, outs[0][1]) and np.array_equal(g, outs[0][2]), c
+    dref = oc.pair_clearance_batch(wl.Q, True, 8)
+    got = np.moveaxis(outs[0][1], -1, 0)
+    assert np.abs(got - dref).max() <= 5e-6
+    stream = torch.cuda.current_stream()
+    sched, rep = autotune.tune_pairs(pc, q.data_ptr(), B, dist.data_ptr(), grad.data_ptr(), stream, repeats=2,
+                                     cache=None)
+    _report("pair_shares", **{k: v for k, v in rep.items() if k != "all"})
+    assert sched.pair_shares in autotune.PAIR_SHARES
+
+
 def _oracle_link_position(wl, link_name, Q, point=(0.0, 0.0, 0.0)):
     from oracle import urdf_parse
 
