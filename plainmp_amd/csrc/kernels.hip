@@ -277,7 +277,14 @@ struct ValidVis32T {
   // centre lies in that grown box: every member sphere is then outside the
   // AABB inflated by its radius plus the band, which is the reference's own
   // AABB pre-test failing with room to spare, so the skipped pairs can
-  // neither collide nor fall in the band.
+  // neither collide nor fall in the band.  A candidate that survives is then
+  // tested as a sphere of radius `reach` (the block's bounding radius times
+  // the growth bound, plus band and a 1e-4 m slack) at the bounding centre,
+  // against the primitive's own distance: the distance is 1-Lipschitz, so
+  // a lane farther than that from the primitive has every member sphere
+  // farther than its radius plus the band.  The ground's test is not a pure
+  // distance (the reference's AABB slab is part of it) and skips this step.
+  float reach = 0.0f;
   template <int K>
   __device__ __forceinline__ void kind_loop(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
                                             V3<float> g, const CullRange& cr, int bi) {
@@ -288,6 +295,10 @@ struct ValidVis32T {
                        g.z > ce.ub[2];
       if (!wave_any(alive() && !out)) continue;
       const DevPrim<float> P = cload(vp, ce.prim);
+      if constexpr (K != kKindGround) {
+        bool nb = false;
+        if (!wave_any(alive() && !out && collide_kind<K>(P, g, reach, 0.0f, nb))) continue;
+      }
       bool ib = false;
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
@@ -322,6 +333,8 @@ struct ValidVis32T {
       n_inb = 2;
     }
     const int n_solid = sc.vrange[kKindCloud];
+    // |R v| <= growf |v| for a lane whose |q|^2 is in [n2_lo, n2_hi]
+    const float growf = 1.0f + 2.0f * fmaxf(0.0f, fmaxf(nd.n2_hi - 1.0f, 1.0f - nd.n2_lo)) + 1e-6f;
     for (int bi = lo; bi < hi; ++bi) {
       if (!wave_any(alive())) return;
       const DevSphereBlock bk = cload(ka.blocks, bi);
@@ -381,6 +394,7 @@ struct ValidVis32T {
       }
       if (!wave_any(alive())) continue;
       const CullRange cr = cload(sc.cull_range, bi);
+      reach = bk.G * growf + band + 1e-4f;
       kind_loop<kKindBoxAA>(c, bk, n, g, cr, bi);
       kind_loop<kKindBoxRot>(c, bk, n, g, cr, bi);
       kind_loop<kKindCylZ>(c, bk, n, g, cr, bi);
@@ -1073,6 +1087,7 @@ struct EvalUnitVis {
   DevEvalUnit u{};
   float best = kCutoff, sbest = kCutoff;
   int bkey = 0x7fffffff, bidx = -1, sidx = -1;
+  float ereach = 0.0f;  // the current block's bounding radius, grown, plus a slack
   __device__ bool alive() const { return true; }
   __device__ int last_node() const { return u.last; }
   __device__ int pair_lo() const { return u.k_lo; }
@@ -1106,6 +1121,10 @@ struct EvalUnitVis {
         orig = ce.orig;
       }
       const DevPrim<float> P = cload(vp, p);
+      // every member sphere's value is at least the bounding centre's
+      // distance minus the block's reach (the SDFs are 1-Lipschitz): a wave
+      // whose lanes all have their best below that bound skips the candidate
+      if (!all && !wave_any(!(sdf_eval(P, g) - ereach > best))) continue;
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         if (k < n) {
@@ -1130,8 +1149,10 @@ struct EvalUnitVis {
     // outside the norm bounds the cull table assumed: every primitive is a candidate
     const float n2 = P.q.x * P.q.x + P.q.y * P.q.y + P.q.z * P.q.z + P.q.w * P.q.w;
     const bool all = wave_any(!(n2 >= nd.n2_lo && n2 <= nd.n2_hi));
+    const float growf = 1.0f + 2.0f * fmaxf(0.0f, fmaxf(nd.n2_hi - 1.0f, 1.0f - nd.n2_lo)) + 1e-6f;
     for (int bi = lo; bi < hi; ++bi) {
       const DevSphereBlock bk = cload(ka.blocks, bi);
+      ereach = bk.G * growf + 1e-4f;
       const int n = bk.n;
       V3<float> c[kBlockSpheres];
       DevSphereMeta m[kBlockSpheres];
