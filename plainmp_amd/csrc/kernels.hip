@@ -36,6 +36,13 @@ __device__ __forceinline__ float box_dist2(const float (&lb)[3], const float (&u
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Minimum of v over the wave (every lane active; wave-uniform result).
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) v = min(v, __shfl_xor(v, s, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Maximum of v over the wave's active lanes (wave-uniform result).
 __device__ __forceinline__ int wave_max_i(int v) {
   for (int m = 32; m > 0; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
@@ -244,7 +251,10 @@ __global__ void __launch_bounds__(256) clearance_kernel(KinArgs<float> ka, Scene
 // kInlineCloud: clouds are tested inside the walk (the edge pass, whose
 // states have no batch-wide cloud pass of their own); otherwise the walk
 // writes cloud queries for the matrix-core pass when use_cw is set.
-template <bool kInlineCloud>
+// kFirstHit: the lanes are states of one edge and only the first certain
+// collision in visiting order (`key`) matters, so a lane whose key is past
+// the wave's first certain collision so far stops taking part.
+template <bool kInlineCloud, bool kFirstHit = false>
 struct ValidVis32T {
   static constexpr bool kSelf = true;
   static constexpr bool kBlocks = true;
@@ -254,6 +264,7 @@ struct ValidVis32T {
   CloudWork cw{};        // cloud queries are written here when use_cw (the scene holds clouds)
   bool use_cw = false;
   int64_t b = 0;
+  int key = 0;  // visiting index of the lane's state (kFirstHit)
   bool idle = false;
   bool definite = false;
   bool uncertain = false;
@@ -336,6 +347,12 @@ struct ValidVis32T {
     // |R v| <= growf |v| for a lane whose |q|^2 is in [n2_lo, n2_hi]
     const float growf = 1.0f + 2.0f * fmaxf(0.0f, fmaxf(nd.n2_hi - 1.0f, 1.0f - nd.n2_lo)) + 1e-6f;
     for (int bi = lo; bi < hi; ++bi) {
+      if constexpr (kFirstHit) {
+        if (wave_any(definite && !idle)) {
+          const int first = wave_min(definite && !idle ? key : INT_MAX);
+          if (key > first) idle = true;
+        }
+      }
       if (!wave_any(alive())) return;
       const DevSphereBlock bk = cload(ka.blocks, bi);
       const int n = bk.n;
@@ -783,12 +800,6 @@ __device__ __forceinline__ EdgeQv edge_waypoint(const EdgeArgs& ea, const EdgeEn
   return EdgeQv{en, t, k == 0};
 }
 
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int s = 32; s > 0; s >>= 1) v = min(v, __shfl_xor(v, s, 64));
-  return __builtin_amdgcn_readfirstlane(v);
-}
-
 // valid[e]: 0 invalid, 1 valid, 2 pending FP64 re-decision.  n_checked[e]:
 // the states the reference's checkMotion evaluates, i.e. the visiting-order
 // index of the first invalid state plus one, or every state of a valid edge
@@ -838,8 +849,9 @@ __global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArg
         for (int64_t base = 0; base < n_wp; base += 64) {
           const int64_t k = base + lane;
           const bool has = k < n_wp;
-          ValidVis32T<kCloud> v{k32, s32, band};
+          ValidVis32T<kCloud, true> v{k32, s32, band};
           v.idle = !has;
+          v.key = (int)k;
           run_fk(k32, edge_waypoint(ea, ends, n_test, ratio, has ? k : 0), taylor, v);
           const bool def = has && v.definite;
           if (__any(def)) kdef = wave_min(def ? (int)k : INT_MAX);
