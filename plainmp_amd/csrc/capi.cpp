@@ -1096,10 +1096,11 @@ pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, in
       }
       flags = (int32_t*)ctx->ws_flags.get(sizeof(int32_t) * B);
     }
-    // small cloud-free batches: a lane per (configuration, unit); the flags
-    // start at zero (cleared when the buffer is new) and the pass leaves them so
+    // batches up to kValidUnitsMaxBatch: a lane per (configuration, unit);
+    // the flags start at zero (cleared when the buffer is new) and the pass
+    // leaves them so
     int32_t* unit_flags = nullptr;
-    if (ctx->n_clouds == 0 && B <= kValidUnitsMaxBatch && ctx->k32.n_units > 0) {
+    if (B <= kValidUnitsMaxBatch && ctx->k32.n_units > 0) {
       const bool fresh = ctx->ws_vunit.bytes < sizeof(int32_t) * (size_t)kValidUnitsMaxBatch;
       unit_flags = (int32_t*)ctx->ws_vunit.get(sizeof(int32_t) * kValidUnitsMaxBatch);
       if (fresh) PMP_HIP(launch_fill_i32(unit_flags, 0, kValidUnitsMaxBatch, st(ctx)));
@@ -1139,9 +1140,8 @@ pmp_status pmp_is_valid_batch_noalloc(pmp_context* ctx, const double* q, int64_t
         cw.rr[oc] = cw.cap2[oc] + Q;
       }
       flags = (int32_t*)carve(need, sizeof(int32_t) * (size_t)B);
-    } else if (B <= kValidUnitsMaxBatch && ctx->k32.n_units > 0) {
-      unit_flags = (int32_t*)carve(need, sizeof(int32_t) * (size_t)B);
     }
+    if (B <= kValidUnitsMaxBatch && ctx->k32.n_units > 0) unit_flags = (int32_t*)carve(need, sizeof(int32_t) * (size_t)B);
     PMP_REQUIRE(need <= kNoAllocScratchBytes,
                 "batch needs " + std::to_string(need) + " bytes of scratch, more than the " +
                     std::to_string(kNoAllocScratchBytes) +

@@ -656,20 +656,26 @@ __global__ void __launch_bounds__(256) valid_kernel(KinArgs<float> k32, SceneArg
   if (active) valid[b] = v.definite ? 0 : (v.uncertain ? 2 : 1);
 }
 
-// Small batches of a scene without clouds: a lane per (configuration, unit),
-// the unit being grid row y, so that a batch of a few thousand
-// configurations still spreads over the GPU.  Each lane ORs its outcome
-// into flags[b] (1: a certain collision, 2: a test in the band);
+// Batches of up to kValidUnitsMaxBatch configurations: a lane per
+// (configuration, unit), the unit being grid row y, so that a batch of a few
+// thousand configurations still spreads over the GPU.  Each lane ORs its
+// outcome into flags[b] (1: a certain collision, 2: a test in the band);
 // valid_units_finish_kernel turns the flags into the answer valid_kernel
 // gives (a certain collision anywhere decides, else any band test leaves
 // the configuration to the FP64 pass) and clears them for the next launch.
+// With clouds in the scene each lane writes the cloud queries of its unit's
+// sphere blocks (the units' blocks partition the spheres).
 __global__ void __launch_bounds__(64) valid_units_kernel(KinArgs<float> k32, SceneArgs<float> s32, const double* q,
-                                                         int64_t ldq, int64_t B, float band, int32_t* flags) {
+                                                         int64_t ldq, int64_t B, float band, int32_t* flags,
+                                                         CloudWork cw) {
   const int64_t b = gid();
   const bool active = b < B;
   BatchQ qs{q, ldq, active ? b : B - 1};
   ValidUnitVis v{{k32, s32, band}};
   v.u = cload(k32.units, (int)blockIdx.y);
+  v.cw = cw;
+  v.use_cw = s32.n_cloud > 0 && active;
+  v.b = b;
   v.idle = !active;
   run_fk(k32, qs, k32.base_type == PMP_BASE_FIXED, v);
   const int f = (v.definite ? 1 : 0) | (v.uncertain ? 2 : 0);
@@ -1775,9 +1781,9 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                         hipStream_t st) {
   if (block <= 0) block = 256;
   block = slot_block(k32, block);
-  if (unit_flags && n_clouds == 0 && k32.n_units > 0) {
+  if (unit_flags && k32.n_units > 0) {
     hipLaunchKernelGGL(valid_units_kernel, dim3(blocks_for(B, 64), (unsigned)k32.n_units), dim3(64),
-                       slot_bytes(k32, 64), st, k32, s32, q, ldq, B, band, unit_flags);
+                       slot_bytes(k32, 64), st, k32, s32, q, ldq, B, band, unit_flags, cw);
     hipLaunchKernelGGL(valid_units_finish_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, unit_flags, B, valid);
   } else {
     hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), slot_bytes(k32, block), st, k32, s32, q,

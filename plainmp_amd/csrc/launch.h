@@ -80,10 +80,9 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                         uint8_t* valid, int32_t* n_recheck, int block, const CloudWork& cw,
                         const CloudRange* clouds, int n_clouds, int32_t* flags, int32_t* unit_flags,
                         hipStream_t st);
-// Batches up to this size in scenes without clouds take the pass with a lane
-// per (configuration, evaluate unit); unit_flags: [B] int32, zero on entry
-// and left zero.
-constexpr int64_t kValidUnitsMaxBatch = 16384;
+// Batches up to this size take the pass with a lane per (configuration,
+// evaluate unit); unit_flags: [B] int32, zero on entry and left zero.
+constexpr int64_t kValidUnitsMaxBatch = 65536;
 // noalloc.hip: the scratch of pmp_is_valid_batch_noalloc, a static device
 // array of kNoAllocScratchBytes per GPU.  noalloc_scratch_init looks up this
 // device's copy (call with `device` current); noalloc_scratch returns it

@@ -4,7 +4,8 @@
 * edges_mixed16: 4,096 edges under the autotuned schedule and as replays of
   the captured graph; validity and the reference-order checked counts exact.
 * cloud_shelves: 16,384 configurations against the 65,536-point cloud;
-  validity exact.
+  validity exact; batches above the unit pass's 65,536 (plain_two_bars
+  81,920, cloud_shelves 69,632) through the lane-per-configuration kernel.
 * grad_mixed32: 8,192 configurations; evaluate values within 5e-6 m and rows
   within 1e-4 (relative, unambiguous argmin), every pair distance within 5e-6.
 * traj_dual_arm: 2,048 trajectories (122,138 waypoints); evaluate values and
@@ -78,6 +79,20 @@ def test_cloud_shelves_full_validity():
     ref = oc.is_valid_batch(wl.Q, THREADS).astype(bool)
     got, n_re = pc.is_valid_batch_stats(wl.Q)
     _report("cloud_full", B=len(ref), valid_frac=ref.mean(), rechecked=n_re)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("name,B", [("plain_two_bars", 81920), ("cloud_shelves", 69632)])
+def test_valid_batches_above_the_unit_pass(name, B):
+    """Batches above 65,536 configurations take the lane-per-configuration
+    validity kernel instead of the unit pass; validity exact."""
+    base = {"plain_two_bars": 1024, "cloud_shelves": 16384}[name]
+    wl = synth.workload(name, scale=B / base)
+    assert len(wl.Q) == B
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref = oc.is_valid_batch(wl.Q, THREADS).astype(bool)
+    got = pc.is_valid_batch(wl.Q)
+    _report("valid_large", workload=name, B=B, valid_frac=ref.mean())
     assert np.array_equal(got, ref)
 
 
