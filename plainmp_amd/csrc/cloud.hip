@@ -41,7 +41,8 @@ __device__ __forceinline__ float wave_max16(float v) {
 // min |c - p|^2 over the cloud's tiles for the queries q = e * B + b
 // (e < Q / B spheres, b < B configurations) whose cap2[q] >= 0; centres in
 // cen[(e * 3 + k) * B + b].  Writes out_d2[q] = min squared distance
-// (FLT_MAX when no point lies within the cap).  With kBounds, the minimum is
+// (FLT_MAX when no point lies within the cap; a wave whose 16 queries are
+// all unqueried or skipped writes nothing).  With kBounds, the minimum is
 // bracketed instead: out_d2[q] = min over tiles of (d2 + err) and out_lo[q] =
 // min over tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 + rho^2) per
 // tile, so the exact minimum lies in [out_lo, out_d2].  `skip` (optional, per
@@ -73,14 +74,19 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   const int64_t b = w % B, e0 = (w / B) * 16;
   const int i = lane & 15, k = lane >> 4;
   const bool skipped = skip && skip[b] == 0;
-  // the A-row query of this lane
-  float cx = 0.f, cy = 0.f, cz = 0.f, capi = -1.f;
+  // the A-row query of this lane; a wave none of whose queries is live ends
+  // here without writing: the decisions read a query's bracket only when the
+  // query is live (rr >= 0, written with cap2), and a skipped configuration
+  // is already decided
+  float capi = -1.f;
+  if (e0 + i < S && !skipped) capi = cap2 ? cap2[(e0 + i) * B + b] : FLT_MAX;
+  if (!__any(capi >= 0.f)) return;
+  float cx = 0.f, cy = 0.f, cz = 0.f;
   if (e0 + i < S) {
     const int64_t e = e0 + i;
     cx = cen[(e * 3 + 0) * B + b];
     cy = cen[(e * 3 + 1) * B + b];
     cz = cen[(e * 3 + 2) * B + b];
-    capi = skipped ? -1.f : (cap2 ? cap2[e * B + b] : FLT_MAX);
   }
   // centres of the four output rows 4k + r held by this lane
   float rx[4], ry[4], rz[4];
@@ -97,7 +103,7 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   float best[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
   float lo[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
-  if (__any(capi >= 0.f)) {
+  {
     // the box around the live queries and their largest cap
     const bool live = capi >= 0.f;
     const float qlx = wave_min16(live ? cx : FLT_MAX), qly = wave_min16(live ? cy : FLT_MAX),
