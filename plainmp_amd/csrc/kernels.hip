@@ -418,7 +418,7 @@ struct ValidVis32T {
       kind_loop<kKindCylRot>(c, bk, n, g, cr, bi);
       kind_loop<kKindSphere>(c, bk, n, g, cr, bi);
       kind_loop<kKindGround>(c, bk, n, g, cr, bi);
-      if constexpr (kInlineCloud) cloud_inline(c, bk, n, g, nd.n2_hi);
+      if constexpr (kInlineCloud) cloud_inline(c, bk, n, g, nd.n2_hi, growf);
     }
   }
 
@@ -433,11 +433,14 @@ struct ValidVis32T {
   // around it, widened by the tile-local form's error bound, left to the
   // FP64 walk.
   __device__ __forceinline__ void cloud_inline(const V3<float> (&c)[kBlockSpheres], const DevSphereBlock& bk, int n,
-                                               V3<float> g, float n2_hi) {
+                                               V3<float> g, float n2_hi, float growf) {
     const DevPrim<float>* vp = reinterpret_cast<const DevPrim<float>*>(sc.vprims);
     for (int p = sc.vrange[kKindCloud]; p < sc.vrange[kKindCloud + 1]; ++p) {
       if (!wave_any(alive())) return;
       const DevPrim<float> Pc = cload(vp, p);
+      // every sphere of the block fails the reference's AABB pre-test when
+      // the bounding centre is outside the AABB grown by the block's reach
+      if (!wave_any(alive() && !outside_aabb(Pc, g, bk.G * growf + band + 1e-4f))) continue;
       const float rc = Pc.p0;
       const float reach = bk.G * n2_hi + rc + band + 1e-5f;
       float hi[kBlockSpheres], lo[kBlockSpheres];
