@@ -36,6 +36,26 @@ __device__ __forceinline__ float box_dist2(const float (&lb)[3], const float (&u
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Squared distance between the boxes [lb, ub] and [l, u] (0 when they meet).
+__device__ __forceinline__ float box_box_dist2(const float (&lb)[3], const float (&ub)[3], V3<float> l, V3<float> u) {
+  const float dx = fmaxf(fmaxf(lb[0] - u.x, l.x - ub[0]), 0.f);
+  const float dy = fmaxf(fmaxf(lb[1] - u.y, l.y - ub[1]), 0.f);
+  const float dz = fmaxf(fmaxf(lb[2] - u.z, l.z - ub[2]), 0.f);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Minimum / maximum of v over the wave (every lane active; wave-uniform).
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) v = fminf(v, __shfl_xor(v, s, 64));
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) v = fmaxf(v, __shfl_xor(v, s, 64));
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // Minimum of v over the wave (every lane active; wave-uniform result).
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -54,60 +74,92 @@ struct alignas(16) Quad4 {
   float v[4];
 };
 
+// One kept tile t of cloud_block_min: the spheres' minima over its points.
+template <bool kBounds>
+__device__ __forceinline__ void cloud_tile_min(const SceneArgs<float>& sc, int t, const CloudTile& T,
+                                               const V3<float> (&c)[kBlockSpheres], int n,
+                                               float (&best)[kBlockSpheres], float (&lo)[kBlockSpheres]) {
+  V3<float> cl[kBlockSpheres];
+  float tb[kBlockSpheres];
+#pragma unroll
+  for (int k = 0; k < kBlockSpheres; ++k) {
+    cl[k] = V3<float>{c[k].x - T.o[0], c[k].y - T.o[1], c[k].z - T.o[2]};
+    tb[k] = FLT_MAX;
+  }
+  const Quad4* Bt = reinterpret_cast<const Quad4*>(sc.tileB + (int64_t)t * 4 * kCloudTile);
+  for (int m = 0; m < 4; ++m) {
+    for (int j4 = 0; j4 < 4; ++j4) {
+      // points 16 m + 4 j4 .. + 3: components x, y, z, |p'|^2
+      const Quad4 X = cload(Bt, m * 16 + 0 * 4 + j4), Y = cload(Bt, m * 16 + 1 * 4 + j4);
+      const Quad4 Z = cload(Bt, m * 16 + 2 * 4 + j4), W = cload(Bt, m * 16 + 3 * 4 + j4);
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+#pragma unroll
+        for (int k = 0; k < kBlockSpheres; ++k) {
+          if (k < n)
+            tb[k] = fminf(tb[k], fmaf(X.v[jj], cl[k].x, fmaf(Y.v[jj], cl[k].y, fmaf(Z.v[jj], cl[k].z, W.v[jj]))));
+        }
+      }
+    }
+  }
+  const float rho2 = kBounds ? tile_rho2(T) : 0.f;
+#pragma unroll
+  for (int k = 0; k < kBlockSpheres; ++k) {
+    const float off = dot(cl[k], cl[k]);
+    if constexpr (kBounds) {
+      const float err = kCloudGemmErr * (off + rho2);
+      best[k] = fminf(best[k], tb[k] + off + err);
+      lo[k] = fminf(lo[k], tb[k] + off - err);
+    } else {
+      best[k] = fminf(best[k], tb[k] + off);
+    }
+  }
+}
+
 // Per sphere k < n of a block, min |c_k - p|^2 over the points of cloud oc
 // that lie in tiles within sqrt(reach2) of the block's bounding centre g
 // (FLT_MAX when none).  Super-tiles, then tiles, are skipped for the whole
-// wave when no `live` lane's g reaches their box; a kept tile's points are
-// compared in tile-local coordinates, |c - p|^2 = |c'|^2 + (|p'|^2 - 2 c'.p'),
-// from the matrix-core pass's operands (-2 p', |p'|^2).  With kBounds the
-// minimum is bracketed: best = min over tiles of (d2 + err), lo = min over
-// tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 + rho^2) of the tile.
+// wave when no `live` lane's g reaches their box: 64 of them at a time are
+// first tested against the box around the live lanes' g, a box a lane, and
+// the survivors of that ballot are tested lane by lane.  A kept tile's points
+// are compared in tile-local coordinates, |c - p|^2 = |c'|^2 + (|p'|^2 -
+// 2 c'.p'), from the matrix-core pass's operands (-2 p', |p'|^2).  With
+// kBounds the minimum is bracketed: best = min over tiles of (d2 + err), lo =
+// min over tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 + rho^2) of the
+// tile.
 template <bool kBounds>
 __device__ __forceinline__ void cloud_block_min(const SceneArgs<float>& sc, int oc, const V3<float> (&c)[kBlockSpheres],
                                                 int n, V3<float> g, float reach2, bool live,
                                                 float (&best)[kBlockSpheres], float (&lo)[kBlockSpheres]) {
 #pragma unroll
   for (int k = 0; k < kBlockSpheres; ++k) best[k] = lo[k] = FLT_MAX;
+  if (!wave_any(live)) return;
+  const V3<float> gl{wave_minf(live ? g.x : FLT_MAX), wave_minf(live ? g.y : FLT_MAX), wave_minf(live ? g.z : FLT_MAX)};
+  const V3<float> gu{wave_maxf(live ? g.x : -FLT_MAX), wave_maxf(live ? g.y : -FLT_MAX),
+                     wave_maxf(live ? g.z : -FLT_MAX)};
+  const int lane = threadIdx.x & 63;
   const int s_end = sc.super_at[oc + 1];
-  for (int si = sc.super_at[oc]; si < s_end; ++si) {
-    const CloudSuper S = cload(sc.supers, si);
-    if (!wave_any(live && box_dist2(S.lb, S.ub, g) <= reach2)) continue;
-    for (int t = S.first; t < S.first + S.count; ++t) {
-      const CloudTile T = cload(sc.tiles, t);
-      if (!wave_any(live && box_dist2(T.lb, T.ub, g) <= reach2)) continue;
-      V3<float> cl[kBlockSpheres];
-      float tb[kBlockSpheres];
-#pragma unroll
-      for (int k = 0; k < kBlockSpheres; ++k) {
-        cl[k] = V3<float>{c[k].x - T.o[0], c[k].y - T.o[1], c[k].z - T.o[2]};
-        tb[k] = FLT_MAX;
-      }
-      const Quad4* Bt = reinterpret_cast<const Quad4*>(sc.tileB + (int64_t)t * 4 * kCloudTile);
-      for (int m = 0; m < 4; ++m) {
-        for (int j4 = 0; j4 < 4; ++j4) {
-          // points 16 m + 4 j4 .. + 3: components x, y, z, |p'|^2
-          const Quad4 X = cload(Bt, m * 16 + 0 * 4 + j4), Y = cload(Bt, m * 16 + 1 * 4 + j4);
-          const Quad4 Z = cload(Bt, m * 16 + 2 * 4 + j4), W = cload(Bt, m * 16 + 3 * 4 + j4);
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-#pragma unroll
-            for (int k = 0; k < kBlockSpheres; ++k) {
-              if (k < n)
-                tb[k] = fminf(tb[k], fmaf(X.v[jj], cl[k].x, fmaf(Y.v[jj], cl[k].y, fmaf(Z.v[jj], cl[k].z, W.v[jj]))));
-            }
-          }
+  for (int s0 = sc.super_at[oc]; s0 < s_end; s0 += 64) {
+    bool snear = false;
+    if (s0 + lane < s_end) {
+      const CloudSuper Sv = sc.supers[s0 + lane];  // a super-tile a lane: vector loads
+      snear = box_box_dist2(Sv.lb, Sv.ub, gl, gu) <= reach2;
+    }
+    for (uint64_t smask = __ballot(snear); smask; smask &= smask - 1) {
+      const CloudSuper S = cload(sc.supers, s0 + __builtin_ctzll(smask));
+      if (!wave_any(live && box_dist2(S.lb, S.ub, g) <= reach2)) continue;
+      const int t_end = S.first + S.count;
+      for (int t0 = S.first; t0 < t_end; t0 += 64) {
+        bool tnear = false;
+        if (t0 + lane < t_end) {
+          const CloudTile Tv = sc.tiles[t0 + lane];
+          tnear = box_box_dist2(Tv.lb, Tv.ub, gl, gu) <= reach2;
         }
-      }
-      const float rho2 = kBounds ? tile_rho2(T) : 0.f;
-#pragma unroll
-      for (int k = 0; k < kBlockSpheres; ++k) {
-        const float off = dot(cl[k], cl[k]);
-        if constexpr (kBounds) {
-          const float err = kCloudGemmErr * (off + rho2);
-          best[k] = fminf(best[k], tb[k] + off + err);
-          lo[k] = fminf(lo[k], tb[k] + off - err);
-        } else {
-          best[k] = fminf(best[k], tb[k] + off);
+        for (uint64_t tmask = __ballot(tnear); tmask; tmask &= tmask - 1) {
+          const int t = t0 + __builtin_ctzll(tmask);
+          const CloudTile T = cload(sc.tiles, t);
+          if (!wave_any(live && box_dist2(T.lb, T.ub, g) <= reach2)) continue;
+          cloud_tile_min<kBounds>(sc, t, T, c, n, best, lo);
         }
       }
     }
