@@ -249,6 +249,45 @@ PMP_HD T sdf_eval(const DevPrim<T>& p, V3<T> c) {
   return T(0);
 }
 
+// sdf_eval for a primitive of kind K (kKind* of prim_kind): the same
+// arithmetic with the type branches resolved at compile time.
+template <int K, class T>
+PMP_HD T sdf_eval_kind(const DevPrim<T>& p, V3<T> c) {
+  if constexpr (K == kKindBoxAA || K == kKindBoxRot) {
+    V3<T> d = sub(c, V3<T>{p.pos[0], p.pos[1], p.pos[2]});
+    if constexpr (K == kKindBoxRot) d = mv(load_m3(p.Rinv), d);
+    T sx = fabs(d.x) - p.p0, sy = fabs(d.y) - p.p1, sz = fabs(d.z) - p.p2;
+    T mx = sx > T(0) ? sx : T(0), my = sy > T(0) ? sy : T(0), mz = sz > T(0) ? sz : T(0);
+    T outside = sqrt(mx * mx + my * my + mz * mz);
+    T ix = sx < T(0) ? sx : T(0), iy = sy < T(0) ? sy : T(0), iz = sz < T(0) ? sz : T(0);
+    T inside = ix > iy ? ix : iy;
+    inside = inside > iz ? inside : iz;
+    return outside + inside;
+  } else if constexpr (K == kKindCylZ || K == kKindCylRot) {
+    T zs, xa, ya;
+    if constexpr (K == kKindCylZ) {
+      zs = fabs(c.z - p.pos[2]) - p.p1;
+      xa = fabs(c.x - p.pos[0]);
+      ya = fabs(c.y - p.pos[1]);
+    } else {
+      V3<T> pc = sub(c, V3<T>{p.pos[0], p.pos[1], p.pos[2]});
+      zs = fabs(pc.x * p.R[2] + pc.y * p.R[5] + pc.z * p.R[8]) - p.p1;
+      xa = fabs(pc.x * p.R[0] + pc.y * p.R[3] + pc.z * p.R[6]);
+      ya = fabs(pc.x * p.R[1] + pc.y * p.R[4] + pc.z * p.R[7]);
+    }
+    T rs = sqrt(xa * xa + ya * ya) - p.p0;
+    T o0 = rs > T(0) ? rs : T(0), o1 = zs > T(0) ? zs : T(0);
+    T outside = sqrt(o0 * o0 + o1 * o1);
+    T i0 = rs < T(0) ? rs : T(0), i1 = zs < T(0) ? zs : T(0);
+    return outside + (i0 > i1 ? i0 : i1);
+  } else if constexpr (K == kKindSphere) {
+    V3<T> d = sub(c, V3<T>{p.pos[0], p.pos[1], p.pos[2]});
+    return sqrt(dot(d, d)) - p.p0;
+  } else {  // ground
+    return c.z + p.p0;
+  }
+}
+
 // Gradient of sdf_eval with respect to the point (clouds excluded).  At kinks
 // the branch taken follows the evaluate() formula's active terms.
 template <class T>

@@ -1197,18 +1197,20 @@ struct EvalUnitVis {
       // every member sphere's value is at least the bounding centre's
       // distance minus the block's reach (the SDFs are 1-Lipschitz): a wave
       // whose lanes all have their best below that bound skips the candidate
-      if (!all && !wave_any(!(sdf_eval(P, g) - ereach > best))) continue;
+      if (!all && !wave_any(!(sdf_eval_kind<K>(P, g) - ereach > best))) continue;
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         if (k < n) {
           const float r = bk.s[k][3];
           if (outside_aabb(P, c[k], r + kCutoff)) continue;
-          const float v = sdf_eval(P, c[k]) - r;
-          const int key = (m[k].group * sc.n_prims + orig) * ka.n_spheres + m[k].out;
-          if (v < best || (bidx >= 0 && v == best && key < bkey)) {
-            best = v;
-            bkey = key;
-            bidx = bk.sph[k] * kMaxPrims + orig;
+          const float v = sdf_eval_kind<K>(P, c[k]) - r;
+          if (v <= best) {  // a new best, or a tie that the visiting order settles
+            const int key = (m[k].group * sc.n_prims + orig) * ka.n_spheres + m[k].out;
+            if (v < best || (bidx >= 0 && key < bkey)) {
+              best = v;
+              bkey = key;
+              bidx = bk.sph[k] * kMaxPrims + orig;
+            }
           }
         }
       }
