@@ -797,6 +797,27 @@ pmp_status pmp_upload_robot(pmp_context* ctx, const pmp_robot_desc* r) {
       DevSelfPair pr{};
       range(g1, pr.begin1, pr.end1);
       range(g2, pr.begin2, pr.end2);
+      // bounding sphere of a group's spheres in its link frame (the centre
+      // of their box, the radius covering every sphere, rounded up)
+      auto bound = [&](int b, int e, float* out) {
+        auto coord = [&](int k, int d) { return d == 0 ? sp64[k].x : (d == 1 ? sp64[k].y : sp64[k].z); };
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int t = b; t < e; ++t)
+          for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], coord(t, d) - sp64[t].r);
+            hi[d] = std::max(hi[d], coord(t, d) + sp64[t].r);
+          }
+        for (int d = 0; d < 3; ++d) out[d] = (float)(0.5 * (lo[d] + hi[d]));
+        double R = 0.0;
+        for (int t = b; t < e; ++t) {
+          const double ex = coord(t, 0) - out[0], ey = coord(t, 1) - out[1], ez = coord(t, 2) - out[2];
+          R = std::max(R, std::sqrt(ex * ex + ey * ey + ez * ez) + sp64[t].r);
+        }
+        out[3] = round_up(R);
+        if (!(R == R) || b >= e) out[3] = INFINITY;  // never cull around a NaN or empty bound
+      };
+      bound(pr.begin1, pr.end1, pr.bound1);
+      bound(pr.begin2, pr.end2, pr.bound2);
       pr.node1 = group_node[g1];
       pr.node2 = group_node[g2];
       pr.slot1 = slot[pr.node1];

@@ -72,6 +72,20 @@ struct is_ranged<V, decltype((void)V::kRanged)> {
   static constexpr bool value = V::kRanged;
 };
 
+// Visitors that declare `static constexpr bool kPairCull = true` provide
+// bool pair_near(V3<T> g1, V3<T> g2, T reach): whether, in this lane, the
+// sphere pairs of a group pair whose bounding spheres (world centres g1, g2,
+// summed radii `reach`) are this far apart can still matter; a group pair
+// that matters to no live lane is skipped.
+template <class V, class = void>
+struct is_pair_cull {
+  static constexpr bool value = false;
+};
+template <class V>
+struct is_pair_cull<V, decltype((void)V::kPairCull)> {
+  static constexpr bool value = V::kPairCull;
+};
+
 // Kept poses (ancestors of later branches, links of self pairs) live in LDS:
 // the launch reserves slot_bytes(ka, block) bytes of dynamic shared memory
 // from byte offset ka.slot_off, ka.slot_wave_bytes per wave, laid out
@@ -234,6 +248,17 @@ __device__ __forceinline__ void run_fk(const KinArgs<T>& ka, const QS& qs, bool 
     const DevSelfPair pr = cload(ka.pairs, k);
     const Pose<T> P1 = slots.get(pr.slot1), P2 = slots.get(pr.slot2);
     const M3<T> R1 = qmat(P1.q), R2 = qmat(P2.q);
+    if constexpr (is_pair_cull<Vis>::value) {
+      // the groups' bounding spheres; |R v| <= grow |v| for a rotation
+      // matrix of a quaternion with |q|^2 = n, grow = 1 + 2 |n - 1|
+      const V3<T> g1 = add(mv(R1, V3<T>{pr.bound1[0], pr.bound1[1], pr.bound1[2]}), P1.t);
+      const V3<T> g2 = add(mv(R2, V3<T>{pr.bound2[0], pr.bound2[1], pr.bound2[2]}), P2.t);
+      const T n1 = P1.q.x * P1.q.x + P1.q.y * P1.q.y + P1.q.z * P1.q.z + P1.q.w * P1.q.w;
+      const T n2 = P2.q.x * P2.q.x + P2.q.y * P2.q.y + P2.q.z * P2.q.z + P2.q.w * P2.q.w;
+      const T dn = fabs(n1 - T(1)) > fabs(n2 - T(1)) ? fabs(n1 - T(1)) : fabs(n2 - T(1));
+      const T reach = (T(pr.bound1[3]) + T(pr.bound2[3])) * (T(1) + T(2) * dn);
+      if (!wave_any(vis.alive() && vis.pair_near(g1, g2, reach))) continue;
+    }
     for (int i = pr.begin1; i < pr.end1; ++i) {
       const DevSphere<T> a = cload(ka.spheres, i);
       const V3<T> ci = add(mv(R1, V3<T>{a.x, a.y, a.z}), P1.t);

@@ -515,6 +515,12 @@ struct ValidVis32T {
     }
   }
 
+  // A sphere pair can collide or fall in the band only within rs + band.
+  static constexpr bool kPairCull = true;
+  __device__ bool pair_near(V3<float> g1, V3<float> g2, float reach) const {
+    const V3<float> d = sub(g1, g2);
+    return !(sqrtf(dot(d, d)) - reach > band + 1e-4f);
+  }
   __device__ void self(const DevSelfPair&, int, int, V3<float> ci, V3<float> cj, float rs) {
     if (!alive()) return;
     const V3<float> d = sub(ci, cj);
@@ -868,8 +874,10 @@ __device__ __forceinline__ EdgeQv edge_waypoint(const EdgeArgs& ea, const EdgeEn
 // more than kMaxEdgeStates states).  A pending edge queues its in-band states
 // that can matter (all of them, or those before the first certain collision)
 // and that collision's index; the FP64 pass decides from them.
+// At least three waves per SIMD: the cloud variant sits at the edge of that
+// register budget.
 template <bool kCloud>
-__global__ void __launch_bounds__(256) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, KinArgs<double> k64,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, KinArgs<double> k64,
                                                     SceneArgs<double> s64, EdgeArgs ea, float band) {
   const int lane = threadIdx.x & 63;
   const bool taylor = k32.base_type == PMP_BASE_FIXED;
@@ -1286,6 +1294,13 @@ struct EvalUnitVis {
     }
   }
 
+  // A sphere pair's value is at least the groups' distance minus their
+  // reach; only a value below the lane's best self value changes it.
+  static constexpr bool kPairCull = true;
+  __device__ bool pair_near(V3<float> g1, V3<float> g2, float reach) const {
+    const V3<float> d = sub(g1, g2);
+    return !(sqrtf(dot(d, d)) - reach > sbest + 1e-5f);
+  }
   __device__ void self(const DevSelfPair&, int i, int j, V3<float> ci, V3<float> cj, float rs) {
     const V3<float> d = sub(ci, cj);
     const float dist = sqrtf(dot(d, d)) - rs;

@@ -131,6 +131,7 @@ struct CloudWork {
 struct alignas(16) DevSelfPair {
   int32_t slot1, begin1, end1, slot2;
   int32_t begin2, end2, node1, node2;
+  float bound1[4], bound2[4];  // each group's bounding sphere in its link frame: centre, radius (rounded up)
 };
 
 // Kernel arguments describing the robot (passed by value: kernarg -> SGPRs).
