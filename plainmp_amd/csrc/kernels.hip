@@ -1101,6 +1101,30 @@ __global__ void __launch_bounds__(64 * kRecheckWaves) edges_recheck_kernel(KinAr
 //   revolute: a_j x (c - o_j)      prismatic: a_j
 // (KinematicModel::get_attached_point_jacobian, kinematics/algorithm.cpp:346-433;
 // the base columns are the analytic form of its finite differences).
+// The base's own q columns (a planar or floating base), as (column, axis,
+// origin) frames: translations along x, y, z, then the roll / pitch / yaw
+// axes of the base's Rz(yaw) Ry(pitch) Rx(roll) rotation.
+template <class QS, class F>
+__device__ __forceinline__ void base_columns(const KinArgs<float>& ka, const Pose<float>& base, const QS& qs, F&& f) {
+  const int n = ka.n_q_cols;
+  if (ka.base_type == PMP_BASE_PLANAR) {
+    f(n, V3<float>{1.f, 0.f, 0.f}, base.t);
+    f(n + 1, V3<float>{0.f, 1.f, 0.f}, base.t);
+    f(n + 2, V3<float>{0.f, 0.f, 1.f}, base.t);
+  } else if (ka.base_type == PMP_BASE_FLOATING) {
+    const float pitch = (float)qs.get(n + 4), yaw = (float)qs.get(n + 5);
+    float sp, cp, sy, cy;
+    sincosf(pitch, &sp, &cp);
+    sincosf(yaw, &sy, &cy);
+    f(n, V3<float>{1.f, 0.f, 0.f}, base.t);
+    f(n + 1, V3<float>{0.f, 1.f, 0.f}, base.t);
+    f(n + 2, V3<float>{0.f, 0.f, 1.f}, base.t);
+    f(n + 3, V3<float>{cy * cp, sy * cp, -sp}, base.t);  // Rz(yaw) Ry(pitch) x
+    f(n + 4, V3<float>{-sy, cy, 0.f}, base.t);           // Rz(yaw) y
+    f(n + 5, V3<float>{0.f, 0.f, 1.f}, base.t);
+  }
+}
+
 template <int MAXC>
 struct Frames {
   float ax[MAXC], ay[MAXC], az[MAXC], ox[MAXC], oy[MAXC], oz[MAXC];
@@ -1118,23 +1142,7 @@ struct Frames {
   }
   template <class QS>
   __device__ __forceinline__ void set_base(const KinArgs<float>& ka, const Pose<float>& base, const QS& qs) {
-    const int n = ka.n_q_cols;
-    if (ka.base_type == PMP_BASE_PLANAR) {
-      set(n, {1.f, 0.f, 0.f}, base.t);
-      set(n + 1, {0.f, 1.f, 0.f}, base.t);
-      set(n + 2, {0.f, 0.f, 1.f}, base.t);
-    } else if (ka.base_type == PMP_BASE_FLOATING) {
-      const float pitch = (float)qs.get(n + 4), yaw = (float)qs.get(n + 5);
-      float sp, cp, sy, cy;
-      sincosf(pitch, &sp, &cp);
-      sincosf(yaw, &sy, &cy);
-      set(n, {1.f, 0.f, 0.f}, base.t);
-      set(n + 1, {0.f, 1.f, 0.f}, base.t);
-      set(n + 2, {0.f, 0.f, 1.f}, base.t);
-      set(n + 3, {cy * cp, sy * cp, -sp}, base.t);  // Rz(yaw) Ry(pitch) x
-      set(n + 4, {-sy, cy, 0.f}, base.t);           // Rz(yaw) y
-      set(n + 5, {0.f, 0.f, 1.f}, base.t);
-    }
+    base_columns(ka, base, qs, [&](int col, V3<float> a, V3<float> o) { set(col, a, o); });
   }
   __device__ __forceinline__ V3<float> col(int j, uint64_t mask, uint64_t rev, V3<float> c) const {
     V3<float> r{0.f, 0.f, 0.f};
@@ -1168,6 +1176,7 @@ struct EvalUnitVis {
   DevEvalUnit u{};
   float best = kCutoff, sbest = kCutoff;
   int bkey = 0x7fffffff, bidx = -1, sidx = -1;
+  V3<float> bc{0.f, 0.f, 0.f}, sc1{0.f, 0.f, 0.f}, sc2{0.f, 0.f, 0.f};  // the winners' world centres
   float ereach = 0.0f;  // the current block's bounding radius, grown, plus a slack
   __device__ bool alive() const { return true; }
   __device__ int last_node() const { return u.last; }
@@ -1218,6 +1227,7 @@ struct EvalUnitVis {
               best = v;
               bkey = key;
               bidx = bk.sph[k] * kMaxPrims + orig;
+              bc = c[k];
             }
           }
         }
@@ -1288,6 +1298,7 @@ struct EvalUnitVis {
             best = v;
             bkey = key;
             bidx = bk.sph[k] * kMaxPrims + orig;
+            bc = c[k];
           }
         }
       }
@@ -1307,6 +1318,8 @@ struct EvalUnitVis {
     if (dist < sbest) {
       sbest = dist;
       sidx = i * kMaxSpheres + j;
+      sc1 = ci;
+      sc2 = cj;
     }
   }
 };
@@ -1349,6 +1362,54 @@ struct EvalRowVis {
     if (h1) c1 = centre(s1);
     if (h2) c2 = centre(s2);
   }
+  __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
+};
+
+// FK walk that builds the Jacobian rows of the winners column by column as
+// the joint frames appear, from the winners' centres and directions known
+// before the walk (ext: the primitive's gradient g0 at c0; self: the unit
+// normal n from c2 to c1): no frame is kept, column j of a row is
+// d . (a_j x (c - o_j)) for a rotating column and d . a_j otherwise, when j
+// moves the winner's sphere (its node's column mask).
+template <int MAXC>
+struct EvalRowAccVis {
+  static constexpr bool kSelf = false;
+  static constexpr bool kRanged = true;
+  static constexpr bool kBlocks = true;
+  const KinArgs<float>& ka;
+  int last = 0;  // wave-uniform
+  V3<float> c0{0.f, 0.f, 0.f}, c1{0.f, 0.f, 0.f}, c2{0.f, 0.f, 0.f}, g0{0.f, 0.f, 0.f}, n{0.f, 0.f, 0.f};
+  uint64_t m0 = 0, m1 = 0, m2 = 0;  // column masks of the winners' nodes (0: no winner)
+  float re[MAXC], rs[MAXC];
+  __device__ bool alive() const { return true; }
+  __device__ int last_node() const { return last; }
+  __device__ int pair_lo() const { return 0; }
+  __device__ int pair_hi() const { return 0; }
+  __device__ __forceinline__ void column(int col, V3<float> a, V3<float> o) {
+    const bool rev = (ka.rev_mask >> col) & 1ull;
+    auto cv = [&](V3<float> c) { return rev ? cross(a, sub(c, o)) : a; };
+    const float ve = ((m0 >> col) & 1ull) ? dot(g0, cv(c0)) : 0.f;
+    const float v1 = ((m1 >> col) & 1ull) ? dot(n, cv(c1)) : 0.f;
+    const float v2 = ((m2 >> col) & 1ull) ? dot(n, cv(c2)) : 0.f;
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j == col) {
+        re[j] = ve;
+        rs[j] = v1 - v2;
+      }
+  }
+  template <class QS>
+  __device__ void begin(const Pose<float>& base, const QS& qs) {
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) re[j] = rs[j] = 0.f;
+    base_columns(ka, base, qs, [&](int col, V3<float> a, V3<float> o) { column(col, a, o); });
+  }
+  __device__ void node(int, const DevNode<float>& nd, const Pose<float>& P) {
+    if (nd.q_col < 0) return;
+    column(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
+  }
+  __device__ void sphere(int, V3<float>, float) {}
+  __device__ void blocks(const DevNode<float>&, const Pose<float>&) {}
   __device__ void self(const DevSelfPair&, int, int, V3<float>, V3<float>, float) {}
 };
 
@@ -1497,6 +1558,68 @@ __device__ __forceinline__ void eval_finish(const KinArgs<float>& ka, const Scen
   if (o.offsets) smooth_terms<MAXC>(q, ldq, b, D, o);
 }
 
+// The rows of one walk's winners (whole mode, where the unit walk kept the
+// winners' centres): the directions are formed first, then a walk that keeps
+// no frames builds the rows column by column (EvalRowAccVis).
+template <int MAXC, class QS>
+__device__ __forceinline__ void eval_finish_whole(const KinArgs<float>& ka, const SceneArgs<float>& sc, const QS& qs,
+                                                  const double* q, int64_t ldq, int64_t b, const EvalOut& o,
+                                                  const EvalUnitVis<MAXC>& v) {
+  const int D = ka.n_cols, NP = sc.n_prims;
+  EvalRowAccVis<MAXC> rv{ka};
+#pragma unroll
+  for (int j = 0; j < MAXC; ++j) rv.re[j] = rv.rs[j] = 0.f;
+  int need = -1;  // the last node this lane's winners need
+  if (v.bidx >= 0) {
+    const int node0 = cload(ka.meta, v.bidx / kMaxPrims).node;
+    need = node0;
+    rv.c0 = v.bc;
+    rv.m0 = cload(ka.nodes, node0).col_mask;
+    const DevPrim<float> P = prim_at(sc, v.bidx % kMaxPrims);
+    if (P.type == PMP_PRIM_CLOUD) {
+      // the gradient of sqrt(min |c - p|^2) at the nearest point, which the
+      // reference's forward difference (eps 1e-6) approximates
+      float d2;
+      const V3<float> d = sub(v.bc, cloud_nearest(sc, (P.flags >> 4) & 15, v.bc, d2));
+      rv.g0 = scale(1.0f / sqrtf(dot(d, d)), d);
+    } else {
+      rv.g0 = sdf_grad(P, v.bc);
+    }
+  }
+  if (v.sidx >= 0) {
+    const int n1 = cload(ka.meta, v.sidx / kMaxSpheres).node, n2 = cload(ka.meta, v.sidx % kMaxSpheres).node;
+    need = max(need, max(n1, n2));
+    rv.c1 = v.sc1;
+    rv.c2 = v.sc2;
+    const V3<float> d = sub(v.sc1, v.sc2);
+    rv.n = scale(1.0f / sqrtf(dot(d, d)), d);
+    rv.m1 = cload(ka.nodes, n1).col_mask;
+    rv.m2 = cload(ka.nodes, n2).col_mask;
+  }
+  // lanes of the wave that reach this point walk together (inactive lanes
+  // have returned); no winner in the wave: the rows are zero without FK
+  const int last = wave_max_i(need);
+  if (last >= 0) {
+    rv.last = last;
+    run_fk(ka, qs, false, rv);
+  }
+  int row = 0;
+  if (NP > 0) {
+    o.vals[b] = v.best;
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j < D) o.jac[(int64_t)j * o.ld + b] = rv.re[j];
+    row = 1;
+  }
+  if (ka.n_pairs > 0) {
+    o.vals[(int64_t)row * o.ld + b] = v.sbest;
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j)
+      if (j < D) o.jac[((int64_t)row * D + j) * o.ld + b] = rv.rs[j];
+  }
+  if (o.offsets) smooth_terms<MAXC>(q, ldq, b, D, o);
+}
+
 // Whole mode: one wave per chunk of 64 configurations walks every sphere
 // block and self pair of them and reduces in place.
 template <int MAXC>
@@ -1509,8 +1632,7 @@ __global__ void __launch_bounds__(64) eval_units_kernel(KinArgs<float> ka, Scene
   EvalUnitVis<MAXC> v{ka, sc, cload(o.units, 0)};
   run_fk(ka, qs, false, v);
   if (!active) return;
-  eval_finish<MAXC>(ka, sc, qs, q, ldq, b, o, v.best, v.bidx, v.sbest, v.sidx, nullptr, nullptr, nullptr, nullptr, 0,
-                    0);
+  eval_finish_whole<MAXC>(ka, sc, qs, q, ldq, b, o, v);
 }
 
 // Split mode: one workgroup per chunk of 64 configurations; its waves take
