@@ -752,11 +752,25 @@ __global__ void __launch_bounds__(256) valid_units_finish_kernel(int32_t* flags,
 }
 
 // FP64 re-decision of the configurations left pending by valid_kernel.
+// With unit_flags (a scene without clouds, after the unit pass) it first
+// turns the flags into valid[b] as valid_units_finish_kernel does, and
+// clears them.
 __global__ void __launch_bounds__(64) valid_recheck_kernel(KinArgs<double> k64, SceneArgs<double> s64,
                                                            const double* q, int64_t ldq, int64_t B, uint8_t* valid,
-                                                           int32_t* n_recheck) {
+                                                           int32_t* n_recheck, int32_t* unit_flags) {
   const int64_t b = gid();
-  if (b >= B || valid[b] != 2) return;
+  if (b >= B) return;
+  if (unit_flags) {
+    const int f = unit_flags[b];
+    unit_flags[b] = 0;
+    const uint8_t v = (f & 1) ? 0 : ((f & 2) ? 2 : 1);
+    if (v != 2) {
+      valid[b] = v;
+      return;
+    }
+  } else if (valid[b] != 2) {
+    return;
+  }
   BatchQ qs{q, ldq, b};
   ValidVis64 v{k64, s64};
   run_fk(k64, qs, k64.base_type == PMP_BASE_FIXED, v);
@@ -1975,10 +1989,13 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                         hipStream_t st) {
   if (block <= 0) block = 256;
   block = slot_block(k32, block);
+  // without clouds the FP64 pass turns the unit flags into answers itself
+  const bool fused = unit_flags && k32.n_units > 0 && n_clouds == 0;
   if (unit_flags && k32.n_units > 0) {
     hipLaunchKernelGGL(valid_units_kernel, dim3(blocks_for(B, 64), (unsigned)k32.n_units), dim3(64),
                        slot_bytes(k32, 64), st, k32, s32, q, ldq, B, band, unit_flags, cw);
-    hipLaunchKernelGGL(valid_units_finish_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, unit_flags, B, valid);
+    if (!fused)
+      hipLaunchKernelGGL(valid_units_finish_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, unit_flags, B, valid);
   } else {
     hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), slot_bytes(k32, block), st, k32, s32, q,
                        ldq, B, band, valid, cw);
@@ -1991,7 +2008,7 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(valid_recheck_kernel, dim3(blocks_for(B, 64)), dim3(64), slot_bytes(k64, 64), st, k64, s64, q,
-                     ldq, B, valid, n_recheck);
+                     ldq, B, valid, n_recheck, fused ? unit_flags : nullptr);
   return hipGetLastError();
 }
 hipError_t launch_edges(const KinArgs<float>& k32, const KinArgs<double>& k64, const SceneArgs<float>& s32,
