@@ -210,6 +210,25 @@ def _traffic_per_launch(kernel):
     return rec.get("bytes_per_launch"), "profiles/traffic.json, this build's sources"
 
 
+def _matrix_cores(kernel):
+    """Matrix-core counters and rocprof duration of `kernel`'s largest grid
+    from the committed table, or None when it was measured on other kernel
+    sources than these."""
+    from plainmp_amd import roofline
+
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    if table.get("_calibration", {}).get("sources_digest") != roofline.sources_digest():
+        return None
+    grids = table.get("_matrix_cores", {}).get(kernel)
+    if not grids:
+        return None
+    return grids[max(grids, key=int)]
+
+
 def _parity_headline(wl, got_valid, got_n):
     """The headline replay's per-edge validity and reference-order checked
     counts against the FP64 oracle (checkMotion, motion_validator.cpp:17-79)
@@ -409,6 +428,19 @@ def _extra(torch, stream, dev):
         "note": "whole validity call (FP32 walk, matrix-core cloud pass, decisions, FP64 re-decisions); the "
                 "cloud kernel's matrix-core utilisation is measured with counters (profiles/)",
     }
+    mc = _matrix_cores("cloud_min_kernel<true>")
+    if mc is not None:
+        flop = mc["mfma_f32_insts"] * roofline.MFMA_F32_16X16X4_FLOP
+        simd_cycles = roofline.N_SIMD * mc["duration_us"] * 1e-6 * roofline.CLOCK_GHZ * 1e9
+        rl["cloud_min_kernel"] = {
+            "workload": "cloud_shelves: 16,384 configs x 90 spheres vs 65,536 points", "bound": "mfma",
+            "achieved": flop / (mc["duration_us"] * 1e-6) / 1e12, "peak": roofline.PEAK_FP32_MFMA_TFLOPS,
+            "unit": "TFLOP/s", "frac": flop / (mc["duration_us"] * 1e-6) / 1e12 / roofline.PEAK_FP32_MFMA_TFLOPS,
+            "mfma_busy": mc["mfma_busy_cycles"] / simd_cycles, "mfma_insts": mc["mfma_f32_insts"],
+            "launch_us": mc["duration_us"],
+            "note": "executed matrix-core work (SQ_INSTS_VALU_MFMA_F32 x 2,048 flop) over the kernel's rocprof "
+                    "duration, and SQ_VALU_MFMA_BUSY_CYCLES over the launch's SIMD-cycles, from the counter "
+                    "runs of this build's sources (profiles/traffic.json); the kernel is bound by its culling"}
     # the same arm and cloud, 4,096 motions between consecutive configurations
     # (box validator 0.05 rad): the cloud variant of the edge kernel
     from plainmp_amd.ompl import BoxMotionValidator

@@ -30,6 +30,9 @@ PEAK_FP32_VALU_TFLOPS = 157.3
 PEAK_FP32_VALU_NONPACKED_TFLOPS = 78.6
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+N_SIMD = 1024  # 256 CUs x 4 SIMDs
+CLOCK_GHZ = 2.4  # peak engine clock
+MFMA_F32_16X16X4_FLOP = 2 * 16 * 16 * 4  # one v_mfma_f32_16x16x4f32
 
 
 def _pair_cost(p) -> int:
@@ -154,6 +157,53 @@ def _kernel_base(name: str) -> str:
     return n[last_sep:cut]
 
 
+def _trace_rows(path):
+    """Rows of a rocprofv3 kernel-trace CSV (a file, or a directory searched
+    for *kernel_trace.csv)."""
+    import csv
+    import glob
+    import os
+
+    if os.path.isdir(path):
+        hits = sorted(glob.glob(os.path.join(path, "**", "*kernel_trace.csv"), recursive=True))
+        if not hits:
+            raise FileNotFoundError(f"no kernel_trace.csv under {path}")
+        path = hits[0]
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def matrix_core_table(mfma_path, trace_path) -> dict:
+    """Per kernel and grid (threads): the FP32 MFMA instructions and matrix-
+    core busy cycles per dispatch (a `--pmc SQ_INSTS_VALU_MFMA_F32
+    SQ_VALU_MFMA_BUSY_CYCLES` run) and the average dispatch duration in µs
+    (a `--kernel-trace` run), for kernels that issue MFMA."""
+    import statistics
+
+    acc = {}
+    for r in _counter_rows(mfma_path):
+        name = r.get("Counter_Name")
+        if name not in ("SQ_INSTS_VALU_MFMA_F32", "SQ_VALU_MFMA_BUSY_CYCLES"):
+            continue
+        key = (_kernel_base(r["Kernel_Name"]), int(r["Grid_Size"]))
+        acc.setdefault(key, {}).setdefault(name, []).append(float(r["Counter_Value"]))
+    dur = {}
+    for r in _trace_rows(trace_path):
+        g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        key = (_kernel_base(r["Kernel_Name"]), g)
+        dur.setdefault(key, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
+    out = {}
+    for (k, g), cs in sorted(acc.items()):
+        insts = statistics.mean(cs.get("SQ_INSTS_VALU_MFMA_F32", [0.0]))
+        if insts <= 0 or (k, g) not in dur:
+            continue
+        out.setdefault(k, {})[str(g)] = {
+            "mfma_f32_insts": round(insts, 1),
+            "mfma_busy_cycles": round(statistics.mean(cs.get("SQ_VALU_MFMA_BUSY_CYCLES", [0.0])), 1),
+            "duration_us": round(statistics.mean(dur[(k, g)]), 3), "launches": len(dur[(k, g)])}
+    return out
+
+
 def traffic_table(fetch_path, write_path, calib_mb: int) -> dict:
     """Per kernel and grid, HBM bytes per launch from a FETCH_SIZE run and a
     WRITE_SIZE run (KiB per dispatch), each scaled by what the same counter
@@ -210,9 +260,13 @@ def main(argv=None):
     ap.add_argument("--fetch", required=True, help="output of the FETCH_SIZE run (CSV or directory)")
     ap.add_argument("--write", required=True, help="output of the WRITE_SIZE run (CSV or directory)")
     ap.add_argument("--calib-mb", type=int, default=1024)
+    ap.add_argument("--mfma", help="output of the MFMA counter run (optional, with --trace)")
+    ap.add_argument("--trace", help="output of the kernel-trace run (optional, with --mfma)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     table = traffic_table(a.fetch, a.write, a.calib_mb)
+    if a.mfma and a.trace:
+        table["_matrix_cores"] = matrix_core_table(a.mfma, a.trace)
     with open(a.out, "w") as f:
         json.dump(table, f, indent=1)
 
