@@ -203,12 +203,13 @@ __global__ void __launch_bounds__(256) cloud_decide_kernel(const float* lo, cons
     atomicOr(&flags[b], 2);
 }
 
-// valid[b] in {0, 1, 2} merged with the cloud flags.
+// valid[b] in {0, 1, 2} merged with the cloud flags; 3 (a configuration
+// the FP64 walk decides whatever the FP32 tests say) is kept.
 __global__ void __launch_bounds__(256) cloud_merge_kernel(uint8_t* valid, const int32_t* flags, int64_t B) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   const uint8_t v = valid[b];
-  if (v == 0) return;
+  if (v == 0 || v == 3) return;
   const int f = flags[b];
   if (f & 1)
     valid[b] = 0;

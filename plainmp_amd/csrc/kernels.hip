@@ -398,6 +398,11 @@ struct ValidVis32T {
     const int n_solid = sc.vrange[kKindCloud];
     // |R v| <= growf |v| for a lane whose |q|^2 is in [n2_lo, n2_hi]
     const float growf = 1.0f + 2.0f * fmaxf(0.0f, fmaxf(nd.n2_hi - 1.0f, 1.0f - nd.n2_lo)) + 1e-6f;
+    // A lane whose |q|^2 is in range has a bounded rotation: with a finite
+    // translation none of its centres can be NaN.  The per-centre test runs
+    // only when some live lane is not such a lane.
+    const bool tfin = fabsf(P.t.x) <= FLT_MAX && fabsf(P.t.y) <= FLT_MAX && fabsf(P.t.z) <= FLT_MAX;
+    const bool nan_test = n_solid > 0 && wave_any(alive() && !(n2 >= nd.n2_lo && n2 <= nd.n2_hi && tfin));
     for (int bi = lo; bi < hi; ++bi) {
       if constexpr (kFirstHit) {
         if (wave_any(definite && !idle)) {
@@ -413,13 +418,20 @@ struct ValidVis32T {
 #pragma unroll
       for (int k = 0; k < kBlockSpheres; ++k) {
         c[k] = P.t;
-        if (k < n) {
-          c[k] = add(mv(R, V3<float>{bk.s[k][0], bk.s[k][1], bk.s[k][2]}), P.t);
-          bad |= is_nan3(c[k]);
-        }
+        if (k < n) c[k] = add(mv(R, V3<float>{bk.s[k][0], bk.s[k][1], bk.s[k][2]}), P.t);
+      }
+      if (nan_test) {
+#pragma unroll
+        for (int k = 0; k < kBlockSpheres; ++k)
+          if (k < n) bad |= is_nan3(c[k]);
       }
       const V3<float> g = add(mv(R, V3<float>{bk.g[0], bk.g[1], bk.g[2]}), P.t);
-      if (bad && n_solid > 0) definite = true;  // NaN centres collide with every primitive
+      // NaN centres (NaN joint values: the reference's comparisons are all
+      // false, so the state collides): left to the FP64 walk
+      if (bad) {
+        uncertain = true;
+        n_inb = 2;
+      }
       if (use_cw && !idle) {
         const int64_t B = cw.B;
 #pragma unroll
@@ -701,7 +713,28 @@ struct PairVis64 {
   __device__ void self(const DevSelfPair&, int, int, V3<double>, V3<double>, double) {}
 };
 
-// valid[b]: 0 collision, 1 free, 2 pending FP64 re-decision.
+// The FP32 walk's band covers its rounding for joint values within these
+// ranges (an angle's FP32 rounding is |q| 2^-24, so within 4 pi it stays
+// below 1e-6 rad; a translation within 16 m, below 1e-6 m).  A state with
+// a value outside them is decided by the FP64 walk.
+constexpr double kFp32AngleRange = 4.0 * M_PI;
+constexpr double kFp32LinearRange = 16.0;
+__device__ __forceinline__ bool fp32_range_col(uint64_t rev_mask, int col, double v) {
+  return fabs(v) <= (((rev_mask >> col) & 1ull) ? kFp32AngleRange : kFp32LinearRange);
+}
+
+// valid[b] of a configuration with a joint value outside the FP32 walk's
+// ranges: the FP64 walk decides it, whatever the FP32 tests said.
+constexpr uint8_t kValidFp64 = 3;
+template <class QS>
+__device__ __forceinline__ bool fp32_range(const KinArgs<float>& ka, const QS& qs) {
+  bool in = true;
+  for (int j = 0; j < ka.n_cols; ++j) in &= fp32_range_col(ka.rev_mask, j, qs.get(j));
+  return in;
+}
+
+// valid[b]: 0 collision, 1 free, 2 pending FP64 re-decision, kValidFp64
+// decided by the FP64 walk only.
 __global__ void __launch_bounds__(256) valid_kernel(KinArgs<float> k32, SceneArgs<float> s32, const double* q,
                                                     int64_t ldq, int64_t B, float band, uint8_t* valid, CloudWork cw) {
   const int64_t b = gid();
@@ -714,7 +747,7 @@ __global__ void __launch_bounds__(256) valid_kernel(KinArgs<float> k32, SceneArg
   v.b = b;
   v.idle = !active;
   run_fk(k32, qs, taylor, v);
-  if (active) valid[b] = v.definite ? 0 : (v.uncertain ? 2 : 1);
+  if (active) valid[b] = !fp32_range(k32, qs) ? kValidFp64 : (v.definite ? 0 : (v.uncertain ? 2 : 1));
 }
 
 // Batches of up to kValidUnitsMaxBatch configurations: a lane per
@@ -739,7 +772,8 @@ __global__ void __launch_bounds__(64) valid_units_kernel(KinArgs<float> k32, Sce
   v.b = b;
   v.idle = !active;
   run_fk(k32, qs, k32.base_type == PMP_BASE_FIXED, v);
-  const int f = (v.definite ? 1 : 0) | (v.uncertain ? 2 : 0);
+  // unit 0 also reports a joint value outside the FP32 walk's ranges (4)
+  const int f = (v.definite ? 1 : 0) | (v.uncertain ? 2 : 0) | (blockIdx.y == 0 && !fp32_range(k32, qs) ? 4 : 0);
   if (active && f != 0) atomicOr(flags + b, f);
 }
 
@@ -747,11 +781,12 @@ __global__ void __launch_bounds__(256) valid_units_finish_kernel(int32_t* flags,
   const int64_t b = gid();
   if (b >= B) return;
   const int f = flags[b];
-  valid[b] = (f & 1) ? 0 : ((f & 2) ? 2 : 1);
+  valid[b] = (f & 4) ? kValidFp64 : ((f & 1) ? 0 : ((f & 2) ? 2 : 1));
   flags[b] = 0;
 }
 
-// FP64 re-decision of the configurations left pending by valid_kernel.
+// FP64 re-decision of the configurations left pending (2) by the FP32 pass
+// and of those it could not decide (kValidFp64).
 // With unit_flags (a scene without clouds, after the unit pass) it first
 // turns the flags into valid[b] as valid_units_finish_kernel does, and
 // clears them.
@@ -763,12 +798,11 @@ __global__ void __launch_bounds__(64) valid_recheck_kernel(KinArgs<double> k64, 
   if (unit_flags) {
     const int f = unit_flags[b];
     unit_flags[b] = 0;
-    const uint8_t v = (f & 1) ? 0 : ((f & 2) ? 2 : 1);
-    if (v != 2) {
-      valid[b] = v;
+    if (!(f & 4) && (f & 3) != 2) {
+      valid[b] = (f & 1) ? 0 : 1;
       return;
     }
-  } else if (valid[b] != 2) {
+  } else if (valid[b] < 2) {
     return;
   }
   BatchQ qs{q, ldq, b};
@@ -888,10 +922,10 @@ __device__ __forceinline__ EdgeQv edge_waypoint(const EdgeArgs& ea, const EdgeEn
 // more than kMaxEdgeStates states).  A pending edge queues its in-band states
 // that can matter (all of them, or those before the first certain collision)
 // and that collision's index; the FP64 pass decides from them.
-// At least three waves per SIMD: the cloud variant sits at the edge of that
-// register budget.
+// Register budget: four waves per SIMD without clouds (the by-index grid
+// of the autotuned schedule holds four per SIMD), three with them.
 template <bool kCloud>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, KinArgs<double> k64,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kCloud ? 3 : 4))) edges_kernel(KinArgs<float> k32, SceneArgs<float> s32, KinArgs<double> k64,
                                                     SceneArgs<double> s64, EdgeArgs ea, float band) {
   const int lane = threadIdx.x & 63;
   const bool taylor = k32.base_type == PMP_BASE_FIXED;
@@ -916,6 +950,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) e
   for (; e < ea.E; e = by_index ? ea.E : __builtin_amdgcn_readfirstlane(atomicAdd(ea.counter, lane == 0 ? 1 : 0))) {
     const EdgeEnds ends = load_edge_ends(ea, k32.n_cols, e, lane);
     const double ratio = edge_step_ratio(ea, k32.n_cols, ends);
+    // endpoints outside the FP32 walk's ranges (so are its waypoints: they
+    // lie between them): the FP64 pass walks every state
+    bool far = false;
+    for (int i = 0; i < k32.n_cols; ++i)
+      far |= !(fp32_range_col(k32.rev_mask, i, lane_f64(ends.a, i)) &&
+               fp32_range_col(k32.rev_mask, i, lane_f64(ends.b, i)));
     int state = 1;
     int64_t checked = 0;
     int n_pend = 0;        // queued in-band states, -1 once more than kPendStates
@@ -924,6 +964,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) e
       if (!(ratio > 0.0) || 1.0 / ratio > kMaxEdgeStates) {  // NaN states, or more states than an edge may hold
         state = 0;
         checked = -1;
+      } else if (far) {
+        state = 2;
+        n_pend = -1;
       } else {
         const int64_t n_test = (int64_t)floor(1.0 / ratio) + 2;
         const int64_t n_wp = n_test - 1;  // s2 plus the interior waypoints

@@ -50,23 +50,47 @@ def test_valid_both_batch_paths(n_configs):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("n_configs", [1024, 20480])
-def test_valid_nan_configurations(n_configs):
+@pytest.mark.parametrize("bad", [np.nan, np.inf, 1e30, 20.0])
+@pytest.mark.parametrize("n_configs", [1024, 20480, 81920])
+def test_valid_nan_configurations(n_configs, bad):
     """A NaN joint value makes every SDF comparison false, so the reference
     reports the configuration invalid (AABB pre-test not outside, is_outside
-    false).  Both the unit pass and the one-lane pass agree with the oracle,
-    NaN rows included."""
+    false); an infinite one too.  A huge finite one (1e30 rad) overflows the
+    FP32 walk to NaN but not the reference's FP64 one, which finds the robot
+    far away: the FP32 NaN goes to the FP64 walk.  Both the unit pass (up to
+    65,536 configurations) and the one-lane pass agree with the oracle."""
     wl = synth.workload("plain_two_bars", scale=n_configs / 1024)
     Q = wl.Q.copy()
     rng = np.random.default_rng(4)
     rows = rng.choice(len(Q), 37, replace=False)
-    Q[rows, rng.integers(0, Q.shape[1], size=37)] = np.nan
+    Q[rows, rng.integers(0, Q.shape[1], size=37)] = bad
     oc, pc = oracle_cst(wl), product_cst(wl)
     ref = oc.is_valid_batch(Q, 8).astype(bool)
     got, n_re = pc.is_valid_batch_stats(Q)
-    _report("valid_nan", B=len(Q), nan_rows=len(rows), rechecked=n_re)
-    assert not ref[rows].any()
+    _report("valid_bad_values", B=len(Q), bad=bad, rows=len(rows), ref_valid_rows=int(ref[rows].sum()),
+            rechecked=n_re)
+    if not np.isfinite(bad):
+        assert not ref[rows].any()
     assert np.array_equal(got, ref)
+
+
+def test_edges_beyond_the_fp32_ranges():
+    """Edges whose endpoints hold joint values beyond the FP32 walk's ranges
+    (angles past 4 pi) are walked in FP64 state by state: validity and the
+    reference-order checked counts match the oracle (motion_validator.cpp:17-79)."""
+    from plainmp_amd.ompl import BoxMotionValidator
+
+    wl = synth.workload("edges_mixed16", scale=1 / 16)  # 256 edges
+    S1, S2 = wl.S1.copy(), wl.S2.copy()
+    far = np.arange(0, len(S1), 3)  # every third edge: one joint shifted by 8 turns at both ends
+    S1[far, 2] += 16 * np.pi
+    S2[far, 2] += 16 * np.pi
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    ref, n_ref = oc.check_motions(S1, S2, True, list(wl.box_width), 0.0, 8)
+    got, n_got = BoxMotionValidator(pc, wl.box_width).check_motions(S1, S2)
+    _report("edges_far", E=len(S1), far=len(far), valid_far=float(ref[far].mean()))
+    assert np.array_equal(got, ref.astype(bool))
+    assert np.array_equal(n_got, n_ref)
 
 
 def test_clearance_and_centers_plain_two_bars():
