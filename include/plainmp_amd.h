@@ -144,6 +144,8 @@ typedef struct pmp_schedule {
 typedef struct pmp_context pmp_context;
 
 const char* pmp_last_error(void);
+/* 2: pmp_schedule gained pair_shares (a caller built against version 1
+   passes a shorter struct to pmp_set_schedule). */
 int32_t pmp_abi_version(void);
 
 pmp_status pmp_create(int32_t device, pmp_context** out);
