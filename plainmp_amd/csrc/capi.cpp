@@ -1115,7 +1115,11 @@ pmp_status pmp_is_valid_batch(pmp_context* ctx, const double* q, int64_t ldq, in
         cw.cap2[oc] = w + 4 * Q * oc;
         cw.rr[oc] = cw.cap2[oc] + Q;
       }
+      // the cloud pass's flags are zero on entry and left zero: cleared when
+      // the buffer is new
+      const bool fresh_flags = ctx->ws_flags.bytes < sizeof(int32_t) * (size_t)B;
       flags = (int32_t*)ctx->ws_flags.get(sizeof(int32_t) * B);
+      if (fresh_flags) PMP_HIP(launch_fill_i32(flags, 0, (int64_t)(ctx->ws_flags.bytes / sizeof(int32_t)), st(ctx)));
     }
     // batches up to kValidUnitsMaxBatch: a lane per (configuration, unit);
     // the flags start at zero (cleared when the buffer is new) and the pass
@@ -1167,9 +1171,10 @@ pmp_status pmp_is_valid_batch_noalloc(pmp_context* ctx, const double* q, int64_t
                 "batch needs " + std::to_string(need) + " bytes of scratch, more than the " +
                     std::to_string(kNoAllocScratchBytes) +
                     " bytes pmp_is_valid_batch_noalloc has; split the batch or call pmp_is_valid_batch");
-    // the scratch keeps the previous call's contents: the unit pass needs its
-    // flags at zero (a fill kernel, not a memset node)
+    // the scratch keeps the previous call's contents: the unit pass and the
+    // cloud pass need their flags at zero (fill kernels, not memset nodes)
     if (unit_flags) PMP_HIP(launch_fill_i32(unit_flags, 0, B, st(ctx)));
+    if (flags) PMP_HIP(launch_fill_i32(flags, 0, B, st(ctx)));
     PMP_HIP(launch_valid(ctx->k32, ctx->k64, ctx->s32, ctx->s64, q, ldq, B, ctx->band, valid, n_recheck,
                          ctx->sched.block, cw, ctx->clouds.data(), ctx->n_clouds, flags, unit_flags, st(ctx)));
   });

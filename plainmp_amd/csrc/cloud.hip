@@ -38,15 +38,25 @@ __device__ __forceinline__ float wave_max16(float v) {
   return v;
 }
 
+// Decision arguments of the bounds instantiation: rr[q] = r + rc of a live
+// query (< 0 when not queried), the band, and the per-configuration flag
+// word the decisions are ORed into.
+struct CloudDecide {
+  const float* rr;
+  float band;
+  int32_t* flags;
+};
+
 // min |c - p|^2 over the cloud's tiles for the queries q = e * B + b
 // (e < Q / B spheres, b < B configurations) whose cap2[q] >= 0; centres in
 // cen[(e * 3 + k) * B + b].  Writes out_d2[q] = min squared distance
-// (FLT_MAX when no point lies within the cap; a wave whose 16 queries are
-// all unqueried or skipped writes nothing).  With kBounds, the minimum is
-// bracketed instead: out_d2[q] = min over tiles of (d2 + err) and out_lo[q] =
-// min over tiles of (d2 - err), err = kCloudGemmErr (|c'|^2 + rho^2) per
-// tile, so the exact minimum lies in [out_lo, out_d2].  `skip` (optional, per
-// configuration b): configurations already decided (0) are not evaluated.
+// (FLT_MAX when no point lies within the cap).  With kBounds (validity) the
+// minimum is bracketed instead, between min over tiles of (d2 - err) and of
+// (d2 + err), err = kCloudGemmErr (|c'|^2 + rho^2) per tile, and each live
+// query is decided from its bracket in place, ORing into dec.flags[b]; a wave
+// whose 16 queries are all unqueried or skipped does nothing.  `skip`
+// (optional, per configuration b): configurations already decided (0) are
+// not evaluated.
 //
 // A wave takes 16 consecutive spheres of one configuration (spheres are
 // ordered by link, so they are close together; consecutive waves take the
@@ -59,7 +69,7 @@ template <bool kBounds>
 __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, const float* tileB,
                                                         const CloudSuper* supers, int super_begin, int n_super,
                                                         const float* cen, const float* cap2, int64_t B, int64_t Q,
-                                                        const uint8_t* skip, float* out_d2, float* out_lo) {
+                                                        const uint8_t* skip, float* out_d2, CloudDecide dec) {
   const int lane = threadIdx.x & 63;
   const int64_t S = Q / B;  // spheres per configuration
   const int64_t groups = (S + 15) / 16;
@@ -171,46 +181,50 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
       best[r] = fminf(best[r], __shfl_xor(best[r], s, 64));
       if constexpr (kBounds) lo[r] = fminf(lo[r], __shfl_xor(lo[r], s, 64));
     }
-  if (i == 0) {
+  if constexpr (kBounds) {
+    // Collision decision per live query: collide iff sqrt(d2) - rc <= r,
+    // i.e. d2 <= (r + rc)^2 (CloudSDF::is_outside, primitive_sdf.hpp:597-603),
+    // where the exact minimum lies in [lo, best]: certain outside the band
+    // around R^2, bit 0 (a collision) or nothing; bit 1 (in the band)
+    // otherwise.  The wave's 16 queries are of one configuration: one OR.
+    int bits = 0;
+    if (i == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t e = e0 + 4 * k + r;
+        if (e < S) {
+          const float R = dec.rr[e * B + b];
+          if (R >= 0.f) {
+            const float R2 = R * R, w = dec.band * (2.f * R + dec.band);
+            if (best[r] - R2 < -w)
+              bits |= 1;
+            else if (!(lo[r] - R2 > w))
+              bits |= 2;
+          }
+        }
+      }
+    }
+    const int f = (__ballot(bits & 1) ? 1 : 0) | (__ballot(bits & 2) ? 2 : 0);
+    if (lane == 0 && f) atomicOr(dec.flags + b, f);
+  } else if (i == 0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t e = e0 + 4 * k + r;
-      if (e < S) {
-        out_d2[e * B + b] = best[r];
-        if constexpr (kBounds) out_lo[e * B + b] = lo[r];
-      }
+      if (e < S) out_d2[e * B + b] = best[r];
     }
   }
 }
 
-// Collision decision per cloud query: collide iff sqrt(d2) - rc <= r, i.e.
-// d2 <= (r + rc)^2 (CloudSDF::is_outside, primitive_sdf.hpp:597-603), where
-// the exact minimum d2 lies in [lo[q], hi[q]] (the FP32 pass's bracket).
-// rr[q] = r + rc of the query, or < 0 when not queried.  flags[b]: bit 0 a
-// collision outside the band, bit 1 a decision inside the band (the bracket
-// reaches within band (2R + band) of R^2).
-__global__ void __launch_bounds__(256) cloud_decide_kernel(const float* lo, const float* hi, const float* rr,
-                                                           int64_t B, int64_t Q, float band, int32_t* flags) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= Q) return;
-  const float R = rr[q];
-  if (R < 0.f) return;
-  const float R2 = R * R, w = band * (2.f * R + band);
-  const int64_t b = q % B;
-  if (hi[q] - R2 < -w)
-    atomicOr(&flags[b], 1);
-  else if (!(lo[q] - R2 > w))
-    atomicOr(&flags[b], 2);
-}
-
 // valid[b] in {0, 1, 2} merged with the cloud flags; 3 (a configuration
-// the FP64 walk decides whatever the FP32 tests say) is kept.
-__global__ void __launch_bounds__(256) cloud_merge_kernel(uint8_t* valid, const int32_t* flags, int64_t B) {
+// the FP64 walk decides whatever the FP32 tests say) is kept.  The flags are
+// cleared for the next cloud or launch (they are zero on entry).
+__global__ void __launch_bounds__(256) cloud_merge_kernel(uint8_t* valid, int32_t* flags, int64_t B) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
+  const int f = flags[b];
+  flags[b] = 0;
   const uint8_t v = valid[b];
   if (v == 0 || v == 3) return;
-  const int f = flags[b];
   if (f & 1)
     valid[b] = 0;
   else if ((f & 2) || v == 2)
@@ -246,13 +260,9 @@ hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const 
                               const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
                               int64_t Q, float band, uint8_t* valid, int32_t* flags, hipStream_t st) {
   auto blocks = [](int64_t n, int t) { return (unsigned)((n + t - 1) / t); };
-  float* hi = const_cast<float*>(rr) + Q;
-  float* lo = hi + Q;
-  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, flags, 0, B);
   const int64_t waves = (Q / B + 15) / 16 * B;  // 16 spheres of one configuration per wave
   hipLaunchKernelGGL(cloud_min_kernel<true>, dim3(blocks(waves * 64, 256)), dim3(256), 0, st, tiles, tileB, supers,
-                     cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, hi, lo);
-  hipLaunchKernelGGL(cloud_decide_kernel, dim3(blocks(Q, 256)), dim3(256), 0, st, lo, hi, rr, B, Q, band, flags);
+                     cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, (float*)nullptr, CloudDecide{rr, band, flags});
   hipLaunchKernelGGL(cloud_merge_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, valid, flags, B);
   return hipGetLastError();
 }
@@ -263,7 +273,7 @@ hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, const Cl
   const int64_t waves = (Q / B + 15) / 16 * B;  // 16 spheres of one configuration per wave
   hipLaunchKernelGGL(cloud_min_kernel<false>, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0, st, tiles,
                      tileB, supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2,
-                     (float*)nullptr);
+                     CloudDecide{nullptr, 0.f, nullptr});
   return hipGetLastError();
 }
 
