@@ -45,6 +45,7 @@ struct CloudDecide {
   const float* rr;
   float band;
   int32_t* flags;
+  const int32_t* unit_flags;  // the unit pass's words instead of `skip`, when not yet turned into valid[]
 };
 
 // min |c - p|^2 over the cloud's tiles for the queries q = e * B + b
@@ -83,7 +84,9 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
   if (w >= groups * B) return;  // wave-uniform
   const int64_t b = w % B, e0 = (w / B) * 16;
   const int i = lane & 15, k = lane >> 4;
-  const bool skipped = skip && skip[b] == 0;
+  // skipped: already decided by a certain collision (unit words: bit 0
+  // without bit 2)
+  const bool skipped = kBounds && dec.unit_flags ? (dec.unit_flags[b] & 5) == 1 : (skip && skip[b] == 0);
   // the A-row query of this lane; a wave none of whose queries is live ends
   // here without writing: the decisions read a query's bracket only when the
   // query is live (rr >= 0, written with cap2), and a skipped configuration
@@ -216,14 +219,25 @@ __global__ void __launch_bounds__(256) cloud_min_kernel(const CloudTile* tiles, 
 }
 
 // valid[b] in {0, 1, 2} merged with the cloud flags; 3 (a configuration
-// the FP64 walk decides whatever the FP32 tests say) is kept.  The flags are
+// the FP64 walk decides whatever the FP32 tests say) is kept.  With
+// unit_flags, valid[b] is first formed from the unit pass's word as
+// valid_units_finish_kernel does (and the word cleared).  The flags are
 // cleared for the next cloud or launch (they are zero on entry).
-__global__ void __launch_bounds__(256) cloud_merge_kernel(uint8_t* valid, int32_t* flags, int64_t B) {
+__global__ void __launch_bounds__(256) cloud_merge_kernel(uint8_t* valid, int32_t* flags, int32_t* unit_flags,
+                                                          int64_t B) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   const int f = flags[b];
   flags[b] = 0;
-  const uint8_t v = valid[b];
+  uint8_t v;
+  if (unit_flags) {
+    const int u = unit_flags[b];
+    unit_flags[b] = 0;
+    v = (u & 4) ? 3 : ((u & 1) ? 0 : ((u & 2) ? 2 : 1));
+    valid[b] = v;
+  } else {
+    v = valid[b];
+  }
   if (v == 0 || v == 3) return;
   if (f & 1)
     valid[b] = 0;
@@ -258,12 +272,14 @@ hipError_t launch_cloud_clearance(float* clearance, const float* d2, const float
 // there: upper ends, then lower ends).
 hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                               const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
-                              int64_t Q, float band, uint8_t* valid, int32_t* flags, hipStream_t st) {
+                              int64_t Q, float band, uint8_t* valid, int32_t* flags, int32_t* unit_flags,
+                              hipStream_t st) {
   auto blocks = [](int64_t n, int t) { return (unsigned)((n + t - 1) / t); };
   const int64_t waves = (Q / B + 15) / 16 * B;  // 16 spheres of one configuration per wave
   hipLaunchKernelGGL(cloud_min_kernel<true>, dim3(blocks(waves * 64, 256)), dim3(256), 0, st, tiles, tileB, supers,
-                     cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, (float*)nullptr, CloudDecide{rr, band, flags});
-  hipLaunchKernelGGL(cloud_merge_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, valid, flags, B);
+                     cr.super_begin, cr.n_super, cen, cap2, B, Q, valid, (float*)nullptr,
+                     CloudDecide{rr, band, flags, unit_flags});
+  hipLaunchKernelGGL(cloud_merge_kernel, dim3(blocks(B, 256)), dim3(256), 0, st, valid, flags, unit_flags, B);
   return hipGetLastError();
 }
 
@@ -273,7 +289,7 @@ hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, const Cl
   const int64_t waves = (Q / B + 15) / 16 * B;  // 16 spheres of one configuration per wave
   hipLaunchKernelGGL(cloud_min_kernel<false>, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0, st, tiles,
                      tileB, supers, cr.super_begin, cr.n_super, cen, cap2, B, Q, (const uint8_t*)nullptr, out_d2,
-                     CloudDecide{nullptr, 0.f, nullptr});
+                     CloudDecide{nullptr, 0.f, nullptr, nullptr});
   return hipGetLastError();
 }
 

@@ -2032,22 +2032,25 @@ hipError_t launch_valid(const KinArgs<float>& k32, const KinArgs<double>& k64, c
                         hipStream_t st) {
   if (block <= 0) block = 256;
   block = slot_block(k32, block);
-  // without clouds the FP64 pass turns the unit flags into answers itself
-  const bool fused = unit_flags && k32.n_units > 0 && n_clouds == 0;
-  if (unit_flags && k32.n_units > 0) {
+  // the unit flags become answers in the FP64 pass (no clouds) or in the
+  // first cloud's merge
+  const int64_t Q = (int64_t)k32.n_ext_spheres * B;
+  const bool units = unit_flags && k32.n_units > 0;
+  const bool fused = units && n_clouds == 0;
+  const bool merged = units && n_clouds > 0 && Q > 0;
+  if (units) {
     hipLaunchKernelGGL(valid_units_kernel, dim3(blocks_for(B, 64), (unsigned)k32.n_units), dim3(64),
                        slot_bytes(k32, 64), st, k32, s32, q, ldq, B, band, unit_flags, cw);
-    if (!fused)
+    if (!fused && !merged)
       hipLaunchKernelGGL(valid_units_finish_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, st, unit_flags, B, valid);
   } else {
     hipLaunchKernelGGL(valid_kernel, dim3(blocks_for(B, block)), dim3(block), slot_bytes(k32, block), st, k32, s32, q,
                        ldq, B, band, valid, cw);
   }
   // sphere-vs-cloud queries on the matrix cores, one pass per cloud
-  const int64_t Q = (int64_t)k32.n_ext_spheres * B;
   for (int oc = 0; oc < n_clouds && Q > 0; ++oc) {
     hipError_t e = launch_cloud_valid(s32.tiles, s32.tileB, s32.supers, clouds[oc], cw.cen, cw.cap2[oc], cw.rr[oc],
-                                      B, Q, band, valid, flags, st);
+                                      B, Q, band, valid, flags, merged && oc == 0 ? unit_flags : nullptr, st);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(valid_recheck_kernel, dim3(blocks_for(B, 64)), dim3(64), slot_bytes(k64, 64), st, k64, s64, q,

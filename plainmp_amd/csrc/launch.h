@@ -91,9 +91,12 @@ constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
 hipError_t noalloc_scratch_init(int device);
 void* noalloc_scratch(int device);
 // cloud.hip
+// unit_flags (optional): valid[] is formed from the unit pass's words by
+// this pass (and the words cleared), instead of by valid_units_finish_kernel.
 hipError_t launch_cloud_valid(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                               const CloudRange& cr, const float* cen, const float* cap2, const float* rr, int64_t B,
-                              int64_t Q, float band, uint8_t* valid, int32_t* flags, hipStream_t st);
+                              int64_t Q, float band, uint8_t* valid, int32_t* flags, int32_t* unit_flags,
+                              hipStream_t st);
 hipError_t launch_cloud_min(const CloudTile* tiles, const float* tileB, const CloudSuper* supers,
                             const CloudRange& cr, const float* cen, const float* cap2, int64_t B, int64_t Q,
                             float* out_d2, hipStream_t st);

@@ -135,7 +135,8 @@ hipError_t launch_valid(const KinArgs<float>&, const KinArgs<double>&, const Sce
   return hipSuccess;
 }
 hipError_t launch_cloud_valid(const CloudTile*, const float*, const CloudSuper*, const CloudRange&, const float*,
-                              const float*, const float*, int64_t, int64_t, float, uint8_t*, int32_t*, hipStream_t) {
+                              const float*, const float*, int64_t, int64_t, float, uint8_t*, int32_t*, int32_t*,
+                              hipStream_t) {
   return hipSuccess;
 }
 hipError_t launch_cloud_min(const CloudTile*, const float*, const CloudSuper*, const CloudRange&, const float*,
