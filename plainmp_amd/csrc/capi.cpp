@@ -1403,8 +1403,10 @@ pmp_status pmp_pair_clearance_vjp(pmp_context* ctx, const double* q, int64_t ldq
     PMP_REQUIRE(B == 0 || (grad_dist && grad_q), "null argument");
     if (B == 0) return;
     set_device(ctx);
-    float* part = (float*)ctx->ws_vjp.get(sizeof(float) * std::max<int64_t>(1, (int64_t)ctx->k32.n_items * ctx->n_cols * B));
-    PMP_HIP(launch_vjp(ctx->k32, ctx->s32, q, ldq, B, grad_dist, part, grad_q, st(ctx)));
+    const int shares = vjp_shares(ctx->k32, ctx->s32.n_prims, B, ctx->sched.pair_shares);
+    float* part = (float*)ctx->ws_vjp.get(
+        sizeof(float) * std::max<int64_t>(1, (int64_t)ctx->k32.n_items * shares * ctx->n_cols * B));
+    PMP_HIP(launch_vjp(ctx->k32, ctx->s32, q, ldq, B, grad_dist, part, grad_q, ctx->sched.pair_shares, st(ctx)));
   });
 }
 

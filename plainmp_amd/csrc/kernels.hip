@@ -1871,6 +1871,7 @@ struct VjpVis {
   const float* gd;
   int64_t B, b;
   DevSphereItem it;
+  int p_lo, p_hi;  // primitives of this lane's share
   float acc[MAXC];
   bool done = false;
   __device__ bool alive() const { return !done; }
@@ -1887,7 +1888,7 @@ struct VjpVis {
     const uint64_t mask = cload(ka.nodes, m.node).col_mask;
     const int NP = sc.n_prims;
     V3<float> g{0.f, 0.f, 0.f};
-    for (int p = 0; p < NP; ++p) {
+    for (int p = p_lo; p < p_hi; ++p) {
       const DevPrim<float> P = prim_at(sc, p);
       const float w = gd[((int64_t)m.out * NP + p) * B + b];
       V3<float> gp;
@@ -1913,23 +1914,30 @@ __global__ void __launch_bounds__(256) vjp_kernel(KinArgs<float> ka, SceneArgs<f
   const bool active = b < B;
   const int64_t bb = active ? b : B - 1;
   BatchQ qs{q, ldq, bb};
-  VjpVis<MAXC> v{ka, sc, {}, gd, B, bb, cload(ka.items, blockIdx.y)};
+  // grid z splits the primitives into shares, as in pair_kernel; each
+  // (item, share) writes its own partial column sums
+  const int p_per = (sc.n_prims + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int p_lo = (int)blockIdx.z * p_per, p_hi = min(sc.n_prims, p_lo + p_per);
+  VjpVis<MAXC> v{ka, sc, {}, gd, B, bb, cload(ka.items, blockIdx.y), p_lo, p_hi};
 #pragma unroll
   for (int j = 0; j < MAXC; ++j) v.acc[j] = 0.f;
   run_fk(ka, qs, false, v);
   if (!active) return;
   const int D = ka.n_cols;
+  const int64_t part_row = (int64_t)blockIdx.y * gridDim.z + blockIdx.z;
 #pragma unroll
   for (int j = 0; j < MAXC; ++j)
-    if (j < D) part[((int64_t)blockIdx.y * D + j) * B + b] = v.acc[j];
+    if (j < D) part[(part_row * D + j) * B + b] = v.acc[j];
 }
 
-__global__ void __launch_bounds__(256) vjp_reduce_kernel(const float* part, int n_items, int D, int64_t B,
+// Sum of the n_parts partial column sums (items x shares, in that order:
+// deterministic).
+__global__ void __launch_bounds__(256) vjp_reduce_kernel(const float* part, int n_parts, int D, int64_t B,
                                                          float* grad_q) {
   const int64_t i = gid();  // j * B + b
   if (i >= (int64_t)D * B) return;
   float s = 0.f;
-  for (int it = 0; it < n_items; ++it) s += part[(int64_t)it * D * B + i];
+  for (int it = 0; it < n_parts; ++it) s += part[(int64_t)it * D * B + i];
   grad_q[i] = s;
 }
 
@@ -2269,15 +2277,24 @@ hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, co
     PMP_DISPATCH_MAXC_CLOUD(ka.n_cols, pair_kernel, false, grid, dim3(bk), lds, st, ka, sc, q, ldq, B, dist, grad);
   return hipGetLastError();
 }
+int vjp_shares(const KinArgs<float>& ka, int n_prims, int64_t B, int shares) {
+  // the schedule's shares, or about three waves per SIMD (one round at the
+  // adjoint's register use); at least two primitives a share
+  const int64_t waves = (B + 63) / 64 * ka.n_items;
+  const int64_t want = shares > 0 ? shares : (waves > 0 ? 3072 / waves : 1), most = (n_prims + 1) / 2;
+  const int64_t z = want < most ? want : most;
+  return z < 1 ? 1 : (int)z;
+}
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
-                      const float* gd, float* part, float* grad_q, hipStream_t st) {
+                      const float* gd, float* part, float* grad_q, int shares, hipStream_t st) {
+  const int nz = vjp_shares(ka, sc.n_prims, B, shares);
   if (ka.n_items > 0) {
     const int bk = slot_block(ka, 256);
-    PMP_DISPATCH_MAXC(ka.n_cols, vjp_kernel, dim3(blocks_for(B, bk), (unsigned)ka.n_items), dim3(bk),
+    PMP_DISPATCH_MAXC(ka.n_cols, vjp_kernel, dim3(blocks_for(B, bk), (unsigned)ka.n_items, (unsigned)nz), dim3(bk),
                       slot_bytes(ka, bk), st, ka, sc, q, ldq, B, gd, part);
   }
   hipLaunchKernelGGL(vjp_reduce_kernel, dim3(blocks_for((int64_t)ka.n_cols * B, 256)), dim3(256), 0, st, part,
-                     ka.n_items, ka.n_cols, B, grad_q);
+                     ka.n_items * nz, ka.n_cols, B, grad_q);
   return hipGetLastError();
 }
 hipError_t launch_prims_eval(const DevPrim<double>* prims, int n, const double* cloud_xyz, const double* pts,

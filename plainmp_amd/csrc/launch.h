@@ -122,9 +122,12 @@ hipError_t launch_eval_units(const KinArgs<float>& ka, const SceneArgs<float>& s
 // shares: primitive shares (grid z), 0 = about four waves per SIMD
 hipError_t launch_pairs(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq,
                         int64_t B, float* dist, float* grad, int shares, hipStream_t st);
-// part: workspace of n_items * n_cols * B floats
+// Primitive shares (grid z) of the adjoint launch: the schedule's (shares >
+// 0) or about three waves per SIMD.
+int vjp_shares(const KinArgs<float>& ka, int n_prims, int64_t B, int shares);
+// part: workspace of n_items * vjp_shares(...) * n_cols * B floats
 hipError_t launch_vjp(const KinArgs<float>& ka, const SceneArgs<float>& sc, const double* q, int64_t ldq, int64_t B,
-                      const float* gd, float* part, float* grad_q, hipStream_t st);
+                      const float* gd, float* part, float* grad_q, int shares, hipStream_t st);
 hipError_t launch_prims_eval(const DevPrim<double>* prims, int n, const double* cloud_xyz, const double* pts,
                              int64_t N, int mode, double radius, double* out, hipStream_t st);
 hipError_t launch_nearest(const double* pts, int64_t N, const double* qry, int64_t M, double* sqd, int32_t* idx,

@@ -175,8 +175,9 @@ hipError_t launch_prims_eval(const DevPrim<double>*, int, const double*, const d
   return hipSuccess;
 }
 hipError_t launch_sqp_ineq(const SqpArgs&, hipStream_t) { return hipSuccess; }
+int vjp_shares(const KinArgs<float>&, int, int64_t, int) { return 1; }
 hipError_t launch_vjp(const KinArgs<float>&, const SceneArgs<float>&, const double*, int64_t, int64_t, const float*,
-                      float*, float*, hipStream_t) {
+                      float*, float*, int, hipStream_t) {
   return hipSuccess;
 }
 hipError_t launch_cloud_build(const double*, int64_t, int32_t, CloudTile*, CloudTile64*, float*, double*, CloudSuper*,
