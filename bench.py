@@ -452,12 +452,15 @@ def _extra(torch, stream, dev):
         s1, s2 = _soa(torch, S1), _soa(torch, S2)
         ev = torch.empty(E, dtype=torch.uint8, device=dev)
         en = torch.empty(E, dtype=torch.int32, device=dev)
+    stream.synchronize()
+    _, tune_cloud = autotune.tune_edges(mv, cst, s1.data_ptr(), s2.data_ptr(), E, ev.data_ptr(), en.data_ptr(), sh,
+                                        key="cloud_shelves_edges", repeats=3, cache=None)
     ms = _time_stream(torch, stream, lambda: mv._dev_check_motions(s1.data_ptr(), s2.data_ptr(), E, E, ev.data_ptr(),
                                                                    en.data_ptr(), sh), steps=10)
     checked = int(en.sum().item())
     res["cloud_shelves_edges"] = {"edges": E, "ms": ms, "edges_per_s": E / (ms * 1e-3),
                                   "waypoints_checked": checked, "configs_per_s": checked / (ms * 1e-3),
-                                  "valid_frac": float(ev.float().mean().item())}
+                                  "valid_frac": float(ev.float().mean().item()), "autotune": tune_cloud}
     # the cloud's spatial structure built on the device from device-resident
     # points (bounds, Morton sort, tiles, super-tiles): the shelf cloud and a
     # 1,048,576-point cloud
