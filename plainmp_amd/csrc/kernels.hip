@@ -2131,64 +2131,94 @@ __global__ void fill_counts_kernel(int32_t* p, int32_t v, int64_t n) {
 // motion-step box pairs q_t - q_{t+1} + w, q_{t+1} - q_t + w of every segment.
 // Column (t, d) holds, in row order, the C collision entries J + 1e-12, the
 // (-1, +1) of segment t - 1 and the (+1, -1) of segment t, so its position in
-// the compressed arrays follows from (t, d) alone.  Threads past the columns
-// close the column pointers of empty trajectories.
-__global__ void __launch_bounds__(256) sqp_ineq_kernel(SqpArgs a) {
+// the compressed arrays follows from (t, d) alone.  Consecutive columns own
+// consecutive ranges of the value and row arrays (trajectories follow each
+// other there too), so a workgroup stages its columns' entries in LDS and
+// writes its whole range with consecutive lanes on consecutive entries.
+// Threads past the columns close the column pointers of empty trajectories.
+constexpr int kSqpBlock = 256;
+__global__ void __launch_bounds__(kSqpBlock) sqp_ineq_kernel(SqpArgs a) {
+  HIP_DYNAMIC_SHARED(double, sqp_lds)
+  __shared__ int64_t range[2];  // the workgroup's first entry and the end of its last
   const int64_t i = gid();
   const int D = a.D, C = a.C;
-  if (i >= a.N * D) {
-    const int64_t tr = i - a.N * D;
-    if (tr < a.n_traj && a.offsets[tr + 1] == a.offsets[tr]) a.colptr[D * a.offsets[tr] + tr] = 0;
-    return;
-  }
-  const int64_t n = i / D;
-  const int d = (int)(i % D);
-  int lo = 0, hi = a.n_traj;  // offsets[lo] <= n < offsets[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (a.offsets[mid] <= n) lo = mid;
-    else hi = mid;
-  }
-  const int64_t b0 = a.offsets[lo], T = a.offsets[lo + 1] - b0, t = n - b0;
+  double* sv = sqp_lds;
+  int32_t* sr = reinterpret_cast<int32_t*>(sqp_lds + (int64_t)kSqpBlock * (C + 4));
+  const int64_t i_first = (int64_t)blockIdx.x * kSqpBlock, n_cols = a.N * D;
+  const int64_t i_last = (i_first + kSqpBlock < n_cols ? i_first + kSqpBlock : n_cols) - 1;
+  const bool col = i < n_cols;
+  // the column: trajectory lo, waypoint t of T, joint d; its entries start at g
+  int lo = 0;
+  int64_t n = 0, b0 = 0, T = 0, t = 0, before = 0, g = 0;
+  int d = 0, et = 0;
   const bool box = a.msbox != nullptr;
-  const int64_t j = t * D + d;
-  // box entries of the columns of the waypoints before t, and of each column of t
-  const int64_t e_prev = box ? 2 * (t > 0 ? t - 1 : 0) + 2 * (t < T - 1 ? t : T - 1) : 0;
-  const int et = box ? 2 * (t > 0) + 2 * (t < T - 1) : 0;
-  const int64_t before = j * C + (int64_t)D * e_prev + (int64_t)d * et;
-  int32_t* cp = a.colptr + D * b0 + lo;
-  cp[j] = (int32_t)before;
-  if (j == D * T - 1) cp[D * T] = (int32_t)(before + C + et);
-  int32_t* ri = a.rowind + a.nnz_off[lo] + before;
-  double* va = a.values + a.nnz_off[lo] + before;
-  for (int c = 0; c < C; ++c) {
-    ri[c] = (int32_t)(t * C + c);
-    va[c] = (double)a.jac[((int64_t)c * D + d) * a.N + n] + 1e-12;
+  if (col) {
+    n = i / D;
+    d = (int)(i % D);
+    int hi = a.n_traj;  // offsets[lo] <= n < offsets[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.offsets[mid] <= n) lo = mid;
+      else hi = mid;
+    }
+    b0 = a.offsets[lo];
+    T = a.offsets[lo + 1] - b0;
+    t = n - b0;
+    const int64_t j = t * D + d;
+    // box entries of the columns of the waypoints before t, and of each column of t
+    const int64_t e_prev = box ? 2 * (t > 0 ? t - 1 : 0) + 2 * (t < T - 1 ? t : T - 1) : 0;
+    et = box ? 2 * (t > 0) + 2 * (t < T - 1) : 0;
+    before = j * C + (int64_t)D * e_prev + (int64_t)d * et;
+    int32_t* cp = a.colptr + D * b0 + lo;
+    cp[j] = (int32_t)before;
+    if (j == D * T - 1) cp[D * T] = (int32_t)(before + C + et);
+    g = a.nnz_off[lo] + before;
+    if (i == i_first) range[0] = g;
+    if (i == i_last) range[1] = g + C + et;
+  } else {
+    const int64_t tr = i - n_cols;
+    if (tr < a.n_traj && a.offsets[tr + 1] == a.offsets[tr]) a.colptr[D * a.offsets[tr] + tr] = 0;
   }
-  double* cv = a.c + a.row_off[lo];
-  if (d == 0)
-    for (int c = 0; c < C; ++c) cv[t * C + c] = (double)a.vals[(int64_t)c * a.N + n];
-  if (box) {
-    const int64_t m0 = C * T;
-    int k = C;
-    if (t > 0) {
-      const int64_t r = m0 + (t - 1) * 2 * D;
-      ri[k] = (int32_t)(r + d);
-      va[k] = -1.0;
-      ri[k + 1] = (int32_t)(r + D + d);
-      va[k + 1] = 1.0;
-      k += 2;
+  __syncthreads();
+  if (col) {
+    // the column's entries in LDS at their offset from the workgroup's first
+    const int64_t o = g - range[0];
+    for (int c = 0; c < C; ++c) {
+      sr[o + c] = (int32_t)(t * C + c);
+      sv[o + c] = (double)a.jac[((int64_t)c * D + d) * a.N + n] + 1e-12;
     }
-    if (t < T - 1) {
-      const int64_t r = m0 + t * 2 * D;
-      ri[k] = (int32_t)(r + d);
-      va[k] = 1.0;
-      ri[k + 1] = (int32_t)(r + D + d);
-      va[k + 1] = -1.0;
-      const double q1 = a.X[(int64_t)d * a.ldx + n], q2 = a.X[(int64_t)d * a.ldx + n + 1], w = a.msbox[d];
-      cv[r + d] = q1 - q2 + w;
-      cv[r + D + d] = q2 - q1 + w;
+    double* cv = a.c + a.row_off[lo];
+    if (d == 0)
+      for (int c = 0; c < C; ++c) cv[t * C + c] = (double)a.vals[(int64_t)c * a.N + n];
+    if (box) {
+      const int64_t m0 = C * T;
+      int64_t k = o + C;
+      if (t > 0) {
+        const int64_t r = m0 + (t - 1) * 2 * D;
+        sr[k] = (int32_t)(r + d);
+        sv[k] = -1.0;
+        sr[k + 1] = (int32_t)(r + D + d);
+        sv[k + 1] = 1.0;
+        k += 2;
+      }
+      if (t < T - 1) {
+        const int64_t r = m0 + t * 2 * D;
+        sr[k] = (int32_t)(r + d);
+        sv[k] = 1.0;
+        sr[k + 1] = (int32_t)(r + D + d);
+        sv[k + 1] = -1.0;
+        const double q1 = a.X[(int64_t)d * a.ldx + n], q2 = a.X[(int64_t)d * a.ldx + n + 1], w = a.msbox[d];
+        cv[r + d] = q1 - q2 + w;
+        cv[r + D + d] = q2 - q1 + w;
+      }
     }
+  }
+  __syncthreads();
+  if (i_first >= n_cols) return;  // a workgroup of threads past the columns (uniform)
+  const int64_t base = range[0], cnt = range[1] - range[0];
+  for (int64_t e = threadIdx.x; e < cnt; e += kSqpBlock) {
+    a.values[base + e] = sv[e];
+    a.rowind[base + e] = sr[e];
   }
 }
 
@@ -2212,7 +2242,9 @@ int edges_resident_waves_per_cu(bool cloud, int block, size_t lds) {
 
 hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st) {
   const int64_t n = a.N * a.D + a.n_traj;
-  if (n > 0) hipLaunchKernelGGL(sqp_ineq_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, a);
+  if (a.C < 0 || a.C > 2) return hipErrorInvalidValue;  // a column holds at most C + 4 <= 6 entries
+  const size_t lds = (size_t)kSqpBlock * (a.C + 4) * (sizeof(double) + sizeof(int32_t));
+  if (n > 0) hipLaunchKernelGGL(sqp_ineq_kernel, dim3(blocks_for(n, kSqpBlock)), dim3(kSqpBlock), lds, st, a);
   return hipGetLastError();
 }
 
