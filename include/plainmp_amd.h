@@ -142,6 +142,9 @@ typedef struct pmp_schedule {
 } pmp_schedule;
 
 typedef struct pmp_context pmp_context;
+/* A captured HIP graph of one of the *_graph entry points below, replayed by
+ * pmp_graph_launch and freed by pmp_graph_destroy. */
+typedef struct pmp_graph pmp_graph;
 
 const char* pmp_last_error(void);
 /* 2: pmp_schedule gained pair_shares (a caller built against version 1
@@ -208,6 +211,19 @@ typedef struct pmp_ik_step_desc {
 } pmp_ik_step_desc;
 pmp_status pmp_ik_step_batch(pmp_context* ctx, double* q, int64_t R, const double* poses,
                              const pmp_ik_step_desc* step);
+
+/* n_iter whole iterations of that IK, captured once into a HIP graph and
+ * replayed by pmp_graph_launch: per iteration the link poses into `poses`
+ * (as pmp_link_poses_batch, exact FK, ldq = R), the link Jacobians into
+ * step->jac0 and, for kind 6 / 7, step->jac_u[k] (as pmp_link_jacobian_batch
+ * at the link origin and at its unit axes e_k), the collision rows into
+ * step->vals / step->cjac when n_cst > 0 (as pmp_evaluate_batch), then the
+ * step on q.  Those buffers are written (device memory the caller owns, valid
+ * while the graph exists); the context rules of pmp_check_edges_graph apply.
+ * Replaces the host loop over iterations of the batched IK (the reference's
+ * counterpart is the SLSQP loop of solve_ik, src/plainmp/ik.py:42-156). */
+pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, double* poses, const pmp_ik_step_desc* step,
+                        int32_t n_iter, pmp_graph** out);
 
 /* Link poses of all nodes, poses[(n*7 + k) * B + b] as x,y,z,qx,qy,qz,qw
  * (FP64).  Replaces set_joint_angles + get_link_pose
@@ -317,7 +333,6 @@ pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const doubl
  * capture: after pmp_upload_robot, pmp_upload_scene or pmp_set_band (e.g. a
  * set_sdf of the host mirror) pmp_graph_launch returns PMP_ERR_INVALID and
  * the graph has to be captured again. */
-typedef struct pmp_graph pmp_graph;
 pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const double* s2, int64_t lds, int64_t E,
                                  const pmp_validator_desc* validator, uint8_t* valid, int32_t* n_checked,
                                  pmp_graph** out);

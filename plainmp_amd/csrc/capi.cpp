@@ -1042,40 +1042,105 @@ pmp_status pmp_link_jacobian_batch(pmp_context* ctx, const double* q, int64_t ld
   });
 }
 
+}  // extern "C"
+
+namespace {
+// Argument checks of an IK step (R > 0) and its kernel arguments.
+IkArgs ik_args(pmp_context* ctx, double* q, int64_t R, const double* poses, const pmp_ik_step_desc* d) {
+  PMP_REQUIRE(q && poses && d->target && d->jac0 && d->lo && d->hi, "null argument");
+  PMP_REQUIRE(d->kind == 3 || (d->jac_u[0] && d->jac_u[1] && d->jac_u[2]), "rotation targets need jac_u");
+  PMP_REQUIRE(d->n_cst == 0 || (d->vals && d->cjac), "null collision rows");
+  IkArgs a{};
+  a.R = R;
+  a.D = ctx->n_cols;
+  a.C = d->n_cst;
+  a.kind = d->kind;
+  a.link = d->link;
+  a.poses = poses;
+  a.tgt = d->target;
+  a.J0 = d->jac0;
+  for (int k = 0; k < 3; ++k) a.Ju[k] = d->jac_u[k];
+  a.vals = d->vals;
+  a.cjac = d->cjac;
+  a.lo = d->lo;
+  a.hi = d->hi;
+  a.damping = d->damping;
+  a.max_step = d->max_step;
+  a.margin = d->margin;
+  a.wpen = std::sqrt(d->penalty);
+  a.q = q;
+  return a;
+}
+void ik_checks(pmp_context* ctx, int64_t R, const pmp_ik_step_desc* d) {
+  require_robot(ctx);
+  PMP_REQUIRE(d != nullptr && R >= 0, "null step descriptor or negative batch");
+  PMP_REQUIRE(d->kind == 3 || d->kind == 6 || d->kind == 7, "All poses must be 3 or 6 or 7 dimensional");
+  PMP_REQUIRE(ctx->n_cols <= 16, "the batched IK step handles at most 16 q columns");
+  PMP_REQUIRE(d->link >= 0 && d->link < ctx->n_nodes, "link index out of range");
+  PMP_REQUIRE(d->n_cst >= 0 && d->n_cst <= 2, "n_cst out of range");
+}
+}  // namespace
+
+extern "C" {
+
 pmp_status pmp_ik_step_batch(pmp_context* ctx, double* q, int64_t R, const double* poses,
                              const pmp_ik_step_desc* d) {
   return guarded([&] {
-    require_robot(ctx);
-    PMP_REQUIRE(d != nullptr && R >= 0, "null step descriptor or negative batch");
-    PMP_REQUIRE(d->kind == 3 || d->kind == 6 || d->kind == 7, "All poses must be 3 or 6 or 7 dimensional");
-    PMP_REQUIRE(ctx->n_cols <= 16, "the batched IK step handles at most 16 q columns");
-    PMP_REQUIRE(d->link >= 0 && d->link < ctx->n_nodes, "link index out of range");
-    PMP_REQUIRE(d->n_cst >= 0 && d->n_cst <= 2, "n_cst out of range");
+    ik_checks(ctx, R, d);
     if (R == 0) return;
-    PMP_REQUIRE(q && poses && d->target && d->jac0 && d->lo && d->hi, "null argument");
-    PMP_REQUIRE(d->kind == 3 || (d->jac_u[0] && d->jac_u[1] && d->jac_u[2]), "rotation targets need jac_u");
-    PMP_REQUIRE(d->n_cst == 0 || (d->vals && d->cjac), "null collision rows");
+    const IkArgs a = ik_args(ctx, q, R, poses, d);
     set_device(ctx);
-    IkArgs a{};
-    a.R = R;
-    a.D = ctx->n_cols;
-    a.C = d->n_cst;
-    a.kind = d->kind;
-    a.link = d->link;
-    a.poses = poses;
-    a.tgt = d->target;
-    a.J0 = d->jac0;
-    for (int k = 0; k < 3; ++k) a.Ju[k] = d->jac_u[k];
-    a.vals = d->vals;
-    a.cjac = d->cjac;
-    a.lo = d->lo;
-    a.hi = d->hi;
-    a.damping = d->damping;
-    a.max_step = d->max_step;
-    a.margin = d->margin;
-    a.wpen = std::sqrt(d->penalty);
-    a.q = q;
     PMP_HIP(launch_ik_step(a, st(ctx)));
+  });
+}
+
+pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, double* poses, const pmp_ik_step_desc* d,
+                        int32_t n_iter, pmp_graph** out) {
+  return guarded([&] {
+    PMP_REQUIRE(out != nullptr, "null output pointer");
+    ik_checks(ctx, R, d);
+    PMP_REQUIRE(R > 0 && n_iter > 0, "graph of an empty IK batch");
+    const IkArgs a = ik_args(ctx, q, R, poses, d);
+    set_device(ctx);
+    auto* g = new pmp_graph();
+    try {
+      hipStream_t cs = ctx->stream;
+      if (!cs) {
+        if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        cs = ctx->capture_stream;
+      }
+      g->stream = cs;
+      EvalArgs e{};
+      e.vals = const_cast<float*>(d->vals);
+      e.jac = const_cast<float*>(d->cjac);
+      e.whole_chunks = ctx->sched.eval_whole_chunks;
+      PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      hipError_t le = hipSuccess;
+      for (int it = 0; it < n_iter && le == hipSuccess; ++it) {
+        le = launch_poses(ctx->k64, q, R, R, 0, poses, cs);
+        const float origin[3] = {0.f, 0.f, 0.f};
+        if (le == hipSuccess)
+          le = launch_link_jacobian(ctx->k32, q, R, R, d->link, origin, const_cast<float*>(d->jac0), cs);
+        for (int k = 0; k < 3 && d->kind > 3 && le == hipSuccess; ++k) {
+          const float unit[3] = {k == 0 ? 1.f : 0.f, k == 1 ? 1.f : 0.f, k == 2 ? 1.f : 0.f};
+          le = launch_link_jacobian(ctx->k32, q, R, R, d->link, unit, const_cast<float*>(d->jac_u[k]), cs);
+        }
+        if (d->n_cst > 0 && le == hipSuccess) le = launch_eval_units(ctx->k32, ctx->s32, q, R, R, e, cs);
+        if (le == hipSuccess) le = launch_ik_step(a, cs);
+      }
+      hipError_t ce = hipStreamEndCapture(cs, &g->graph);
+      PMP_HIP(le);
+      PMP_HIP(ce);
+      PMP_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+      g->ctx = ctx;
+      g->tables_gen = ctx->tables_gen;
+    } catch (...) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+      throw;
+    }
+    *out = g;
   });
 }
 

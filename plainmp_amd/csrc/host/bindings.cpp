@@ -664,6 +664,34 @@ PYBIND11_MODULE(_native, m) {
                                           (float*)cost, (float*)grad, &g->g));
            return g;
          });
+  // n_iter iterations of the batched IK (poses, link Jacobians, collision
+  // rows, step) captured once and replayed by launch()
+  mo.def("_dev_capture_ik_graph",
+         [](std::shared_ptr<SphereCollisionCst> c, uintptr_t q, int64_t R, uintptr_t poses, int kind, int link,
+            uintptr_t target, uintptr_t jac0, const std::vector<uintptr_t>& jac_u, uintptr_t vals, uintptr_t cjac,
+            int n_cst, uintptr_t lo, uintptr_t hi, double damping, double max_step, double margin, double penalty,
+            int n_iter, uintptr_t stream) {
+           pmp_context* ctx = c->device_context((void*)stream);
+           pmp_ik_step_desc d{};
+           d.kind = kind;
+           d.link = link;
+           d.target = (const double*)target;
+           d.jac0 = (const float*)jac0;
+           for (size_t k = 0; k < 3 && k < jac_u.size(); ++k) d.jac_u[k] = (const float*)jac_u[k];
+           d.vals = (const float*)vals;
+           d.cjac = (const float*)cjac;
+           d.n_cst = n_cst;
+           d.lo = (const double*)lo;
+           d.hi = (const double*)hi;
+           d.damping = damping;
+           d.max_step = max_step;
+           d.margin = margin;
+           d.penalty = penalty;
+           auto g = std::make_shared<Graph>();
+           g->cst = c;
+           check(pmp_ik_graph(ctx, (double*)q, R, (double*)poses, &d, n_iter, &g->g));
+           return g;
+         });
   py::class_<MotionValidator, std::shared_ptr<MotionValidator>>(mo, "MotionValidator")
       .def(py::init<std::shared_ptr<SphereCollisionCst>, const ValidatorConfig&>())
       .def("checkMotion", [](MotionValidator& v, const DArr& s1, const DArr& s2) {

@@ -158,22 +158,25 @@ def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[
             e = torch.cat([e, quat - tgt[3:]], dim=1)
         return e, quat, rpy
 
-    if D <= 16:
+    if D <= 16 and cfg.n_iter > 0:
         # fused: per iteration the pose, Jacobian and evaluate launches and one
-        # step kernel (the normal equations solved per restart in registers)
+        # step kernel (the normal equations solved per restart in registers),
+        # one iteration captured into a HIP graph and replayed n_iter times
+        from ._native.ompl import _dev_capture_ik_graph
+
         qs = Q.T.contiguous()
-        pen = float(cfg.penalty)
         C_used = C if use_col else 0
+        graph = _dev_capture_ik_graph(cst, qs.data_ptr(), R, poses.data_ptr(), kind, link, tgt.data_ptr(),
+                                      J.data_ptr(), [u.data_ptr() for u in Ju], vals.data_ptr(), cjac.data_ptr(),
+                                      C_used, lo.data_ptr(), hi.data_ptr(), cfg.damping, cfg.max_step, cfg.margin,
+                                      float(cfg.penalty), 1, sh)
+        # the graph replays on the context's stream, or on a stream of its own
+        # when the caller's is the null stream: the inputs are made ready
+        # first, and destroy() waits for the replays
+        stream.synchronize()
         for _ in range(cfg.n_iter):
-            cst._dev_link_poses(qs.data_ptr(), R, R, 1, poses.data_ptr(), sh)
-            cst._dev_link_jacobian(qs.data_ptr(), R, R, link, [], J.data_ptr(), sh)
-            for k in range(len(Ju)):
-                cst._dev_link_jacobian(qs.data_ptr(), R, R, link, units[k], Ju[k].data_ptr(), sh)
-            if use_col:
-                cst._dev_evaluate(qs.data_ptr(), R, R, vals.data_ptr(), cjac.data_ptr(), sh)
-            cst._dev_ik_step(qs.data_ptr(), R, poses.data_ptr(), kind, link, tgt.data_ptr(), J.data_ptr(),
-                             [u.data_ptr() for u in Ju], vals.data_ptr(), cjac.data_ptr(), C_used, lo.data_ptr(),
-                             hi.data_ptr(), cfg.damping, cfg.max_step, cfg.margin, pen, sh)
+            graph.launch()
+        graph.destroy()
         Q = qs.T.contiguous()
     for _ in range(cfg.n_iter if D > 16 else 0):
         qs = Q.T.contiguous()

@@ -167,3 +167,61 @@ def test_traj_graph_replays_match_direct_call():
     vref, _ = oc.evaluate_batch(X, 8)
     assert np.abs(direct[0].T - vref).max() <= 5e-6
     _report("traj_graph", N=N, T=T)
+
+
+@pytest.mark.parametrize("kind", [3, 7])
+def test_ik_graph_matches_direct_iterations(kind):
+    """Batched IK iterations captured into one graph (poses, link Jacobians,
+    collision rows and the step, n_iter times) give bit for bit the
+    configurations of the same iterations launched one call at a time, on
+    two replays from the same start."""
+    import torch
+
+    from plainmp_amd._native.ompl import _dev_capture_ik_graph
+
+    wl = synth.workload("plain_two_bars", scale=1 / 16)
+    pc = product_cst(wl)
+    kin = pc.get_kin()
+    link = int(kin.get_link_ids([wl.robot.sphere_specs[-1][0]])[0])
+    n_nodes, D, C, R, n_iter = int(kin.n_links), pc.q_dim(), pc.cst_dim(), 96, 7
+    rng = np.random.default_rng(3)
+    lb, ub = np.asarray(wl.robot.lower, float), np.asarray(wl.robot.upper, float)
+    Q0 = lb + (ub - lb) * rng.random((R, D))
+    target = [0.3, 0.2, 0.5] + ([0.0, 0.0, 0.0, 1.0] if kind == 7 else [])
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    with torch.cuda.stream(stream):
+        q = torch.from_numpy(np.ascontiguousarray(Q0.T)).cuda()
+        lo = torch.from_numpy(lb).cuda()
+        hi = torch.from_numpy(ub).cuda()
+        tgt = torch.tensor(target, dtype=torch.float64, device="cuda")
+        poses = torch.empty((n_nodes * 7, R), dtype=torch.float64, device="cuda")
+        J = torch.empty((3, D, R), dtype=torch.float32, device="cuda")
+        Ju = [torch.empty((3, D, R), dtype=torch.float32, device="cuda") for _ in range(3 if kind > 3 else 0)]
+        vals = torch.empty((C, R), dtype=torch.float32, device="cuda")
+        cjac = torch.empty((C, D, R), dtype=torch.float32, device="cuda")
+    stream.synchronize()
+    units = ([1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0])
+    args = (kind, link, tgt.data_ptr())
+    tail = (C, lo.data_ptr(), hi.data_ptr(), 1e-3, 0.2, 0.02, 100.0)
+    for _ in range(n_iter):
+        pc._dev_link_poses(q.data_ptr(), R, R, 1, poses.data_ptr(), sh)
+        pc._dev_link_jacobian(q.data_ptr(), R, R, link, [], J.data_ptr(), sh)
+        for k in range(len(Ju)):
+            pc._dev_link_jacobian(q.data_ptr(), R, R, link, units[k], Ju[k].data_ptr(), sh)
+        pc._dev_evaluate(q.data_ptr(), R, R, vals.data_ptr(), cjac.data_ptr(), sh)
+        pc._dev_ik_step(q.data_ptr(), R, poses.data_ptr(), *args, J.data_ptr(), [u.data_ptr() for u in Ju],
+                        vals.data_ptr(), cjac.data_ptr(), *tail, sh)
+    stream.synchronize()
+    direct = q.cpu().numpy().copy()
+    assert not np.array_equal(direct, Q0.T)  # the iterations moved the restarts
+    g = _dev_capture_ik_graph(pc, q.data_ptr(), R, poses.data_ptr(), *args, J.data_ptr(),
+                              [u.data_ptr() for u in Ju], vals.data_ptr(), cjac.data_ptr(), *tail, n_iter, sh)
+    for rep in range(2):
+        with torch.cuda.stream(stream):
+            q.copy_(torch.from_numpy(np.ascontiguousarray(Q0.T)))
+        g.launch()
+        stream.synchronize()
+        assert np.array_equal(q.cpu().numpy(), direct), rep
+    g.destroy()
+    _report("ik_graph", kind=kind, R=R, n_iter=n_iter)
