@@ -327,7 +327,8 @@ pmp_status pmp_check_edges_batch(pmp_context* ctx, const double* s1, const doubl
                                  const pmp_validator_desc* validator, uint8_t* valid, int32_t* n_checked);
 
 /* Same as pmp_check_edges_batch, captured once into a HIP graph and replayed
- * by pmp_graph_launch on the context stream.  The pointers must stay valid
+ * by pmp_graph_launch on the context stream (with none set, on a stream of
+ * the context's own that the null stream synchronizes with).  The pointers must stay valid
  * while the graph exists, and the graph must be destroyed before its context.
  * The graph reads the context's robot, scene and cull tables as they were at
  * capture: after pmp_upload_robot, pmp_upload_scene or pmp_set_band (e.g. a

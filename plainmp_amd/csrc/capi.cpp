@@ -224,6 +224,17 @@ void set_device(pmp_context* ctx) {
 }
 hipStream_t st(pmp_context* ctx) { return ctx->stream; }
 
+// The stream a graph is captured on and replayed on: the context stream, or
+// with none set (the null stream) a stream of the context's own created
+// without hipStreamNonBlocking, which the null stream synchronizes with, so
+// that replays are ordered with the caller's null-stream work as direct calls
+// are (the null stream itself cannot be captured).
+hipStream_t graph_stream(pmp_context* ctx) {
+  if (ctx->stream) return ctx->stream;
+  if (!ctx->capture_stream) PMP_HIP(hipStreamCreate(&ctx->capture_stream));
+  return ctx->capture_stream;
+}
+
 // FP64 quaternion product (Eigen order) and q * v (Eigen's formula, which for
 // a quaternion s q' with unit q' gives (1 - s^2) v + s^2 R(q') v).
 void hq_mul(const double* a, const double* b, double* o) {
@@ -1104,11 +1115,7 @@ pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, double* poses, c
     set_device(ctx);
     auto* g = new pmp_graph();
     try {
-      hipStream_t cs = ctx->stream;
-      if (!cs) {
-        if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
-        cs = ctx->capture_stream;
-      }
+      hipStream_t cs = graph_stream(ctx);
       g->stream = cs;
       EvalArgs e{};
       e.vals = const_cast<float*>(d->vals);
@@ -1289,11 +1296,7 @@ pmp_status pmp_traj_cost_grad_graph(pmp_context* ctx, const double* X, int64_t l
       g->traj_off.upload(off, st(ctx));
       g->traj_w2.upload(w2, st(ctx));
       PMP_HIP(hipStreamSynchronize(st(ctx)));
-      hipStream_t cs = ctx->stream;
-      if (!cs) {
-        if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
-        cs = ctx->capture_stream;
-      }
+      hipStream_t cs = graph_stream(ctx);
       g->stream = cs;
       EvalArgs e{};
       e.vals = vals;
@@ -1575,11 +1578,7 @@ pmp_status pmp_check_edges_graph(pmp_context* ctx, const double* s1, const doubl
       EdgeArgs ea{};
       prepare_edges(ctx, s1, s2, lds, E, v, valid, n_checked, g->eb, ea);
       PMP_HIP(hipStreamSynchronize(st(ctx)));  // width tables are in place before capture
-      hipStream_t cs = ctx->stream;
-      if (!cs) {
-        if (!ctx->capture_stream) PMP_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
-        cs = ctx->capture_stream;
-      }
+      hipStream_t cs = graph_stream(ctx);
       const int block = slot_block(ctx->k64, ctx->sched.block > 0 ? ctx->sched.block : 256);
       g->stream = cs;
       PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));

@@ -47,6 +47,10 @@ hipError_t hipMemcpyAsync(void* dst, const void* src, size_t n, hipMemcpyKind, h
   return hipSuccess;
 }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamCreate(hipStream_t* s) {
+  *s = reinterpret_cast<hipStream_t>(std::malloc(16));
+  return hipSuccess;
+}
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {
   *s = reinterpret_cast<hipStream_t>(std::malloc(16));
   return hipSuccess;

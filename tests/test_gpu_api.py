@@ -225,3 +225,35 @@ def test_ik_graph_matches_direct_iterations(kind):
         assert np.array_equal(q.cpu().numpy(), direct), rep
     g.destroy()
     _report("ik_graph", kind=kind, R=R, n_iter=n_iter)
+
+
+def test_graph_on_the_null_stream_orders_with_it():
+    """A graph captured with no context stream (the caller's null stream)
+    replays in order with the caller's null-stream work: outputs cleared on
+    the null stream before each replay and read there after it hold the
+    direct call's values, with no synchronization of any other stream."""
+    import torch
+
+    from plainmp_amd.constraint import capture_traj_graph
+
+    wl = synth.workload("traj_dual_arm", scale=0.01)
+    pc = product_cst(wl)
+    off, X = [int(o) for o in wl.traj_offsets], wl.traj_X
+    N, D, C, T = len(X), pc.q_dim(), pc.cst_dim(), len(off) - 1
+    w = list(np.linspace(0.5, 2.0, D))
+    x = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    outs = [torch.empty(s, dtype=torch.float32, device="cuda") for s in ((C, N), (C, D, N), (T,), (D, N))]
+    ptrs = [o.data_ptr() for o in outs]
+    torch.cuda.synchronize()
+    pc._dev_traj_cost_grad(x.data_ptr(), N, N, off, w, *ptrs, 0)
+    torch.cuda.synchronize()
+    direct = [o.cpu().numpy().copy() for o in outs]
+    g = capture_traj_graph(pc, x.data_ptr(), N, N, off, w, *ptrs, 0)
+    for rep in range(3):
+        for o in outs:
+            o.fill_(float("nan"))
+        g.launch()
+        for o, d in zip(outs, direct):
+            assert np.array_equal(o.cpu().numpy(), d), rep
+    g.destroy()
+    _report("null_stream_graph", N=N, T=T)
