@@ -782,6 +782,51 @@ def test_autotuned_schedules_keep_results():
     assert np.array_equal(got, ref.astype(bool)) and np.array_equal(n_got, n_ref)
 
 
+def test_pair_shares_with_a_cloud_and_primitives():
+    """A scene of eight primitives and a 4,096-point cloud: the pair
+    distances and gradients are the same, bit for bit, under every primitive
+    split (the cloud falls in one share), the distances match the oracle's,
+    and the adjoint (split over primitive shares too) matches the
+    contraction of the full Jacobian tensor."""
+    import torch
+
+    from plainmp_amd import autotune
+    from plainmp_amd.autograd import pair_clearance
+
+    base = synth.workload("cloud_shelves", scale=128 / 16384)
+    wl = synth.Workload("cloud_and_prims", base.robot, synth.mixed_primitives(8, seed=12),
+                        cloud=synth.shelf_cloud(4096, seed=13), Q=base.Q)
+    oc, pc = oracle_cst(wl), product_cst(wl)
+    B, S, P, D = len(wl.Q), pc.n_spheres(), pc.n_prims(), pc.q_dim()
+    assert P == 9
+    q = torch.from_numpy(np.ascontiguousarray(wl.Q.T)).cuda()
+    sh = torch.cuda.current_stream().cuda_stream
+    outs = []
+    for c in (1,) + autotune.PAIR_SHARES:
+        dist = torch.full((S, P, B), float("nan"), dtype=torch.float32, device="cuda")
+        grad = torch.full((S, P, D, B), float("nan"), dtype=torch.float32, device="cuda")
+        autotune.Schedule(pair_shares=c).apply(pc)
+        pc._dev_pair_clearance(q.data_ptr(), B, B, dist.data_ptr(), grad.data_ptr(), sh)
+        torch.cuda.synchronize()
+        outs.append((c, dist.cpu().numpy(), grad.cpu().numpy()))
+    for c, d, g in outs[1:]:
+        assert np.array_equal(d, outs[0][1]) and np.array_equal(g, outs[0][2]), c
+    dref = oc.pair_clearance_batch(wl.Q, True, 8)
+    derr = np.abs(np.moveaxis(outs[0][1], -1, 0) - dref).max()
+    assert derr <= 5e-6
+    autotune.Schedule().apply(pc)
+    qt = torch.tensor(wl.Q, device="cuda", dtype=torch.float64, requires_grad=True)
+    d = pair_clearance(pc, qt)
+    w = torch.randn(d.shape, device="cuda", dtype=torch.float32, generator=torch.Generator("cuda").manual_seed(1))
+    (d * w).sum().backward()
+    grad = np.moveaxis(outs[0][2], -1, 0)  # (B, S, P, D)
+    ref = np.einsum("bsp,bspj->bj", w.cpu().numpy().astype(np.float64), grad.astype(np.float64))
+    scale = np.abs(w.cpu().numpy()).sum(axis=(1, 2))[:, None] * np.abs(grad).max()
+    got = qt.grad.cpu().numpy()
+    _report("pair_shares_cloud", B=B, P=P, dist_err=derr, vjp_rel=(np.abs(got - ref) / scale).max())
+    assert (np.abs(got - ref) <= 1e-5 * scale).all()
+
+
 def test_pair_shares_keep_results():
     """The pair-distance launch gives the same distances and gradients,
     bit for bit, under every primitive split the autotuner tries, and
