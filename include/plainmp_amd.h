@@ -213,17 +213,19 @@ pmp_status pmp_ik_step_batch(pmp_context* ctx, double* q, int64_t R, const doubl
                              const pmp_ik_step_desc* step);
 
 /* n_iter whole iterations of that IK, captured once into a HIP graph and
- * replayed by pmp_graph_launch: per iteration the link poses into `poses`
- * (as pmp_link_poses_batch, exact FK, ldq = R), the link Jacobians into
+ * replayed by pmp_graph_launch: per iteration the link Jacobians into
  * step->jac0 and, for kind 6 / 7, step->jac_u[k] (as pmp_link_jacobian_batch
- * at the link origin and at its unit axes e_k), the collision rows into
- * step->vals / step->cjac when n_cst > 0 (as pmp_evaluate_batch), then the
- * step on q.  Those buffers are written (device memory the caller owns, valid
- * while the graph exists); the context rules of pmp_check_edges_graph apply.
- * Replaces the host loop over iterations of the batched IK (the reference's
- * counterpart is the SLSQP loop of solve_ik, src/plainmp/ik.py:42-156). */
-pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, double* poses, const pmp_ik_step_desc* step,
-                        int32_t n_iter, pmp_graph** out);
+ * at the link origin and at its unit axes e_k, ldq = R), the collision rows
+ * into step->vals / step->cjac when n_cst > 0 (as pmp_evaluate_batch), then
+ * the step on q.  The step reads the link pose from the FP32 walk of the
+ * first Jacobian launch (into a buffer of the graph's own) instead of FP64
+ * link poses.  Those buffers are written (device memory the caller owns,
+ * valid while the graph exists); the context rules of pmp_check_edges_graph
+ * apply.  Replaces the host loop over iterations of the batched IK (the
+ * reference's counterpart is the SLSQP loop of solve_ik,
+ * src/plainmp/ik.py:42-156). */
+pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, const pmp_ik_step_desc* step, int32_t n_iter,
+                        pmp_graph** out);
 
 /* Link poses of all nodes, poses[(n*7 + k) * B + b] as x,y,z,qx,qy,qz,qw
  * (FP64).  Replaces set_joint_angles + get_link_pose

@@ -201,6 +201,7 @@ struct pmp_graph {
   hipStream_t stream = nullptr;
   EdgeBufs eb;
   DevBuf traj_off, traj_w2;  // a trajectory graph's offsets and squared weights
+  DevBuf ik_pose;            // an IK graph's FP32 link poses
   const pmp_context* ctx = nullptr;
   uint64_t tables_gen = 0;  // the context's tables_gen at capture
 };
@@ -1049,7 +1050,7 @@ pmp_status pmp_link_jacobian_batch(pmp_context* ctx, const double* q, int64_t ld
     if (B == 0) return;
     set_device(ctx);
     const float off[3] = {point ? (float)point[0] : 0.f, point ? (float)point[1] : 0.f, point ? (float)point[2] : 0.f};
-    PMP_HIP(launch_link_jacobian(ctx->k32, q, ldq, B, link, off, jac, st(ctx)));
+    PMP_HIP(launch_link_jacobian(ctx->k32, q, ldq, B, link, off, jac, nullptr, st(ctx)));
   });
 }
 
@@ -1058,7 +1059,7 @@ pmp_status pmp_link_jacobian_batch(pmp_context* ctx, const double* q, int64_t ld
 namespace {
 // Argument checks of an IK step (R > 0) and its kernel arguments.
 IkArgs ik_args(pmp_context* ctx, double* q, int64_t R, const double* poses, const pmp_ik_step_desc* d) {
-  PMP_REQUIRE(q && poses && d->target && d->jac0 && d->lo && d->hi, "null argument");
+  PMP_REQUIRE(q && d->target && d->jac0 && d->lo && d->hi, "null argument");
   PMP_REQUIRE(d->kind == 3 || (d->jac_u[0] && d->jac_u[1] && d->jac_u[2]), "rotation targets need jac_u");
   PMP_REQUIRE(d->n_cst == 0 || (d->vals && d->cjac), "null collision rows");
   IkArgs a{};
@@ -1099,22 +1100,27 @@ pmp_status pmp_ik_step_batch(pmp_context* ctx, double* q, int64_t R, const doubl
   return guarded([&] {
     ik_checks(ctx, R, d);
     if (R == 0) return;
+    PMP_REQUIRE(poses != nullptr, "null argument");
     const IkArgs a = ik_args(ctx, q, R, poses, d);
     set_device(ctx);
     PMP_HIP(launch_ik_step(a, st(ctx)));
   });
 }
 
-pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, double* poses, const pmp_ik_step_desc* d,
-                        int32_t n_iter, pmp_graph** out) {
+pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, const pmp_ik_step_desc* d, int32_t n_iter,
+                        pmp_graph** out) {
   return guarded([&] {
     PMP_REQUIRE(out != nullptr, "null output pointer");
     ik_checks(ctx, R, d);
     PMP_REQUIRE(R > 0 && n_iter > 0, "graph of an empty IK batch");
-    const IkArgs a = ik_args(ctx, q, R, poses, d);
+    IkArgs a = ik_args(ctx, q, R, nullptr, d);
     set_device(ctx);
     auto* g = new pmp_graph();
     try {
+      // the link pose comes from the Jacobian launch's FP32 walk, into the
+      // graph's own buffer
+      float* pose = (float*)g->ik_pose.get(sizeof(float) * 7 * (size_t)R);
+      a.pose32 = pose;
       hipStream_t cs = graph_stream(ctx);
       g->stream = cs;
       EvalArgs e{};
@@ -1124,13 +1130,11 @@ pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, double* poses, c
       PMP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       hipError_t le = hipSuccess;
       for (int it = 0; it < n_iter && le == hipSuccess; ++it) {
-        le = launch_poses(ctx->k64, q, R, R, 0, poses, cs);
         const float origin[3] = {0.f, 0.f, 0.f};
-        if (le == hipSuccess)
-          le = launch_link_jacobian(ctx->k32, q, R, R, d->link, origin, const_cast<float*>(d->jac0), cs);
+        le = launch_link_jacobian(ctx->k32, q, R, R, d->link, origin, const_cast<float*>(d->jac0), pose, cs);
         for (int k = 0; k < 3 && d->kind > 3 && le == hipSuccess; ++k) {
           const float unit[3] = {k == 0 ? 1.f : 0.f, k == 1 ? 1.f : 0.f, k == 2 ? 1.f : 0.f};
-          le = launch_link_jacobian(ctx->k32, q, R, R, d->link, unit, const_cast<float*>(d->jac_u[k]), cs);
+          le = launch_link_jacobian(ctx->k32, q, R, R, d->link, unit, const_cast<float*>(d->jac_u[k]), nullptr, cs);
         }
         if (d->n_cst > 0 && le == hipSuccess) le = launch_eval_units(ctx->k32, ctx->s32, q, R, R, e, cs);
         if (le == hipSuccess) le = launch_ik_step(a, cs);
@@ -1144,6 +1148,7 @@ pmp_status pmp_ik_graph(pmp_context* ctx, double* q, int64_t R, double* poses, c
     } catch (...) {
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
       if (g->graph) (void)hipGraphDestroy(g->graph);
+      g->ik_pose.release();
       delete g;
       throw;
     }
@@ -1619,6 +1624,7 @@ pmp_status pmp_graph_destroy(pmp_graph* g) {
     g->eb.release();
     g->traj_off.release();
     g->traj_w2.release();
+    g->ik_pose.release();
     delete g;
   });
 }

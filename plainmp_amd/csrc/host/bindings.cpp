@@ -667,7 +667,7 @@ PYBIND11_MODULE(_native, m) {
   // n_iter iterations of the batched IK (poses, link Jacobians, collision
   // rows, step) captured once and replayed by launch()
   mo.def("_dev_capture_ik_graph",
-         [](std::shared_ptr<SphereCollisionCst> c, uintptr_t q, int64_t R, uintptr_t poses, int kind, int link,
+         [](std::shared_ptr<SphereCollisionCst> c, uintptr_t q, int64_t R, int kind, int link,
             uintptr_t target, uintptr_t jac0, const std::vector<uintptr_t>& jac_u, uintptr_t vals, uintptr_t cjac,
             int n_cst, uintptr_t lo, uintptr_t hi, double damping, double max_step, double margin, double penalty,
             int n_iter, uintptr_t stream) {
@@ -689,7 +689,7 @@ PYBIND11_MODULE(_native, m) {
            d.penalty = penalty;
            auto g = std::make_shared<Graph>();
            g->cst = c;
-           check(pmp_ik_graph(ctx, (double*)q, R, (double*)poses, &d, n_iter, &g->g));
+           check(pmp_ik_graph(ctx, (double*)q, R, &d, n_iter, &g->g));
            return g;
          });
   py::class_<MotionValidator, std::shared_ptr<MotionValidator>>(mo, "MotionValidator")

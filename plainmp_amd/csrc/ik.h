@@ -11,6 +11,7 @@ struct IkArgs {
   int32_t D, C, kind;    // q columns (<= 16), collision rows, target length (3, 6 or 7)
   int32_t link;          // node index of the target link
   const double* poses;   // [(node * 7 + k) * R + r] link poses (x, y, z, qx, qy, qz, qw), FP64
+  const float* pose32;   // or, when set, [k * R + r] the target link's pose in FP32
   const double* tgt;     // [kind] target
   const float* J0;       // [(k * D + j) * R + r] point Jacobian at the link origin
   const float* Ju[3];    // the same at the link's unit axes (kind > 3)

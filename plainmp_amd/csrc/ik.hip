@@ -41,9 +41,17 @@ __global__ void __launch_bounds__(256) ik_step_kernel(IkArgs a) {
       for (int j = 0; j <= i; ++j) M[i * (i + 1) / 2 + j] += row[i] * row[j];
     }
   };
-  const double* P = a.poses + (int64_t)a.link * 7 * R + r;
-  const double pos[3] = {P[0], P[R], P[2 * R]};
-  const double qx = P[3 * R], qy = P[4 * R], qz = P[5 * R], qw = P[6 * R];
+  double pv[7];
+  if (a.pose32) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) pv[k] = (double)a.pose32[(int64_t)k * R + r];
+  } else {
+    const double* P = a.poses + (int64_t)a.link * 7 * R + r;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) pv[k] = P[(int64_t)k * R];
+  }
+  const double pos[3] = {pv[0], pv[1], pv[2]};
+  const double qx = pv[3], qy = pv[4], qz = pv[5], qw = pv[6];
   for (int k = 0; k < 3; ++k) {
 #pragma unroll
     for (int j = 0; j < MAXD; ++j) row[j] = j < D ? (double)a.J0[((int64_t)k * D + j) * R + r] : 0.0;

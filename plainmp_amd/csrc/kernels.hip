@@ -1819,6 +1819,7 @@ struct LinkJacVis {
   int target;
   V3<float> off;
   V3<float> p{0.f, 0.f, 0.f};
+  Pose<float> lp{};  // the link's pose
   uint64_t mask = 0;
   bool done = false;
   __device__ bool alive() const { return !done; }
@@ -1828,6 +1829,7 @@ struct LinkJacVis {
     if (nd.q_col >= 0) fr.set(nd.q_col, qrot(P.q, V3<float>{nd.axis[0], nd.axis[1], nd.axis[2]}), P.t);
     if (n == target) {
       p = add(qrot(P.q, off), P.t);
+      lp = P;
       mask = nd.col_mask;
       done = true;
     }
@@ -1838,7 +1840,8 @@ struct LinkJacVis {
 
 template <int MAXC>
 __global__ void __launch_bounds__(256) link_jac_kernel(KinArgs<float> ka, const double* q, int64_t ldq, int64_t B,
-                                                       int target, float ox, float oy, float oz, float* jac) {
+                                                       int target, float ox, float oy, float oz, float* jac,
+                                                       float* pose) {
   const int64_t b = gid();
   const bool active = b < B;
   BatchQ qs{q, ldq, active ? b : B - 1};
@@ -1846,6 +1849,11 @@ __global__ void __launch_bounds__(256) link_jac_kernel(KinArgs<float> ka, const 
   run_fk(ka, qs, false, v);
   if (!active) return;
   const int D = ka.n_cols;
+  if (pose) {
+    const float pv[7] = {v.lp.t.x, v.lp.t.y, v.lp.t.z, v.lp.q.x, v.lp.q.y, v.lp.q.z, v.lp.q.w};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) pose[(int64_t)k * B + b] = pv[k];
+  }
 #pragma unroll
   for (int j = 0; j < MAXC; ++j) {
     if (j < D) {
@@ -2223,11 +2231,11 @@ __global__ void __launch_bounds__(kSqpBlock) sqp_ineq_kernel(SqpArgs a) {
 }
 
 hipError_t launch_link_jacobian(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int link,
-                                const float (&off)[3], float* jac, hipStream_t st) {
+                                const float (&off)[3], float* jac, float* pose, hipStream_t st) {
   if (B > 0) {
     const int bk = slot_block(ka, 256);
     PMP_DISPATCH_MAXC(ka.n_cols, link_jac_kernel, dim3(blocks_for(B, bk)), dim3(bk), slot_bytes(ka, bk), st, ka, q,
-                      ldq, B, link, off[0], off[1], off[2], jac);
+                      ldq, B, link, off[0], off[1], off[2], jac, pose);
   }
   return hipGetLastError();
 }

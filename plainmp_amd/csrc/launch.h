@@ -59,8 +59,10 @@ hipError_t launch_sqp_ineq(const SqpArgs& a, hipStream_t st);
 // Waves of the edge kernel variant (cloud or not) one CU holds at once with
 // workgroups of `block` threads (0 when the runtime cannot tell).
 int edges_resident_waves_per_cu(bool cloud, int block, size_t lds);
+// pose (optional): [k * B + b] the link's FP32 pose (x, y, z, qx, qy, qz, qw)
+// from the same walk.
 hipError_t launch_link_jacobian(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int link,
-                                const float (&off)[3], float* jac, hipStream_t st);
+                                const float (&off)[3], float* jac, float* pose, hipStream_t st);
 // p[0, n) = v by a kernel (safe under stream capture, unlike a memset node).
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 hipError_t launch_centers(const KinArgs<float>& ka, const double* q, int64_t ldq, int64_t B, int taylor, float* out,

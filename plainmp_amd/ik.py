@@ -159,14 +159,15 @@ def solve_ik_batched(cst, link_name: str, target: Sequence[float], lb: Sequence[
         return e, quat, rpy
 
     if D <= 16 and cfg.n_iter > 0:
-        # fused: per iteration the pose, Jacobian and evaluate launches and one
-        # step kernel (the normal equations solved per restart in registers),
-        # one iteration captured into a HIP graph and replayed n_iter times
+        # fused: per iteration the Jacobian launches (the first also giving the
+        # link pose), the evaluate launch and one step kernel (the normal
+        # equations solved per restart in registers), one iteration captured
+        # into a HIP graph and replayed n_iter times
         from ._native.ompl import _dev_capture_ik_graph
 
         qs = Q.T.contiguous()
         C_used = C if use_col else 0
-        graph = _dev_capture_ik_graph(cst, qs.data_ptr(), R, poses.data_ptr(), kind, link, tgt.data_ptr(),
+        graph = _dev_capture_ik_graph(cst, qs.data_ptr(), R, kind, link, tgt.data_ptr(),
                                       J.data_ptr(), [u.data_ptr() for u in Ju], vals.data_ptr(), cjac.data_ptr(),
                                       C_used, lo.data_ptr(), hi.data_ptr(), cfg.damping, cfg.max_step, cfg.margin,
                                       float(cfg.penalty), 1, sh)

@@ -164,7 +164,7 @@ hipError_t launch_eval_units(const KinArgs<float>&, const SceneArgs<float>&, con
   return hipSuccess;
 }
 hipError_t launch_link_jacobian(const KinArgs<float>&, const double*, int64_t, int64_t, int, const float (&)[3],
-                                float*, hipStream_t) {
+                                float*, float*, hipStream_t) {
   return hipSuccess;
 }
 hipError_t launch_nearest(const double*, int64_t, const double*, int64_t, double*, int32_t*, hipStream_t) {

@@ -171,10 +171,11 @@ def test_traj_graph_replays_match_direct_call():
 
 @pytest.mark.parametrize("kind", [3, 7])
 def test_ik_graph_matches_direct_iterations(kind):
-    """Batched IK iterations captured into one graph (poses, link Jacobians,
-    collision rows and the step, n_iter times) give bit for bit the
-    configurations of the same iterations launched one call at a time, on
-    two replays from the same start."""
+    """Batched IK iterations captured into one graph (link Jacobians with the
+    FP32 link pose, collision rows and the step, n_iter times) give the same
+    configurations on two replays from the same start, bit for bit, and
+    those of the same iterations launched one call at a time with FP64 link
+    poses within 1e-4 rad."""
     import torch
 
     from plainmp_amd._native.ompl import _dev_capture_ik_graph
@@ -215,16 +216,20 @@ def test_ik_graph_matches_direct_iterations(kind):
     stream.synchronize()
     direct = q.cpu().numpy().copy()
     assert not np.array_equal(direct, Q0.T)  # the iterations moved the restarts
-    g = _dev_capture_ik_graph(pc, q.data_ptr(), R, poses.data_ptr(), *args, J.data_ptr(),
-                              [u.data_ptr() for u in Ju], vals.data_ptr(), cjac.data_ptr(), *tail, n_iter, sh)
+    g = _dev_capture_ik_graph(pc, q.data_ptr(), R, *args, J.data_ptr(), [u.data_ptr() for u in Ju],
+                              vals.data_ptr(), cjac.data_ptr(), *tail, n_iter, sh)
+    replays = []
     for rep in range(2):
         with torch.cuda.stream(stream):
             q.copy_(torch.from_numpy(np.ascontiguousarray(Q0.T)))
         g.launch()
         stream.synchronize()
-        assert np.array_equal(q.cpu().numpy(), direct), rep
+        replays.append(q.cpu().numpy().copy())
     g.destroy()
-    _report("ik_graph", kind=kind, R=R, n_iter=n_iter)
+    assert np.array_equal(replays[0], replays[1])
+    err = np.abs(replays[0] - direct).max()
+    _report("ik_graph", kind=kind, R=R, n_iter=n_iter, max_diff_vs_fp64_poses=err)
+    assert err <= 1e-4
 
 
 def test_graph_on_the_null_stream_orders_with_it():
